@@ -1,0 +1,100 @@
+"""In-tree build of the native libraries (no cmake/ninja).
+
+  lib/libgame_host.so      host C: grid generator and initial-state builder
+  lib/libgame_rkhevi.so    HIP (gfx950): the device RKHEVI step and its C ABI
+  lib/libgame_dropin_r{R}l{L}m{M}.so   struct-layout drop-in `manage_rkhevi`
+                           (include/game_abi.h) for one compile-time layout
+
+Every call compiles (the build check must compile each time).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HOST_FLAGS = ["-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-std=gnu99", "-Wall"]
+# packed fp32 instructions are disabled for all device code; FP64 kernels keep
+# the reference's operation order, so no contraction into FMA either.
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+
+# drop-in layouts built by default: the reference's compiled-in one
+# (src/game_types.h: RES_ID 5, 26 layers, moist) and the measured workloads
+DROPIN_LAYOUTS = [(5, 26, 1), (4, 26, 0), (6, 26, 0)]
+
+
+def run(cmd, cwd=None):
+    print("+ " + " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.stdout.strip():
+        print(r.stdout[-6000:], flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_host():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libgame_host.so")
+    run(["gcc", *HOST_FLAGS, "-shared", "-o", out + ".tmp",
+         os.path.join(CSRC, "gridgen.c"), os.path.join(CSRC, "init_state.c"), "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_gpu():
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    for src in ("rkhevi_fp64.hip", "rkhevi_mixed.hip", "rkhevi_runtime.hip"):
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(LIB, src.replace(".hip", ".o"))
+        run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+        objs.append(obj)
+    out = os.path.join(LIB, "libgame_rkhevi.so")
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
+    os.replace(out + ".tmp", out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_dropins():
+    outs = []
+    for (r, l, m) in DROPIN_LAYOUTS:
+        out = os.path.join(LIB, f"libgame_dropin_r{r}l{l}m{m}.so")
+        run(["gcc", "-O2", "-fPIC", "-std=gnu99", "-Wall", "-shared",
+             f"-DGAME_RES_ID={r}", f"-DGAME_NO_OF_LAYERS={l}", f"-DGAME_MOISTURE_ON={m}",
+             "-o", out + ".tmp", os.path.join(CSRC, "dropin_shim.c"),
+             "-L" + LIB, "-lgame_rkhevi", "-Wl,-rpath,$ORIGIN"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
+def build_oracle():
+    """The CPU oracle (test infrastructure) and, where the reference snapshot
+    exists, its own buildable sources into oracle/_ref."""
+    odir = os.path.join(ROOT, "oracle")
+    run(["make", "-C", odir, "-B", "all"])
+    if os.path.isdir("/root/reference/src"):
+        run(["make", "-C", odir, "-j8", "ref"])
+
+
+def build_all(gpu=True):
+    build_host()
+    if gpu:
+        build_gpu()
+        if os.path.exists(os.path.join(CSRC, "dropin_shim.c")):
+            build_dropins()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all(gpu="--no-gpu" not in sys.argv)

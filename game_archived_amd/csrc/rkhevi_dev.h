@@ -1,0 +1,85 @@
+/*
+ * rkhevi_dev.h -- device-side data of the RKHEVI step (shared by the HIP files).
+ *
+ * Fields keep the reference's element order (see include/game_rkhevi.h), so
+ * one thread per (layer, horizontal index) with the horizontal index fastest
+ * reads and writes whole cache lines; neighbour gathers go through the
+ * reference's own index tables.
+ */
+#ifndef GAME_RKHEVI_DEV_H
+#define GAME_RKHEVI_DEV_H
+
+#include <hip/hip_runtime.h>
+
+#define GAME_N_PENT 12
+#define GAME_MAX_LAYERS 64
+
+/* src/game_constants.h */
+#define GC_T_0 273.15
+#define GC_N_A (6.02214076e23)
+#define GC_P_0 100000.0
+#define GC_C_D_V 717.942189
+#define GC_C_V_V 1396.475121
+#define GC_C_D_P 1005.0
+#define GC_M_D (GC_N_A*0.004810e-23)
+#define GC_M_V (GC_N_A*0.002991e-23)
+#define GC_R_D 287.057811
+#define GC_R_V 461.524879
+
+typedef long long i64;
+
+/* grid tables and metrics (struct grid / dualgrid members) on the device */
+struct DevGrid {
+    int nsh, nvh, nt, vpl, dvpl, L, n_oro, n_const, n_cond, moist;
+    i64 s, v, curl;
+    double radius, toa;
+    const double *nd, *vol, *area, *zs, *zv, *phi, *gm, *slope, *thbg, *exbg, *exbg_grad, *ipw;
+    const double *tw, *drw, *dnd, *dzv, *darea, *fvec;
+    const int *from, *to, *adj, *sgn, *ti, *tmc, *dri, *dfrom, *dto, *vit, *vst;
+    /* per-edge differences that calc_pot_vort forms for its density
+       extrapolation to the top and the surface, read exactly as the
+       reference's struct layout serves them (vorticities.c:48-71) */
+    const double *pv_top_dz1, *pv_top_dz2, *pv_sfc_dz1, *pv_sfc_dz2;
+    /* Klemp damping coefficient of every interior interface, (L-1)*NSH */
+    const double *damp;
+};
+
+struct DevState {
+    double *rho, *rt, *tp, *ep, *wind;   /* rho, rhotheta_v, theta_v_pert, exner_pert, wind */
+};
+
+struct DevWork {
+    /* diagnostics */
+    double *flux, *flux2, *flux_divv, *rvt, *pv, *temp, *vsq;
+    /* forcings */
+    double *pgrad_old, *pgrad_nl, *pgrad_l, *pgc_v, *vsq_grad, *pvt;
+    /* irreversible quantities */
+    double *ptr, *phr, *sed_heat;
+};
+
+struct StepConsts {
+    double dt, impl_weight, damping_start, damping_max, snow_v, rain_v, cloud_v;
+};
+
+__device__ __forceinline__ double density_total(const DevGrid &G, const double *rho, i64 i)
+{   /* derived_quantities.c:83-95: left-to-right sum from 0 */
+    double r = 0.0;
+    for (int k = 0; k < G.n_const; ++k) r += rho[k*G.s + i];
+    return r;
+}
+
+/* per-kernel timing with HIP events on the launch stream (off unless enabled) */
+struct KernelTimer {
+    virtual void begin(const char *name, hipStream_t st) = 0;
+    virtual void end(hipStream_t st) = 0;
+};
+extern KernelTimer *g_kernel_timer;
+
+#define LAUNCH(NAME, KERNEL, GRID, BLOCK, STREAM, ...)                              \
+    do {                                                                            \
+        if (g_kernel_timer) g_kernel_timer->begin(NAME, STREAM);                    \
+        hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);            \
+        if (g_kernel_timer) g_kernel_timer->end(STREAM);                            \
+    } while (0)
+
+#endif

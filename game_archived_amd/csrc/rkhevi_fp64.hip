@@ -1,0 +1,1105 @@
+/*
+ * rkhevi_fp64.hip -- FP64 device kernels of GAME's RKHEVI step for gfx950.
+ *
+ * Each kernel evaluates one or more loops of the reference with the same
+ * floating-point operations in the same order (this file is compiled with
+ * -ffp-contract=off), so that the FP64 path reproduces the CPU reference to
+ * the last bit wherever no transcendental function is involved.  Fusion only
+ * ever recomputes a value with identical arithmetic; no reduction order is
+ * changed.  All kernels are HBM-bandwidth bound gathers/stencils; there is no
+ * dense contraction, hence no MFMA.
+ */
+#include "rkhevi_dev.h"
+#include "rkhevi_kernels.h"
+
+#define TPB 256
+#define GRID1(n) dim3((unsigned)(((n) + TPB - 1)/TPB))
+
+/* ------------------------------------------------------------------ */
+/* small device helpers restating reference functions                   */
+/* ------------------------------------------------------------------ */
+
+/* raw vertical covariant gradient at an interior w point (gradient_operators.c:35-54) */
+__device__ __forceinline__ double grad_w_raw(const DevGrid &G, const double *f, int lev, int h)
+{
+    i64 vi = h + (i64)lev*G.vpl;
+    return (f[h + (i64)(lev - 1)*G.nsh] - f[h + (i64)lev*G.nsh])/G.nd[vi];
+}
+
+/* contravariant horizontal gradient at edge e of layer l (gradient_operators.c:14-33,
+   averaging.c:14-39, 113-133), vertical gradients recomputed on the fly */
+__device__ __forceinline__ double grad_edge(const DevGrid &G, const double *f, int l, int e)
+{
+    int fr = G.from[e], to = G.to[e];
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    double g = (f[to + (i64)l*G.nsh] - f[fr + (i64)l*G.nsh])/G.nd[vi];
+    if (l >= G.L - G.n_oro) {
+        const double *ipw = G.ipw;
+        double c = ipw[8*((i64)l*G.nsh + fr) + 6]*grad_w_raw(G, f, l, fr);
+        c += ipw[8*((i64)l*G.nsh + to) + 6]*grad_w_raw(G, f, l, to);
+        if (l < G.L - 1) {
+            c += ipw[8*((i64)l*G.nsh + fr) + 7]*grad_w_raw(G, f, l + 1, fr);
+            c += ipw[8*((i64)l*G.nsh + to) + 7]*grad_w_raw(G, f, l + 1, to);
+        }
+        c = 0.5*c;
+        g += -G.slope[vi]*c;
+    }
+    return g;
+}
+
+/* averaging.c:14-39 on a stored vector field */
+__device__ __forceinline__ double remap_verpri2horpri(const DevGrid &G, const double *f, int l, int e)
+{
+    int fr = G.from[e], to = G.to[e];
+    const double *ipw = G.ipw;
+    double c = ipw[8*((i64)l*G.nsh + fr) + 6]*f[(i64)l*G.vpl + fr];
+    c += ipw[8*((i64)l*G.nsh + to) + 6]*f[(i64)l*G.vpl + to];
+    if (l < G.L - 1) {
+        c += ipw[8*((i64)l*G.nsh + fr) + 7]*f[(i64)(l + 1)*G.vpl + fr];
+        c += ipw[8*((i64)l*G.nsh + to) + 7]*f[(i64)(l + 1)*G.vpl + to];
+    }
+    return 0.5*c;
+}
+
+/* averaging.c:97-111 */
+__device__ __forceinline__ double horizontal_covariant(const DevGrid &G, const double *w, int l, int e)
+{
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    double r = w[vi];
+    if (l >= G.L - G.n_oro) {
+        double vc = remap_verpri2horpri(G, w, l, e);
+        r += G.slope[vi]*vc;
+    }
+    return r;
+}
+
+/* averaging.c:41-95 */
+__device__ __forceinline__ double vertical_contravariant_corr(const DevGrid &G, const double *f, int l, int h)
+{
+    double r = 0;
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    if (l >= G.L - G.n_oro) {
+        if (l == G.L - G.n_oro) {
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+        } else {
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)(l - 1)*G.nsh + h, vi = G.nsh + (i64)(l - 1)*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+        }
+    }
+    return r;
+}
+
+/* divergences.c:14-64 for cell h of layer l */
+__device__ __forceinline__ double divv_h_cell(const DevGrid &G, const double *in, int l, int h)
+{
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    i64 i = (i64)l*G.nsh + h;
+    double comp_h = 0.0;
+    for (int j = 0; j < ne; ++j) {
+        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
+        comp_h += in[vi]*G.sgn[6*h + j]*G.area[vi];
+    }
+    double comp_v = 0.0;
+    int L = G.L, no = G.n_oro;
+    if (l == L - no - 1) {
+        double cl = vertical_contravariant_corr(G, in, l + 1, h);
+        comp_v = -cl*G.area[h + (i64)(l + 1)*G.vpl];
+    } else if (l == L - 1) {
+        double cu = vertical_contravariant_corr(G, in, l, h);
+        comp_v = cu*G.area[h + (i64)l*G.vpl];
+    } else if (l > L - no - 1) {
+        double cu = vertical_contravariant_corr(G, in, l, h);
+        double cl = vertical_contravariant_corr(G, in, l + 1, h);
+        comp_v = cu*G.area[h + (i64)l*G.vpl] - cl*G.area[h + (i64)(l + 1)*G.vpl];
+    }
+    return 1.0/G.vol[i]*(comp_h + comp_v);
+}
+
+/* divergences.c:66-148 for cell h of layer l */
+__device__ __forceinline__ double divv_h_tracer_cell(const DevGrid &G, const double *in, const double *dens,
+                                                     const double *wind, int l, int h)
+{
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    i64 i = (i64)l*G.nsh + h;
+    double comp_h = 0.0;
+    for (int j = 0; j < ne; ++j) {
+        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
+        comp_h += in[vi]*G.sgn[6*h + j]*G.area[vi];
+    }
+    double comp_v = 0.0;
+    int L = G.L, no = G.n_oro;
+    if (l == L - no - 1) {
+        double cl = vertical_contravariant_corr(G, wind, l + 1, h);
+        double dl = (cl <= 0.0) ? dens[i] : dens[i + G.nsh];
+        comp_v = -dl*cl*G.area[h + (i64)(l + 1)*G.vpl];
+    } else if (l == L - 1) {
+        double cu = vertical_contravariant_corr(G, wind, l, h);
+        double du = (cu <= 0.0) ? dens[i - G.nsh] : dens[i];
+        comp_v = du*cu*G.area[h + (i64)l*G.vpl];
+    } else if (l > L - no - 1) {
+        double cu = vertical_contravariant_corr(G, wind, l, h);
+        double du = (cu <= 0.0) ? dens[i - G.nsh] : dens[i];
+        double cl = vertical_contravariant_corr(G, wind, l + 1, h);
+        double dl = (cl <= 0.0) ? dens[i] : dens[i + G.nsh];
+        comp_v = du*cu*G.area[h + (i64)l*G.vpl] - dl*cl*G.area[h + (i64)(l + 1)*G.vpl];
+    }
+    return 1.0/G.vol[i]*(comp_h + comp_v);
+}
+
+/* dictionary.c:111-190 */
+__device__ __forceinline__ double c_p_cond(int k, double t)
+{
+    if ((k & 1) == 0) {
+        if (t > GC_T_0) t = GC_T_0;
+        if (t < 20.0) t = 20.0;
+        double q = t/125.1;
+        double r = -2.0572 + 0.14644*t + 0.06163*t*exp(-(q*q));
+        return r/GC_M_V;
+    }
+    double tc = t - GC_T_0, r;
+    if (tc >= 0.0) {
+        if (tc > 35.0) tc = 35.0;
+        double a = tc - 35.0;
+        r = 0.9979 + 3.1e-6*(a*a) + 3.8e-9*pow(a, 4.0);
+    } else {
+        if (tc < -37.0) tc = -37.0;
+        r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
+    }
+    return 4186.8*r;
+}
+
+/* derived_quantities.c:97-115 */
+__device__ __forceinline__ double c_v_mass_weighted(const DevGrid &G, const double *rho, double T, i64 i)
+{
+    double r = 0.0;
+    for (int k = 0; k < G.n_cond; ++k) r += rho[k*G.s + i]*c_p_cond(k, T);
+    r += rho[(i64)G.n_cond*G.s + i]*GC_C_D_V;
+    if (G.moist == 1) r += rho[(i64)(G.n_cond + 1)*G.s + i]*GC_C_V_V;
+    return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* kernels                                                              */
+/* ------------------------------------------------------------------ */
+
+/* derived_quantities.c:15-40 */
+__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, double *T)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    double t = (G.thbg[i] + s.tp[i])*(G.exbg[i] + s.ep[i]);
+    if (G.moist)
+        t = t/(1.0 + s.rho[(i64)(G.n_cond + 1)*G.s + i]/s.rho[(i64)G.n_cond*G.s + i]*(GC_M_D/GC_M_V - 1.0));
+    T[i] = t;
+}
+
+/* manage_pressure_gradient.c:17-87 at the interior w points (layers 1..L-1),
+   fused with calc_pressure_grad_condensates_v */
+__global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWork W, int first_step, int with_pgrad)
+{
+    i64 n = (i64)(G.L - 1)*G.nsh;
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= n) return;
+    int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
+    i64 w = h + (i64)lev*G.vpl, up = h + (i64)(lev - 1)*G.nsh, lo = h + (i64)lev*G.nsh;
+    i64 gi = (i64)G.n_cond*G.s;
+    double rt_up = density_total(G, s.rho, up), rt_lo = density_total(G, s.rho, lo);
+    double d_up = s.rho[gi + up]/rt_up, d_lo = s.rho[gi + lo]/rt_lo;
+    if (with_pgrad) {
+        if (!first_step) W.pgrad_old[w] = -W.pgrad_nl[w] - W.pgrad_l[w];
+        double raw = (s.ep[up] - s.ep[lo])/G.nd[w];
+        double p1_up = GC_C_D_P*(G.thbg[up] + s.tp[up]), p1_lo = GC_C_D_P*(G.thbg[lo] + s.tp[lo]);
+        double nl = (0.5*(p1_up + p1_lo))*raw;
+        double p2_up = GC_C_D_P*s.tp[up], p2_lo = GC_C_D_P*s.tp[lo];
+        double ll = (0.5*(p2_up + p2_lo))*G.exbg_grad[w];
+        nl = (0.5*(d_up + d_lo))*nl;
+        ll = (0.5*(d_up + d_lo))*ll;
+        W.pgrad_nl[w] = nl;
+        W.pgrad_l[w] = ll;
+        if (first_step) W.pgrad_old[w] = -nl - ll;
+    }
+    double c_up = s.rho[gi + up]/rt_up - 1.0, c_lo = s.rho[gi + lo]/rt_lo - 1.0;
+    W.pgc_v[w] = (0.5*(c_up + c_lo))*G.gm[w];
+}
+
+/* manage_pressure_gradient.c:17-72 at the edges */
+__global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWork W, int first_step)
+{
+    i64 n = (i64)G.L*G.nvh;
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= n) return;
+    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    if (!first_step) W.pgrad_old[vi] = -W.pgrad_nl[vi] - W.pgrad_l[vi];
+    i64 sf = (i64)l*G.nsh + G.from[e], st = (i64)l*G.nsh + G.to[e];
+    i64 gi = (i64)G.n_cond*G.s;
+    double g = grad_edge(G, s.ep, l, e);
+    double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
+    double nl = (0.5*(p1_t + p1_f))*g;
+    double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
+    double ll = (0.5*(p2_t + p2_f))*G.exbg_grad[vi];
+    double d_t = s.rho[gi + st]/density_total(G, s.rho, st), d_f = s.rho[gi + sf]/density_total(G, s.rho, sf);
+    nl = (0.5*(d_t + d_f))*nl;
+    ll = (0.5*(d_t + d_f))*ll;
+    W.pgrad_nl[vi] = nl;
+    W.pgrad_l[vi] = ll;
+    if (first_step) W.pgrad_old[vi] = -nl - ll;
+}
+
+/* scalar_times_vector(rho_gas, wind) into the mass flux density (multiplications.c:35-117) */
+__global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_gas, const double *wind, double *flux)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
+    if (t < ne) {
+        int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+        i64 vi = G.nsh + (i64)l*G.vpl + e;
+        double sv = 0.5*(rho_gas[G.to[e] + (i64)l*G.nsh] + rho_gas[G.from[e] + (i64)l*G.nsh]);
+        flux[vi] = sv*wind[vi];
+    } else if (t < ne + nw) {
+        t -= ne;
+        int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
+        i64 w = (i64)lev*G.vpl + h;
+        double sv = 0.5*(rho_gas[h + (i64)(lev - 1)*G.nsh] + rho_gas[h + (i64)lev*G.nsh]);
+        flux[w] = sv*wind[w];
+    }
+}
+
+/* vorticities.c:167-222 */
+__global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, double *res)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= (i64)G.L*G.nt) return;
+    int l = (int)(t/G.nt), h = (int)(t - (i64)l*G.nt);
+    double r = 0;
+    for (int j = 0; j < 3; ++j) {
+        i64 vi = G.nsh + (i64)l*G.vpl + G.vit[3*h + j];
+        double vel = w[vi], lrf = 1;
+        if (l >= G.L - G.n_oro) {
+            i64 di = G.nvh + (i64)l*G.dvpl + h;
+            lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
+            double delta_z = G.dzv[di] - G.zv[vi];
+            i64 ivg;
+            if (delta_z > 0) ivg = vi - G.vpl;
+            else if (l == G.L - 1) ivg = vi - G.vpl;
+            else ivg = vi + G.vpl;
+            double vg = (w[vi] - w[ivg])/(G.zv[vi] - G.zv[ivg]);
+            vel += delta_z*vg;
+        }
+        r += lrf*G.nd[vi]*G.vst[3*h + j]*vel;
+    }
+    res[t] = r/G.darea[G.nvh + (i64)l*G.dvpl + h];
+}
+
+/* tangential relative vorticity of edge e at level l (vorticities.c:131-156) */
+__device__ __forceinline__ double rel_vort_tangential(const DevGrid &G, const double *w, int l, int e)
+{
+    i64 base = (i64)l*G.vpl;
+    const double *nd = G.nd;
+    if (l == G.L) {
+        i64 i2 = base - G.nvh + e;
+        double cv2 = horizontal_covariant(G, w, l - 1, e);
+        return 1/G.darea[e + (i64)l*G.dvpl]*nd[i2]*cv2;
+    }
+    i64 i0 = base + G.nsh + e, i1 = base + G.from[e], i2 = base - G.nvh + e, i3 = base + G.to[e];
+    double cv0 = horizontal_covariant(G, w, l, e);
+    double cv2 = horizontal_covariant(G, w, l - 1, e);
+    return 1/G.darea[e + (i64)l*G.dvpl]*(
+        - nd[i0]*cv0
+        + nd[i1]*w[i1]
+        + nd[i2]*cv2
+        - nd[i3]*w[i3]);
+}
+
+/* calc_pot_vort = calc_rel_vort + add_f_to_rel_vort + division by the
+   interpolated density (vorticities.c:18-165), one thread per curl element */
+__global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, const double *rho, const double *rvt,
+                                                  double *pv)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.curl) return;
+    int nvh = G.nvh, nsh = G.nsh;
+    int l = (int)(i/(2*nvh)), h = (int)(i - (i64)l*2*nvh);
+    double rv, dv;
+    const int *fr = G.from, *to = G.to;
+    if (h >= nvh) {
+        int e = h - nvh;
+        i64 base = nvh + (i64)l*G.dvpl;
+        int fd = G.dfrom[e], td = G.dto[e];
+        rv = (G.darea[base + fd]*rvt[(i64)l*G.nt + fd] + G.darea[base + td]*rvt[(i64)l*G.nt + td])
+             /(G.darea[base + fd] + G.darea[base + td]);
+        dv = 0;
+        for (int j = 0; j < 4; ++j) dv += G.drw[4*e + j]*rho[(i64)l*nsh + G.dri[4*e + j]];
+    } else {
+        /* the top level copies the tangential vorticity of the level below (vorticities.c:158-163) */
+        rv = rel_vort_tangential(G, w, l == 0 ? 1 : l, h);
+        if (l == 0) {
+            dv = 0.5*(rho[fr[h]] + rho[to[h]])
+                 + (0.5*(rho[fr[h]] + rho[to[h]]) - 0.5*(rho[fr[h] + nsh] + rho[to[h] + nsh]))
+                 /G.pv_top_dz1[h]*G.pv_top_dz2[h];
+        } else if (l == G.L) {
+            i64 a = (i64)(l - 1)*nsh, b = (i64)(l - 2)*nsh;
+            dv = 0.5*(rho[a + fr[h]] + rho[a + to[h]])
+                 + (0.5*(rho[b + fr[h]] + rho[b + to[h]]) - 0.5*(rho[a + fr[h]] + rho[a + to[h]]))
+                 /G.pv_sfc_dz1[h]*G.pv_sfc_dz2[h];
+        } else {
+            i64 uf = (i64)(l - 1)*nsh + fr[h], ut = (i64)(l - 1)*nsh + to[h];
+            dv = 0.25*(rho[uf] + rho[ut] + rho[uf + nsh] + rho[ut + nsh]);
+        }
+    }
+    pv[i] = (rv + G.fvec[h])/dv;
+}
+
+/* vorticity_flux.c:16-187, horizontal part (one thread per edge and layer) */
+__global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const double *F, const double *pv, double *out)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= (i64)G.L*G.nvh) return;
+    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+    int nsh = G.nsh, nvh = G.nvh;
+    i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
+    const double *tw = G.tw + 10*(i64)e;
+    const int *ti = G.ti + 10*(i64)e, *tm = G.tmc + 10*(i64)e;
+    int fr = G.from[e], to = G.to[e];
+    double o = 0;
+    if (fr < GAME_N_PENT) {
+        for (int j = 0; j < 4; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+    } else {
+        for (int j = 0; j < 5; ++j) {
+            if (j == 2) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
+            else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+        }
+    }
+    if (to < GAME_N_PENT) {
+        for (int j = 5; j < 9; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+    } else {
+        for (int j = 5; j < 10; ++j) {
+            if (j == 7) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
+            else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+        }
+    }
+    const double *ipw = G.ipw;
+    i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
+    double q0 = pv[e + (i64)l*2*nvh], q1 = pv[e + (i64)(l + 1)*2*nvh];
+    o -= 0.5*ipw[8*sf + 6]*F[(i64)l*G.vpl + fr]*q0;
+    o -= 0.5*ipw[8*st + 6]*F[(i64)l*G.vpl + to]*q0;
+    o -= 0.5*ipw[8*sf + 7]*F[(i64)(l + 1)*G.vpl + fr]*q1;
+    o -= 0.5*ipw[8*st + 7]*F[(i64)(l + 1)*G.vpl + to]*q1;
+    out[G.nsh + (i64)l*G.vpl + e] = o;
+}
+
+/* vorticity_flux.c:373-417, vertical part (levels 0..L) */
+__global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const double *F, const double *pv, double *out)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= (i64)(G.L + 1)*G.nsh) return;
+    int l = (int)(t/G.nsh), h = (int)(t - (i64)l*G.nsh);
+    int nsh = G.nsh, nvh = G.nvh;
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    double vw = 0.5;
+    if (l == 0 || l == G.L) vw = 1;
+    double o = 0;
+    const int *adj = G.adj + 6*(i64)h;
+    if (l >= 1)
+        for (int j = 0; j < ne; ++j)
+            o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*F[nsh + (i64)(l - 1)*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
+    if (l <= G.L - 1)
+        for (int j = 0; j < ne; ++j)
+            o += vw*G.ipw[8*((i64)l*nsh + h) + j]*F[nsh + (i64)l*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
+    out[(i64)l*G.vpl + h] = o;
+}
+
+/* inner_product.c:13-43 */
+__global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *a, const double *b, double *out)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    int l = (int)(i/G.nsh), h = (int)(i - (i64)l*G.nsh);
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    const double *w = G.ipw + 8*i;
+    double o = 0;
+    for (int j = 0; j < ne; ++j) {
+        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
+        o += w[j]*a[vi]*b[vi];
+    }
+    o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
+    o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
+    out[i] = o;
+}
+
+/* grad (gradient_operators.c:56-74) of a scalar field: edges and interior w points */
+__global__ void __launch_bounds__(TPB) k_grad(DevGrid G, const double *f, double *out)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
+    if (t < ne) {
+        int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+        out[G.nsh + (i64)l*G.vpl + e] = grad_edge(G, f, l, e);
+    } else if (t < ne + nw) {
+        t -= ne;
+        int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
+        out[h + (i64)lev*G.vpl] = grad_w_raw(G, f, lev, h);
+    }
+}
+
+/* vector_tendencies_expl.c:74-131 fused with the horizontal wind update of
+   manage_rkhevi.c:67-75 */
+__global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWork W, double *tend_w, const double *wind_old,
+                                                     double *wind_new, double new_w, double chw, double ohw,
+                                                     double cvw, double dt)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.v) return;
+    double old_w = 1.0 - new_w;
+    int l = (int)(i/G.vpl);
+    int h = (int)(i - (i64)l*G.vpl);
+    if (i < G.nsh || i >= G.v - G.nsh) { tend_w[i] = 0.0; return; }
+    /* friction_acc is zero with momentum diffusion and PBL off: x + 0.0 == x */
+    if (h >= G.nsh) {
+        double tv = old_w*tend_w[i] + new_w*(
+            ohw*W.pgrad_old[i]
+            - chw*(W.pgrad_nl[i] + W.pgrad_l[i])
+            + W.pvt[i]
+            - 0.5*W.vsq_grad[i]);
+        tend_w[i] = tv;
+        wind_new[i] = wind_old[i] + dt*tv;
+    } else {
+        tend_w[i] = old_w*tend_w[i] + new_w*(
+            -cvw*(W.pgrad_nl[i] + W.pgrad_l[i])
+            + W.pvt[i]
+            - 0.5*W.vsq_grad[i]
+            + W.pgc_v[i]);
+    }
+}
+
+/* scalar_tendencies_expl.c:102-111, 141-150 at the edges: centred mass flux and
+   the rhotheta_v flux of the main gas */
+__global__ void __launch_bounds__(TPB) k_scalar_flux_main(DevGrid G, DevState s, double *flux, double *flux2)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= (i64)G.L*G.nvh) return;
+    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    i64 sf = (i64)l*G.nsh + G.from[e], st = (i64)l*G.nsh + G.to[e];
+    const double *rg = s.rho + (i64)G.n_cond*G.s;
+    double f1 = (0.5*(rg[st] + rg[sf]))*s.wind[vi];
+    double th_t = s.rt[st]/rg[st], th_f = s.rt[sf]/rg[sf];
+    flux[vi] = f1;
+    flux2[vi] = (0.5*(th_t + th_f))*f1;
+}
+
+/* scalar_tendencies_expl.c:119-177 for the main gas: rho and rhotheta_v tendencies */
+__global__ void __launch_bounds__(TPB) k_scalar_tend_main(DevGrid G, DevState s, DevWork W, double *tend_rho,
+                                                          double *tend_rt, double new_w)
+{
+    i64 j = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (j >= G.s) return;
+    int l = (int)(j/G.nsh), h = (int)(j - (i64)l*G.nsh);
+    double old_w = 1.0 - new_w;
+    i64 shift = (i64)G.n_cond*G.s;
+    double div1 = divv_h_cell(G, W.flux, l, h);
+    double div2 = divv_h_cell(G, W.flux2, l, h);
+    double rg = s.rho[shift + j];
+    double th = s.rt[j]/rg;
+    if (!G.moist) {
+        /* dry: every source term of the reference sum is a zero array */
+        tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div1);
+        tend_rt[j] = old_w*tend_rt[j] + new_w*(-div2);
+        return;
+    }
+    /* the moist-air gas takes the vapour phase-transition rate (index n_cond of
+       phase_trans_rates, scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
+    double ptr_v = W.ptr[shift + j];
+    tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div1 + ptr_v);
+    double cv = c_v_mass_weighted(G, s.rho, W.temp[j], j);
+    double heat = W.phr[j] + W.sed_heat[j];
+    tend_rt[j] = old_w*tend_rt[j] + new_w*(
+        -div2
+        + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[j] + s.ep[j]))
+        + ptr_v*th);
+}
+
+/* upstream tracer flux (multiplications.c:69-96) */
+__global__ void __launch_bounds__(TPB) k_tracer_flux(DevGrid G, const double *rho_k, const double *wind, double *flux)
+{
+    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t >= (i64)G.L*G.nvh) return;
+    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    double u = wind[vi];
+    double sv = (u >= 0.0) ? rho_k[G.from[e] + (i64)l*G.nsh] : rho_k[G.to[e] + (i64)l*G.nsh];
+    flux[vi] = sv*u;
+}
+
+/* tracer tendency (scalar_tendencies_expl.c:113-133) */
+__global__ void __launch_bounds__(TPB) k_tracer_tend(DevGrid G, DevState s, DevWork W, double *tend_rho, int k,
+                                                     double new_w)
+{
+    i64 j = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (j >= G.s) return;
+    int l = (int)(j/G.nsh), h = (int)(j - (i64)l*G.nsh);
+    double old_w = 1.0 - new_w;
+    i64 shift = (i64)k*G.s;
+    i64 shift_pt = (k == G.n_cond + 1) ? shift - G.s : shift;
+    double div = divv_h_tracer_cell(G, W.flux, s.rho + shift, s.wind, l, h);
+    tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div + W.ptr[shift_pt + j]);
+}
+
+/* three_band_solver_ver_waves (column_solvers.c:19-383), one column per lane.
+   The coefficient rows are generated and eliminated in one downward sweep;
+   the back substitution walks upwards and writes each layer as soon as both
+   of its interface fluxes are known.  Quantities used in both sweeps are
+   recomputed with identical arithmetic rather than stored. */
+struct ColumnLayer {
+    double re, rte, al, be, ga, tpe, epe;
+};
+
+__device__ __forceinline__ ColumnLayer column_layer(const DevGrid &G, const DevState &so, const DevState &sn,
+                                                    const DevState &td, double dt, int rk, int i, int j)
+{
+    ColumnLayer c;
+    i64 b = i + (i64)j*G.nsh, gi = (i64)G.n_cond*G.s;
+    double vol = G.vol[b];
+    double rho_o = so.rho[gi + b], rt_o = so.rt[b];
+    c.re = rho_o + dt*td.rho[gi + b];
+    c.rte = rt_o + dt*td.rt[b];
+    if (rk == 0) {
+        c.al = -rt_o/(rho_o*rho_o)/vol;
+        c.be = 1.0/rho_o/vol;
+        c.ga = GC_R_D/(GC_C_D_V*rt_o)*(G.exbg[b] + so.ep[b])/vol;
+    } else {
+        const double pdw = 0.5;
+        double rho_n = sn.rho[gi + b], rt_n = sn.rt[b];
+        double ao = -rt_o/(rho_o*rho_o);
+        double bo = 1.0/rho_o;
+        double go = GC_R_D/(GC_C_D_V*rt_o)*(G.exbg[b] + so.ep[b]);
+        double an = -rt_n/(rho_n*rho_n);
+        double bn = 1.0/rho_n;
+        double gn = GC_R_D/(GC_C_D_V*rt_n)*(G.exbg[b] + sn.ep[b]);
+        c.al = ((1.0 - pdw)*ao + pdw*an)/vol;
+        c.be = ((1.0 - pdw)*bo + pdw*bn)/vol;
+        c.ga = ((1.0 - pdw)*go + pdw*gn)/vol;
+    }
+    c.tpe = so.tp[b] + dt*vol*(c.al*td.rho[gi + b] + c.be*td.rt[b]);
+    c.epe = so.ep[b] + dt*vol*c.ga*td.rt[b];
+    return c;
+}
+
+/* interface values of the column at interface j (between layers j and j+1) */
+__device__ __forceinline__ double theta_int_new(const DevGrid &G, const DevState &sn, int i, int j)
+{
+    i64 gi = (i64)G.n_cond*G.s, b = i + (i64)j*G.nsh, lo = b + G.nsh;
+    return 0.5*(sn.rt[b]/sn.rho[gi + b] + sn.rt[lo]/sn.rho[gi + lo]);
+}
+
+__device__ __forceinline__ double rho_int_old(const DevGrid &G, const DevState &so, int i, int j)
+{
+    i64 gi = (i64)G.n_cond*G.s, b = i + (i64)j*G.nsh;
+    return 0.5*(so.rho[gi + b] + so.rho[gi + b + G.nsh]);
+}
+
+template <int LT>
+__global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevState sn, DevState td, StepConsts K,
+                                                   int rk)
+{
+    int i = blockIdx.x*128 + threadIdx.x;
+    if (i >= G.nsh) return;
+    const int L = LT > 0 ? LT : G.L;
+    const int nsh = G.nsh, vpl = G.vpl;
+    const i64 gi = (i64)G.n_cond*G.s;
+    const double dt = K.dt, iw = K.impl_weight;
+    const double dt2 = dt*dt;
+    const double *zs = G.zs, *exbg = G.exbg, *vol = G.vol, *area = G.area;
+    double ep[(LT > 0 ? LT : GAME_MAX_LAYERS) - 1], x[(LT > 0 ? LT : GAME_MAX_LAYERS) - 1];
+
+    /* downward sweep: build row j and eliminate it (column_solvers.c:141-189, 641-660) */
+    ColumnLayer ca = column_layer(G, so, sn, td, dt, rk, i, 0);   /* layer j */
+    ColumnLayer cb = column_layer(G, so, sn, td, dt, rk, i, 1);   /* layer j+1 */
+    double tin = theta_int_new(G, sn, i, 0), rio = rho_int_old(G, so, i, 0);
+    double c_prev = 0.0;
+#pragma unroll
+    for (int j = 0; j < L - 1; ++j) {
+        i64 b = i + (i64)j*nsh, lo = b + nsh;
+        i64 wi = i + (i64)(j + 1)*vpl;
+        double w_old = so.wind[wi];
+        double rie = 0.5*(ca.re + cb.re);
+        double d = -(tin*tin)*(ca.ga + cb.ga)
+                   + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
+                   - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
+                     *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
+        double r = -(w_old + dt*td.wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+                   + tin*(ca.epe - cb.epe)/dt
+                   + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
+                   - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
+        double e = 0.0, c = 0.0, tin_n = 0.0, rio_n = 0.0;
+        ColumnLayer cc = cb;
+        if (j < L - 2) {
+            i64 l2 = lo + nsh;
+            tin_n = theta_int_new(G, sn, i, j + 1);
+            rio_n = rho_int_old(G, so, i, j + 1);
+            /* upper diagonal e[j] and lower diagonal c[j] (column_solvers.c:173-189) */
+            c = tin_n*cb.ga*tin
+                + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
+                - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
+            e = tin*cb.ga*tin_n
+                - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
+                + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
+            cc = column_layer(G, so, sn, td, dt, rk, i, j + 2);
+        }
+        if (j == 0) {
+            if (L - 1 > 1) ep[0] = e/d;
+            x[0] = r/d;
+        } else {
+            double den = d - ep[j - 1]*c_prev;
+            if (j < L - 2) ep[j] = e/den;
+            x[j] = (r - x[j - 1]*c_prev)/den;
+        }
+        c_prev = c;
+        ca = cb; cb = cc;
+        tin = tin_n; rio = rio_n;
+    }
+    /* back substitution (column_solvers.c:662-667) */
+#pragma unroll
+    for (int j = L - 3; j >= 0; --j) x[j] = x[j] - ep[j]*x[j + 1];
+    /* Klemp (2008) damping of the solved fluxes (column_solvers.c:284-298) */
+#pragma unroll
+    for (int j = 0; j < L - 1; ++j) x[j] = x[j]/(1.0 + dt*G.damp[(i64)j*nsh + i]);
+    /* new state, layer by layer, each layer read before it is written (column_solvers.c:300-370) */
+    double tin_m = 0.0, rho_new_m = 0.0, rio_m = 0.0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        i64 b = i + (i64)j*nsh;
+        ColumnLayer cl = column_layer(G, so, sn, td, dt, rk, i, j);
+        double tin_j = (j < L - 1) ? theta_int_new(G, sn, i, j) : 0.0;
+        double rio_j = (j < L - 1) ? rho_int_old(G, so, i, j) : 0.0;
+        double v = vol[b];
+        double rho_n, rt_n;
+        if (j == 0) {
+            rho_n = cl.re + dt*(x[j])/v;
+            rt_n = cl.rte + dt*(tin_j*x[j])/v;
+        } else if (j == L - 1) {
+            rho_n = cl.re + dt*(-x[j - 1])/v;
+            rt_n = cl.rte + dt*(-tin_m*x[j - 1])/v;
+        } else {
+            rho_n = cl.re + dt*(-x[j - 1] + x[j])/v;
+            rt_n = cl.rte + dt*(-tin_m*x[j - 1] + tin_j*x[j])/v;
+        }
+        double rt_o = so.rt[b];
+        sn.rho[gi + b] = rho_n;
+        sn.rt[b] = rt_n;
+        sn.tp[b] = rt_n/rho_n - G.thbg[b];
+        sn.ep[b] = so.ep[b] + v*cl.ga*(rt_n - rt_o);
+        if (j >= 1) {
+            i64 wi = i + (i64)j*vpl;
+            double din = 0.5*(rho_new_m + rho_n);
+            sn.wind[wi] = (2.0*x[j - 1]/area[wi] - din*so.wind[wi])/rio_m;
+        }
+        tin_m = tin_j;
+        rho_new_m = rho_n;
+        rio_m = rio_j;
+    }
+}
+
+/* three_band_solver_gen_densities (column_solvers.c:385-639) for tracer k,
+   one column per lane */
+template <int LT>
+__global__ void __launch_bounds__(128) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td, DevWork W,
+                                                       StepConsts K, int rk, int k)
+{
+    int i = blockIdx.x*128 + threadIdx.x;
+    if (i >= G.nsh) return;
+    const int L = LT > 0 ? LT : G.L;
+    const int nsh = G.nsh, vpl = G.vpl, ncc = G.n_cond;
+    const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
+    const double *vol = G.vol, *area = G.area, *T = W.temp;
+    double *sh = W.sed_heat;
+    const double *rho_o = so.rho + (i64)k*G.s;
+    double vsink = 0.0;
+    if (k < ncc/4) vsink = K.snow_v;
+    else if (k < ncc/2) vsink = K.rain_v;
+    else if (k < ncc) vsink = K.cloud_v;
+    double ep[LT > 0 ? LT : GAME_MAX_LAYERS], x[LT > 0 ? LT : GAME_MAX_LAYERS];
+    i64 sfc_area = i + G.v - nsh, low = i + G.s - nsh;
+    double fi_m = 0.0, fr_m = 0.0, fe_m = 0.0, c_prev = 0.0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        i64 b = i + (i64)j*nsh;
+        double fi = 0.0, fr = 0.0, fe = 0.0;
+        if (j < L - 1) {
+            /* vertical fluxes through interface j (column_solvers.c:414-465) */
+            if (rk == 0 && k == 0) sh[b] = 0.0;
+            fi = so.wind[i + (i64)(j + 1)*vpl];
+            fr = sn.wind[i + (i64)(j + 1)*vpl];
+            if (k < ncc/4) { fi -= K.snow_v; fr -= K.snow_v; }
+            else if (k < ncc/2) { fi -= K.rain_v; fr -= K.rain_v; }
+            else if (k < ncc) { fi -= K.cloud_v; fr -= K.cloud_v; }
+            double a = area[i + (i64)(j + 1)*vpl];
+            fi = a*fi;
+            fr = a*fr;
+            double doi, toi;
+            if (fr >= 0.0) { doi = rho_o[b + nsh]; toi = T[b + nsh]; }
+            else { doi = rho_o[b]; toi = T[b]; }
+            fr = doi*fr;
+            if (k < ncc) fe = c_p_cond(k, toi)*toi*fr;
+        } else if (rk == 0 && k == 0) sh[b] = 0.0;
+        /* row j (column_solvers.c:474-615) */
+        double d;
+        if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - iw*dt/vol[b]*fi;
+        else if (j == L - 1) {
+            d = (fi_m >= 0.0) ? 1.0 + iw*dt/vol[b]*fi_m : 1.0;
+            if (k < ncc) d += iw*vsink*dt*area[sfc_area]/vol[b];
+        } else {
+            d = 1.0;
+            if (fi_m >= 0.0) d += iw*dt/vol[b]*fi_m;
+            if (fi < 0.0) d -= iw*dt/vol[b]*fi;
+        }
+        double r = rho_o[b] + dt*td.rho[(i64)k*G.s + b];
+        if (j == 0) {
+            r += ew*dt*fr/vol[b];
+            if (rk == 0 && k < ncc) sh[b] += fe/vol[b];
+        } else if (j == L - 1) {
+            r += -ew*dt*fr_m/vol[b];
+            if (rk == 0 && k < ncc) sh[b] += -fe_m/vol[b];
+            if (k < ncc) {
+                r += -ew*vsink*dt*rho_o[low]*area[sfc_area]/vol[b];
+                if (rk == 0) sh[b] += -vsink*T[low]*c_p_cond(k, T[low])*rho_o[low]*area[sfc_area]/vol[b];
+            }
+        } else {
+            r += ew*dt*(-fr_m + fr)/vol[b];
+            if (rk == 0 && k < ncc) sh[b] += (-fe_m + fe)/vol[b];
+        }
+        /* off-diagonals of interface j (column_solvers.c:475-488) */
+        double c = 0.0, e = 0.0;
+        if (j < L - 1) {
+            if (fi >= 0.0) { c = 0.0; e = -iw*dt/vol[b]*fi; }
+            else { c = iw*dt/vol[b + nsh]*fi; e = 0.0; }
+        }
+        if (j == 0) {
+            ep[0] = e/d;
+            x[0] = r/d;
+        } else {
+            double den = d - ep[j - 1]*c_prev;
+            if (j < L - 1) ep[j] = e/den;
+            x[j] = (r - x[j - 1]*c_prev)/den;
+        }
+        c_prev = c;
+        fi_m = fi; fr_m = fr; fe_m = fe;
+    }
+#pragma unroll
+    for (int j = L - 2; j >= 0; --j) x[j] = x[j] - ep[j]*x[j + 1];
+    double *rho_n = sn.rho + (i64)k*G.s;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        double v = x[j];
+        if (v < 0.0) v = 0.0;
+        rho_n[i + (i64)j*nsh] = v;
+    }
+}
+
+/* phase_trans.c:16-311 (surface phase transitions off), one thread per grid box */
+__device__ __forceinline__ double sat_water(double t)
+{   /* dictionary.c:287-333 */
+    double tc = t - GC_T_0, r;
+    if (tc > 100.0) tc = 100.0;
+    if (tc > 0.0) r = exp(34.494 - 4924.99/(tc + 237.1))/pow(tc + 105.0, 1.57);
+    else {
+        if (tc < -50.0) tc = -50.0;
+        r = 6.107799961
+            + 4.436518521e-1*tc
+            + 1.428945805e-2*(tc*tc)
+            + 2.650648471e-4*pow(tc, 3.0)
+            + 3.031240396e-6*pow(tc, 4.0)
+            + 2.034080948e-8*pow(tc, 5.0)
+            + 6.136820929e-11*pow(tc, 6.0);
+        r = 100.0*r;
+    }
+    return r;
+}
+
+__device__ __forceinline__ double sat_ice(double t)
+{   /* dictionary.c:361-385 */
+    double tc = t - GC_T_0;
+    if (tc < -80.0) tc = -80.0;
+    if (tc > 0.0) tc = 0.0;
+    double a = tc + 868.0;
+    return exp(43.494 - 6545.8/(tc + 278.0))/(a*a);
+}
+
+__device__ __forceinline__ double dsat_water(double t)
+{   /* dictionary.c:335-359 */
+    double tc = t - GC_T_0;
+    if (tc > 100.0) tc = 100.0;
+    if (tc < 0.0) tc = 0.0;
+    double a = tc + 237.1;
+    return sat_water(t)*(4924.99/(a*a) - 1.57/(tc + 105.0));
+}
+
+__device__ __forceinline__ double dsat_ice(double t)
+{   /* dictionary.c:387-412 */
+    double tc = t - GC_T_0;
+    if (tc < -80.0) tc = -80.0;
+    if (tc > 0.0) tc = 0.0;
+    double a = tc + 278.0;
+    return sat_ice(t)*(6545.8/(a*a) - 2.0/(tc + 868.0));
+}
+
+__device__ __forceinline__ double enthalpy_evaporation(double t)
+{   /* dictionary.c:222-258 */
+    if (t < GC_T_0) {
+        if (t < 30.0) t = 30.0;
+        return (56579.0 - 42.212*t + exp(0.1149*(281.6 - t)))/GC_M_V;
+    }
+    if (t > GC_T_0 + 100.0) t = GC_T_0 + 100.0;
+    return 3151378.0 - 2386.0*t;
+}
+
+__device__ __forceinline__ double enthalpy_sublimation(double t)
+{   /* dictionary.c:260-285 */
+    if (t < 30.0) t = 30.0;
+    if (t > GC_T_0) t = GC_T_0;
+    double q = t/123.75;
+    return (46782.5 + 35.8925*t - 0.07414*(t*t) + 541.5*exp(-(q*q)))/GC_M_V;
+}
+
+__device__ __forceinline__ double phase_trans_heat(int dir, double t)
+{   /* dictionary.c:192-220 */
+    if (dir == 0) return enthalpy_evaporation(t);
+    if (dir == 1) return enthalpy_sublimation(t);
+    return enthalpy_sublimation(t) - enthalpy_evaporation(t);
+}
+
+__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, DevWork W, double dt)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    const i64 S = G.s;
+    const double *rho = s.rho;
+    double *ptr = W.ptr;
+    double T = W.temp[i];
+    const double mcwc = 0.2e-3;
+    int ncc = G.n_cond;
+    double sp = (T >= GC_T_0) ? sat_water(T) : sat_ice(T);
+    double wvp = rho[(i64)(ncc + 1)*S + i]*GC_R_V*T;
+    double dp = (rho[(i64)ncc*S + i] - rho[(i64)(ncc + 1)*S + i])*GC_R_D*T;
+    double ap = dp + wvp;
+    double ef = (T >= GC_T_0) ? 1.00071*exp(0.000000045*ap) : 0.99882*exp(0.00000008*ap);
+    sp = ef*sp;
+    double cvm = c_v_mass_weighted(G, rho, T, i);
+    int warm = T >= GC_T_0;
+    int dir = warm ? 0 : 1;
+    double pth = phase_trans_heat(dir, T);
+    double ds = warm ? dsat_water(T) : dsat_ice(T);
+    double a = -GC_R_V*pth/cvm;
+    double b = GC_R_V*T - GC_R_V*rho[(i64)(ncc + 1)*S + i]*pth/cvm + ef*ds*pth/cvm;
+    double cc = wvp - sp;
+    double p = b/a, q = cc/a;
+    double dd = -0.5*p - pow(0.25*(p*p) - q, 0.5);
+    double r0, r1, r2, r3, r4, heat;
+    if (sp >= wvp) {
+        if (warm) {
+            r2 = -rho[2*S + i]/dt;
+            double ptd = fmin(rho[3*S + i], dd);
+            r4 = ptd/dt;
+            r3 = rho[2*S + i]/dt - ptd/dt;
+            heat = r2*phase_trans_heat(2, T) - ptd*phase_trans_heat(0, T)/dt;
+        } else {
+            r3 = -rho[3*S + i]/dt;
+            double ptd = fmin(rho[2*S + i], dd);
+            r4 = ptd/dt;
+            r2 = rho[3*S + i]/dt - ptd/dt;
+            heat = -r3*phase_trans_heat(2, T) - ptd*phase_trans_heat(1, T)/dt;
+        }
+    } else {
+        if (warm) {
+            r2 = -rho[2*S + i]/dt;
+            r4 = dd/dt;
+            r3 = rho[2*S + i]/dt - dd/dt;
+            heat = r2*phase_trans_heat(2, T) - dd*phase_trans_heat(0, T)/dt;
+        } else {
+            r3 = -rho[3*S + i]/dt;
+            r4 = dd/dt;
+            r2 = rho[3*S + i]/dt - dd/dt;
+            heat = -r3*phase_trans_heat(2, T) - dd*phase_trans_heat(1, T)/dt;
+        }
+    }
+    r0 = 0.0;
+    r1 = 0.0;
+    if (T < GC_T_0) {
+        r0 = fmax(rho[2*S + i] - mcwc*rho[4*S + i], 0.0)/1000.0;
+        r2 -= r0;
+    } else if (T >= GC_T_0) {
+        r1 = fmax(rho[3*S + i] - mcwc*rho[4*S + i], 0.0)/1000.0;
+        r3 -= r1;
+    }
+    if (T >= GC_T_0 && rho[i] > 0.0) {
+        r0 = -rho[i]/dt;
+        r1 -= r0;
+        heat += r0*phase_trans_heat(2, T);
+    }
+    if (T < GC_T_0 && rho[S + i] > 0.0) {
+        r1 = -rho[S + i]/dt;
+        r0 -= r1;
+        heat += -r1*phase_trans_heat(2, T);
+    }
+    ptr[i] = r0;
+    ptr[S + i] = r1;
+    ptr[2*S + i] = r2;
+    ptr[3*S + i] = r3;
+    ptr[4*S + i] = r4;
+    W.phr[i] = heat;
+}
+
+/* linear_combine_two_states.c:14-43 */
+__global__ void __launch_bounds__(TPB) k_lincomb_scalars(DevGrid G, DevState a, DevState b, DevState o, double c0,
+                                                         double c1)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    for (int k = 0; k < G.n_const; ++k) o.rho[k*G.s + i] = c0*a.rho[k*G.s + i] + c1*b.rho[k*G.s + i];
+    double rt = c0*a.rt[i] + c1*b.rt[i];
+    o.rt[i] = rt;
+    o.tp[i] = rt/o.rho[(i64)G.n_cond*G.s + i] - G.thbg[i];
+    o.ep[i] = c0*a.ep[i] + c1*b.ep[i];
+}
+
+__global__ void __launch_bounds__(TPB) k_lincomb_vector(i64 n, const double *a, const double *b, double *o, double c0,
+                                                        double c1)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= n) return;
+    o[i] = c0*a[i] + c1*b[i];
+}
+
+/* per-block compensated partial sums of x*volume (write_output.c:1659-1668 with
+   Neumaier summation); the host adds the block partials, also compensated */
+__global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *x, const double *y, const double *vol,
+                                                           double *partials)
+{
+    __shared__ double ss[TPB], sc[TPB];
+    double s = 0.0, c = 0.0;
+    for (i64 i = (i64)blockIdx.x*TPB + threadIdx.x; i < n; i += (i64)gridDim.x*TPB) {
+        double v = x[i]*vol[i];
+        if (y) v = x[i]*y[i]*vol[i];
+        double t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = 0.0, Cc = 0.0;
+        for (int k = 0; k < TPB; ++k) {
+            double v = ss[k], t = S + v;
+            if (fabs(S) >= fabs(v)) Cc += (S - t) + v;
+            else Cc += (v - t) + S;
+            S = t;
+            Cc += sc[k];
+        }
+        partials[2*blockIdx.x] = S;
+        partials[2*blockIdx.x + 1] = Cc;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* host-side launchers                                                  */
+/* ------------------------------------------------------------------ */
+namespace fp64 {
+
+void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature, GRID1(G.s), dim3(TPB), st, G, s, T); }
+
+void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
+{
+    LAUNCH("k_pgrad_w", k_pgrad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
+    LAUNCH("k_pgrad_e", k_pgrad_e, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s, W, first);
+}
+
+void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st)
+{ LAUNCH("k_pgrad_w", k_pgrad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
+
+void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)
+{
+    LAUNCH("k_mass_flux", k_mass_flux, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas, wind, W.flux);
+    LAUNCH("k_rel_vort_tri", k_rel_vort_tri, GRID1((i64)G.L*G.nt), dim3(TPB), st, G, wind, W.rvt);
+    LAUNCH("k_pot_vort", k_pot_vort, GRID1(G.curl), dim3(TPB), st, G, wind, rho_gas, W.rvt, W.pv);
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
+    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID1((i64)(G.L + 1)*G.nsh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
+    LAUNCH("k_inner_product", k_inner_product, GRID1(G.s), dim3(TPB), st, G, wind, wind, W.vsq);
+    LAUNCH("k_grad", k_grad, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, W.vsq, W.vsq_grad);
+}
+
+void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
+{ LAUNCH("k_inner_product", k_inner_product, GRID1(G.s), dim3(TPB), st, G, a, b, out); }
+
+void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
+                 double new_w, double itw, double dt, hipStream_t st)
+{
+    double chw = 0.5 + itw, ohw = 1.0 - chw, cvw = 1.0 - itw;
+    LAUNCH("k_vector_tend", k_vector_tend, GRID1(G.v), dim3(TPB), st, G, W, tend_w, w_old, w_new, new_w, chw, ohw, cvw, dt);
+}
+
+void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st)
+{
+    for (int k = 0; k < G.n_const; ++k) {
+        double new_w = (rk == 1 && k != G.n_cond) ? 0.5 : 1.0;
+        if (k == G.n_cond) {
+            LAUNCH("k_scalar_flux_main", k_scalar_flux_main, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s, W.flux, W.flux2);
+            LAUNCH("k_scalar_tend_main", k_scalar_tend_main, GRID1(G.s), dim3(TPB), st, G, s, W, td.rho, td.rt, new_w);
+        } else {
+            LAUNCH("k_tracer_flux", k_tracer_flux, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s.rho + (i64)k*G.s, s.wind, W.flux);
+            LAUNCH("k_tracer_tend", k_tracer_tend, GRID1(G.s), dim3(TPB), st, G, s, W, td.rho, k, new_w);
+        }
+    }
+}
+
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
+               int rk, hipStream_t st)
+{
+    dim3 g((G.nsh + 127)/128), b(128);
+    if (G.L == 26) LAUNCH("k_ver_waves", k_ver_waves<26>, g, b, st, G, so, sn, td, K, rk);
+    else if (G.L == 55) LAUNCH("k_ver_waves", k_ver_waves<55>, g, b, st, G, so, sn, td, K, rk);
+    else LAUNCH("k_ver_waves", k_ver_waves<0>, g, b, st, G, so, sn, td, K, rk);
+}
+
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const DevWork &W,
+                   const StepConsts &K, int rk, hipStream_t st)
+{
+    dim3 g((G.nsh + 127)/128), b(128);
+    for (int k = 0; k < G.n_const; ++k) {
+        if (k == G.n_cond) continue;
+        if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities<26>, g, b, st, G, so, sn, td, W, K, rk, k);
+        else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities<55>, g, b, st, G, so, sn, td, W, K, rk, k);
+        else LAUNCH("k_gen_densities", k_gen_densities<0>, g, b, st, G, so, sn, td, W, K, rk, k);
+    }
+}
+
+void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st)
+{ LAUNCH("k_phase_trans", k_phase_trans, GRID1(G.s), dim3(TPB), st, G, s, W, dt); }
+
+void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
+                    hipStream_t st)
+{
+    LAUNCH("k_lincomb_scalars", k_lincomb_scalars, GRID1(G.s), dim3(TPB), st, G, a, b, o, c0, c1);
+    LAUNCH("k_lincomb_vector", k_lincomb_vector, GRID1(G.v), dim3(TPB), st, G.v, (const double *)a.wind, (const double *)b.wind,
+                       o.wind, c0, c1);
+}
+
+int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st)
+{
+    int blocks = 1024;
+    LAUNCH("k_integral_partials", k_integral_partials, dim3(blocks), dim3(TPB), st, n, x, y, vol, partials);
+    return blocks;
+}
+
+}  // namespace fp64

@@ -1,0 +1,29 @@
+/*
+ * rkhevi_kernels.h -- launchers of the device kernels (one namespace per
+ * precision form of the step).
+ */
+#ifndef GAME_RKHEVI_KERNELS_H
+#define GAME_RKHEVI_KERNELS_H
+
+#include "rkhevi_dev.h"
+
+namespace fp64 {
+void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st);
+void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st);
+void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st);
+void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st);
+void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st);
+void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
+                 double new_w, double itw, double dt, hipStream_t st);
+void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st);
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
+               int rk, hipStream_t st);
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const DevWork &W,
+                   const StepConsts &K, int rk, hipStream_t st);
+void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st);
+void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
+                    hipStream_t st);
+int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st);
+}
+
+#endif

@@ -1,0 +1,536 @@
+/*
+ * rkhevi_runtime.hip -- device-resident RKHEVI model behind the C ABI of
+ * include/game_rkhevi.h.
+ *
+ * One process holds one model: the grid tables are uploaded once, the three
+ * states (old, new, tendency) and every carried field (forcings, diagnostics,
+ * irreversible quantities) stay in HBM, and the step is a fixed sequence of
+ * kernel launches on one stream.  The sequence is manage_rkhevi
+ * (src/time_stepping/manage_rkhevi.c:20-100).
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/game_rkhevi.h"
+#include "rkhevi_dev.h"
+#include "rkhevi_kernels.h"
+
+KernelTimer *g_kernel_timer = nullptr;
+
+namespace {
+
+char g_err[512];
+
+#define HIP_OK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            snprintf(g_err, sizeof g_err, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                                   \
+            return GAME_ERR_HIP;                                                            \
+        }                                                                                   \
+    } while (0)
+
+struct Stat {
+    std::string name;
+    long long launches = 0;
+    double ms = 0.0;
+};
+
+struct EventTimer : KernelTimer {
+    std::vector<Stat> stats;
+    struct Pending { int idx; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    int cur = -1;
+    hipEvent_t cur_a = nullptr;
+
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    int index(const char *name) {
+        for (size_t k = 0; k < stats.size(); ++k) if (stats[k].name == name) return (int)k;
+        Stat s; s.name = name; stats.push_back(s);
+        return (int)stats.size() - 1;
+    }
+    void begin(const char *name, hipStream_t st) override {
+        cur = index(name);
+        cur_a = get();
+        (void)hipEventRecord(cur_a, st);
+    }
+    void end(hipStream_t st) override {
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, st);
+        pending.push_back({cur, cur_a, b});
+    }
+    void collect() {
+        for (auto &p : pending) {
+            float ms = 0.f;
+            (void)hipEventSynchronize(p.b);
+            (void)hipEventElapsedTime(&ms, p.a, p.b);
+            stats[p.idx].launches += 1;
+            stats[p.idx].ms += ms;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+};
+
+struct Model {
+    game_dims_t d;
+    game_config_t cfg;
+    int precision = GAME_FP64;
+    DevGrid G;
+    DevState s[3];
+    DevWork W;
+    StepConsts K;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    long long bytes = 0;
+    double *partials = nullptr, *h_partials = nullptr;
+    double *tmp_s = nullptr;
+    EventTimer timer;
+};
+
+Model *M = nullptr;
+
+int dalloc(void **p, size_t n)
+{
+    if (n == 0) n = 8;
+    HIP_OK(hipMalloc(p, n));
+    HIP_OK(hipMemsetAsync(*p, 0, n, M->stream));
+    M->allocs.push_back(*p);
+    M->bytes += (long long)n;
+    return 0;
+}
+
+template <class T>
+int upload(const T **dst, const T *src, size_t count)
+{
+    void *p;
+    int rc = dalloc(&p, count*sizeof(T));
+    if (rc) return rc;
+    if (src) HIP_OK(hipMemcpyAsync(p, src, count*sizeof(T), hipMemcpyHostToDevice, M->stream));
+    *dst = (const T *)p;
+    return 0;
+}
+
+int dzeros(double **dst, size_t count)
+{
+    void *p;
+    int rc = dalloc(&p, count*sizeof(double));
+    *dst = (double *)p;
+    return rc;
+}
+
+/* what z_vector[k] reads in the reference's struct grid, where gravity_potential
+   and gravity_m follow z_vector (src/game_types.h:76-79); see calc_pot_vort,
+   src/spatial_operators/vorticities.c:48-71 */
+double ref_z_vector_read(const game_dims_t &d, const game_grid_view_t &g, long long k)
+{
+    if (k < d.v) return g.z_vector[k];
+    k -= d.v;
+    if (k < d.s) return g.gravity_potential[k];
+    k -= d.s;
+    return g.gravity_m[k];
+}
+
+int check_config(const game_config_t *c)
+{
+    if (c->momentum_diff_h || c->momentum_diff_v || c->mass_diff_h || c->mass_diff_v || c->temperature_diff_h
+        || c->temperature_diff_v || c->rad_on || c->pbl_scheme || c->sfc_sensible_heat_flux || c->sfc_phase_trans
+        || c->prog_soil_temp) {
+        snprintf(g_err, sizeof g_err, "a physics switch outside the RKHEVI dynamical-core path is on "
+                 "(diffusion, radiation, PBL, surface fluxes and soil must be off)");
+        return GAME_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+DevState *slot(int k) { return (k >= 0 && k < 3) ? &M->s[k] : nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char *game_gpu_last_error(void) { return g_err; }
+
+long long game_gpu_bytes_allocated(void) { return M ? M->bytes : 0; }
+
+int game_gpu_finalize(void)
+{
+    if (!M) return 0;
+    (void)hipStreamSynchronize(M->stream);
+    for (void *p : M->allocs) (void)hipFree(p);
+    if (M->h_partials) (void)hipHostFree(M->h_partials);
+    for (auto e : M->timer.pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(M->stream);
+    g_kernel_timer = nullptr;
+    delete M;
+    M = nullptr;
+    return 0;
+}
+
+int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game_dualgrid_view_t *dg,
+                  const game_config_t *config, int precision)
+{
+    g_err[0] = 0;
+    if (M) game_gpu_finalize();
+    if (!dims || !g || !dg || !config) { snprintf(g_err, sizeof g_err, "null argument"); return GAME_ERR_ARG; }
+    game_dims_t d = *dims;
+    game_dims_derive(&d);
+    if (d.n_layers < 3 || d.n_layers > GAME_MAX_LAYERS - 1 || g->no_of_oro_layers < 1
+        || g->no_of_oro_layers >= d.n_layers || (d.moisture_on != 0 && d.moisture_on != 1)) {
+        snprintf(g_err, sizeof g_err, "unsupported sizes: layers %d (3..63), orography layers %d (1..layers-1)",
+                 d.n_layers, g->no_of_oro_layers);
+        return GAME_ERR_ARG;
+    }
+    if (precision != GAME_FP64) {
+        snprintf(g_err, sizeof g_err, "precision mode %d is not available", precision);
+        return GAME_ERR_ARG;
+    }
+    int rc = check_config(config);
+    if (rc) return rc;
+    int ndev = 0;
+    HIP_OK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) { snprintf(g_err, sizeof g_err, "no HIP device"); return GAME_ERR_HIP; }
+    M = new Model();
+    M->d = d;
+    M->cfg = *config;
+    M->precision = precision;
+    HIP_OK(hipStreamCreate(&M->stream));
+    DevGrid &G = M->G;
+    G.nsh = d.nsh; G.nvh = d.nvh; G.nt = d.nt; G.vpl = d.vpl; G.dvpl = d.dvpl; G.L = d.n_layers;
+    G.n_oro = g->no_of_oro_layers; G.n_const = d.n_const; G.n_cond = d.n_cond; G.moist = d.moisture_on;
+    G.s = d.s; G.v = d.v; G.curl = d.curl; G.radius = g->radius; G.toa = g->toa;
+    size_t S = (size_t)d.s, V = (size_t)d.v, NVH = (size_t)d.nvh, NSH = (size_t)d.nsh, NT = (size_t)d.nt;
+#define UP(dst, src, n) do { if ((rc = upload(&(dst), (src), (n)))) return rc; } while (0)
+    UP(G.nd, g->normal_distance, V); UP(G.vol, g->volume, S); UP(G.area, g->area, V); UP(G.zs, g->z_scalar, S);
+    UP(G.zv, g->z_vector, V); UP(G.phi, g->gravity_potential, S); UP(G.gm, g->gravity_m, V);
+    UP(G.slope, g->slope, V); UP(G.thbg, g->theta_v_bg, S); UP(G.exbg, g->exner_bg, S);
+    UP(G.exbg_grad, g->exner_bg_grad, V); UP(G.ipw, g->inner_product_weights, 8*S);
+    UP(G.tw, g->trsk_weights, 10*NVH); UP(G.drw, g->density_to_rhombi_weights, 4*NVH);
+    UP(G.dnd, dg->normal_distance, (size_t)d.dual_v); UP(G.dzv, dg->z_vector, (size_t)d.dual_v);
+    UP(G.darea, dg->area, (size_t)d.curl); UP(G.fvec, dg->f_vec, 2*NVH);
+    UP(G.from, g->from_index, NVH); UP(G.to, g->to_index, NVH); UP(G.adj, g->adjacent_vector_indices_h, 6*NSH);
+    UP(G.sgn, g->adjacent_signs_h, 6*NSH); UP(G.ti, g->trsk_indices, 10*NVH);
+    UP(G.tmc, g->trsk_modified_curl_indices, 10*NVH); UP(G.dri, g->density_to_rhombi_indices, 4*NVH);
+    UP(G.dfrom, dg->from_index, NVH); UP(G.dto, dg->to_index, NVH);
+    UP(G.vit, dg->vorticity_indices_triangles, 3*NT); UP(G.vst, dg->vorticity_signs_triangles, 3*NT);
+    /* per-edge differences of calc_pot_vort's density extrapolation (vorticities.c:48-71) */
+    {
+        int L = d.n_layers;
+        long long Sll = d.s, vpl = d.vpl;
+        std::vector<double> t1(NVH), t2(NVH), s1(NVH), s2(NVH);
+        for (size_t e = 0; e < NVH; ++e) {
+            t1[e] = ref_z_vector_read(d, *g, Sll + (long long)e) - ref_z_vector_read(d, *g, Sll + vpl + (long long)e);
+            t2[e] = g->z_vector[0] - ref_z_vector_read(d, *g, Sll + (long long)e);
+            s1[e] = ref_z_vector_read(d, *g, Sll + (long long)(L - 2)*vpl + (long long)e)
+                    - ref_z_vector_read(d, *g, Sll + (long long)(L - 1)*vpl + (long long)e);
+            s2[e] = 0.5*(g->z_vector[(long long)L*vpl + g->from_index[e]] + g->z_vector[(long long)L*vpl + g->to_index[e]])
+                    - ref_z_vector_read(d, *g, Sll + (long long)(L - 1)*vpl + (long long)e);
+        }
+        UP(G.pv_top_dz1, t1.data(), NVH); UP(G.pv_top_dz2, t2.data(), NVH);
+        UP(G.pv_sfc_dz1, s1.data(), NVH); UP(G.pv_sfc_dz2, s2.data(), NVH);
+        HIP_OK(hipStreamSynchronize(M->stream));
+    }
+    /* Klemp damping coefficients (column_solvers.c:284-298): a grid constant */
+    {
+        int L = d.n_layers;
+        double dsh = config->damping_start_height_over_toa*g->z_vector[0];
+        std::vector<double> dmp((size_t)(L - 1)*NSH);
+        for (int j = 0; j < L - 1; ++j)
+            for (size_t i = 0; i < NSH; ++i) {
+                double zad = g->z_vector[i + (size_t)(j + 1)*d.vpl] - dsh, dc;
+                if (zad < 0.0) dc = 0.0;
+                else dc = config->damping_coeff_max*pow(sin(0.5*M_PI*zad/(g->z_vector[0] - dsh)), 2);
+                dmp[(size_t)j*NSH + i] = dc;
+            }
+        UP(G.damp, dmp.data(), dmp.size());
+        HIP_OK(hipStreamSynchronize(M->stream));
+    }
+#undef UP
+    for (int k = 0; k < 3; ++k) {
+        DevState &s = M->s[k];
+        if ((rc = dzeros(&s.rho, (size_t)d.n_const*S))) return rc;
+        if ((rc = dzeros(&s.rt, S))) return rc;
+        if ((rc = dzeros(&s.tp, S))) return rc;
+        if ((rc = dzeros(&s.ep, S))) return rc;
+        if ((rc = dzeros(&s.wind, V))) return rc;
+    }
+    DevWork &W = M->W;
+    double **vfields[] = {&W.flux, &W.flux2, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad, &W.pvt};
+    for (double **f : vfields) if ((rc = dzeros(f, V))) return rc;
+    double **sfields[] = {&W.flux_divv, &W.temp, &W.vsq, &W.phr, &W.sed_heat, &M->tmp_s};
+    for (double **f : sfields) if ((rc = dzeros(f, S))) return rc;
+    if ((rc = dzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
+    if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
+    if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+    if ((rc = dzeros(&M->partials, 2*4096))) return rc;
+    HIP_OK(hipHostMalloc((void **)&M->h_partials, sizeof(double)*2*4096, hipHostMallocDefault));
+    StepConsts &K = M->K;
+    K.impl_weight = config->impl_thermo_weight;
+    K.damping_start = config->damping_start_height_over_toa*g->z_vector[0];
+    K.damping_max = config->damping_coeff_max;
+    K.snow_v = config->snow_velocity;
+    K.rain_v = config->rain_velocity;
+    K.cloud_v = config->cloud_droplets_velocity;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+int game_gpu_set_config(const game_config_t *config)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    int rc = check_config(config);
+    if (rc) return rc;
+    M->cfg = *config;
+    M->K.impl_weight = config->impl_thermo_weight;
+    M->K.snow_v = config->snow_velocity;
+    M->K.rain_v = config->rain_velocity;
+    M->K.cloud_v = config->cloud_droplets_velocity;
+    return 0;
+}
+
+static int copy_state(int k, const game_state_view_t *h, int to_device)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    DevState *s = slot(k);
+    if (!s || !h) { snprintf(g_err, sizeof g_err, "bad state slot %d", k); return GAME_ERR_ARG; }
+    size_t S = (size_t)M->d.s*sizeof(double), V = (size_t)M->d.v*sizeof(double);
+    struct { double *dev; double *host; size_t n; } f[] = {
+        {s->rho, h->rho, S*(size_t)M->d.n_const}, {s->rt, h->rhotheta_v, S}, {s->tp, h->theta_v_pert, S},
+        {s->ep, h->exner_pert, S}, {s->wind, h->wind, V}};
+    for (auto &x : f) {
+        if (!x.host) continue;
+        if (to_device) HIP_OK(hipMemcpyAsync(x.dev, x.host, x.n, hipMemcpyHostToDevice, M->stream));
+        else HIP_OK(hipMemcpyAsync(x.host, x.dev, x.n, hipMemcpyDeviceToHost, M->stream));
+    }
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+int game_gpu_set_state(int k, const game_state_view_t *h) { return copy_state(k, h, 1); }
+int game_gpu_get_state(int k, game_state_view_t *h) { return copy_state(k, h, 0); }
+
+static int d2h(double *host, const double *dev, size_t n)
+{
+    if (host) HIP_OK(hipMemcpyAsync(host, dev, n*sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    return 0;
+}
+
+int game_gpu_get_forcings(game_forcings_view_t *h)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    size_t V = (size_t)M->d.v;
+    int rc;
+    if ((rc = d2h(h->pgrad_acc_old, M->W.pgrad_old, V))) return rc;
+    if ((rc = d2h(h->pressure_gradient_acc_neg_nl, M->W.pgrad_nl, V))) return rc;
+    if ((rc = d2h(h->pressure_gradient_acc_neg_l, M->W.pgrad_l, V))) return rc;
+    if ((rc = d2h(h->pressure_grad_condensates_v, M->W.pgc_v, V))) return rc;
+    if ((rc = d2h(h->v_squared_grad, M->W.vsq_grad, V))) return rc;
+    if ((rc = d2h(h->pot_vort_tend, M->W.pvt, V))) return rc;
+    if (h->radiation_tendency) memset(h->radiation_tendency, 0, sizeof(double)*(size_t)M->d.s);
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+int game_gpu_get_diagnostics(game_diag_view_t *h)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    int rc;
+    if ((rc = d2h(h->flux_density, M->W.flux, (size_t)M->d.v))) return rc;
+    if ((rc = d2h(h->flux_density_divv, M->W.flux_divv, (size_t)M->d.s))) return rc;
+    if ((rc = d2h(h->rel_vort_on_triangles, M->W.rvt, (size_t)M->d.dual_v_vectors))) return rc;
+    if ((rc = d2h(h->pot_vort, M->W.pv, (size_t)M->d.curl))) return rc;
+    if ((rc = d2h(h->temperature, M->W.temp, (size_t)M->d.s))) return rc;
+    if ((rc = d2h(h->v_squared, M->W.vsq, (size_t)M->d.s))) return rc;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+int game_gpu_get_irrev(game_irrev_view_t *h)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    int rc;
+    size_t S = (size_t)M->d.s;
+    if ((rc = d2h(h->condensates_sediment_heat, M->W.sed_heat, S))) return rc;
+    if ((rc = d2h(h->phase_trans_rates, M->W.ptr, (size_t)(M->d.n_cond + 1)*S))) return rc;
+    if ((rc = d2h(h->phase_trans_heating_rate, M->W.phr, S))) return rc;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+/* manage_rkhevi.c:20-100 on the device-resident states */
+int game_gpu_manage_rkhevi(double dt, int first)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    hipStream_t st = M->stream;
+    const DevGrid &G = M->G;
+    DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
+    DevWork &W = M->W;
+    M->K.dt = dt;
+    const double *rho_gas_new = sn.rho + (i64)G.n_cond*G.s;
+    fp64::temperature(G, so, W.temp, st);
+    if (G.moist) fp64::phase_trans(G, so, W, dt, st);
+    for (int rk = 0; rk < 2; ++rk) {
+        if (rk == 0) fp64::pgrad(G, sn, W, first, st);
+        else fp64::pgc_v(G, sn, W, st);
+        if (rk == 1 || first) fp64::advection(G, sn.wind, rho_gas_new, W, st);
+        fp64::vector_tend(G, W, td.wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
+        fp64::scalar_tend(G, sn, td, W, rk, st);
+        fp64::ver_waves(G, so, sn, td, M->K, rk, st);
+        if (G.n_const > 1) fp64::gen_densities(G, so, sn, td, W, M->K, rk, st);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int game_gpu_linear_combine_two_states(int k0, int k1, int ko, double c0, double c1)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    DevState *a = slot(k0), *b = slot(k1), *o = slot(ko);
+    if (!a || !b || !o) { snprintf(g_err, sizeof g_err, "bad state slot"); return GAME_ERR_ARG; }
+    fp64::linear_combine(M->G, *a, *b, *o, c0, c1, M->stream);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int game_gpu_run_steps(int n, double dt, int first)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    for (int k = 0; k < n; ++k) {
+        int rc = game_gpu_linear_combine_two_states(GAME_STATE_NEW, GAME_STATE_OLD, GAME_STATE_OLD, 1.0, 0.0);
+        if (rc) return rc;
+        rc = game_gpu_manage_rkhevi(dt, first);
+        if (rc) return rc;
+        first = 0;
+    }
+    return 0;
+}
+
+int game_gpu_temperature_diagnostics(int k)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    DevState *s = slot(k);
+    if (!s) return GAME_ERR_ARG;
+    fp64::temperature(M->G, *s, M->W.temp, M->stream);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+static int integral(i64 n, const double *x, const double *y, double *out)
+{
+    int blocks = fp64::integral_partials(n, x, y, M->G.vol, M->partials, M->stream);
+    HIP_OK(hipMemcpyAsync(M->h_partials, M->partials, sizeof(double)*2*(size_t)blocks, hipMemcpyDeviceToHost,
+                          M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        double v = M->h_partials[2*b], t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+        c += M->h_partials[2*b + 1];
+    }
+    *out = s + c;
+    return 0;
+}
+
+/* write_output.c:1515-1596 with compensated sums */
+int game_gpu_integrals(int k, double *out)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    DevState *s = slot(k);
+    if (!s || !out) return GAME_ERR_ARG;
+    const DevGrid &G = M->G;
+    int rc;
+    for (int c = 0; c < G.n_const; ++c)
+        if ((rc = integral(G.s, s->rho + (i64)c*G.s, nullptr, &out[c]))) return rc;
+    if ((rc = integral(G.s, s->rt, nullptr, &out[G.n_const]))) return rc;
+    const double *rg = s->rho + (i64)G.n_cond*G.s;
+    fp64::inner_product(G, s->wind, s->wind, M->tmp_s, M->stream);
+    if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 1]))) return rc;
+    out[G.n_const + 1] *= 0.5;
+    if ((rc = integral(G.s, rg, G.phi, &out[G.n_const + 2]))) return rc;
+    fp64::temperature(G, *s, M->tmp_s, M->stream);
+    if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 3]))) return rc;
+    out[G.n_const + 3] *= GC_C_D_V;
+    return 0;
+}
+
+/* v_squared and pot_vort_tend of a (wind, density) pair, the terms of the
+   initial hydrostatic balance (src/io/set_initial_state.c:155-166) */
+int game_gpu_init_balance_terms(const double *wind, const double *density, double *v_squared, double *pvt)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    DevState &t = M->s[GAME_STATE_TENDENCY];   /* used as scratch, cleared after */
+    size_t S = (size_t)M->d.s, V = (size_t)M->d.v;
+    HIP_OK(hipMemcpyAsync(t.wind, wind, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    HIP_OK(hipMemcpyAsync(t.rt, density, S*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    fp64::advection(M->G, t.wind, t.rt, M->W, M->stream);
+    HIP_OK(hipMemcpyAsync(v_squared, M->W.vsq, S*sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipMemcpyAsync(pvt, M->W.pvt, V*sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipMemsetAsync(t.wind, 0, V*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(t.rt, 0, S*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.pvt, 0, V*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.vsq, 0, S*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.vsq_grad, 0, V*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.flux, 0, V*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.pv, 0, (size_t)M->d.curl*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(M->W.rvt, 0, (size_t)M->d.dual_v_vectors*sizeof(double), M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+int game_gpu_enable_timing(int on)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    g_kernel_timer = on ? &M->timer : nullptr;
+    return 0;
+}
+
+int game_gpu_synchronize(void)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    M->timer.collect();
+    return 0;
+}
+
+int game_gpu_reset_timing(void)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    game_gpu_synchronize();
+    M->timer.stats.clear();
+    return 0;
+}
+
+int game_gpu_kernel_count(void)
+{
+    if (!M) return 0;
+    M->timer.collect();
+    return (int)M->timer.stats.size();
+}
+
+int game_gpu_kernel_stats(int k, const char **name, long long *launches, double *ms)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    M->timer.collect();
+    if (k < 0 || k >= (int)M->timer.stats.size()) return GAME_ERR_ARG;
+    *name = M->timer.stats[k].name.c_str();
+    *launches = M->timer.stats[k].launches;
+    *ms = M->timer.stats[k].ms;
+    return 0;
+}
+
+}  /* extern "C" */
