@@ -31,6 +31,7 @@ typedef long long i64;
 /* grid tables and metrics (struct grid / dualgrid members) on the device */
 struct DevGrid {
     int nsh, nvh, nt, vpl, dvpl, L, n_oro, n_const, n_cond, moist;
+    int flat;   /* slope == 0 everywhere: every terrain-correction term is an exact zero */
     i64 s, v, curl;
     double radius, toa;
     const double *nd, *vol, *area, *zs, *zv, *phi, *gm, *slope, *thbg, *exbg, *exbg_grad, *ipw;
@@ -55,6 +56,9 @@ struct DevWork {
     double *pgrad_old, *pgrad_nl, *pgrad_l, *pgc_v, *vsq_grad, *pvt;
     /* irreversible quantities */
     double *ptr, *phr, *sed_heat;
+    /* vertical contravariant corrections of wind, gas mass flux and rhotheta_v
+       flux in the terrain-following levels, 3 x n_oro x NSH (unused when flat) */
+    double *vcc;
 };
 
 struct StepConsts {

@@ -231,28 +231,50 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWork 
     W.pgc_v[w] = (0.5*(c_up + c_lo))*G.gm[w];
 }
 
-/* manage_pressure_gradient.c:17-72 at the edges */
+/* manage_pressure_gradient.c:17-72 at the edges: one thread per edge walks
+   all layers, so the edge's table entries are read once */
+template <bool FLAT>
+__device__ __forceinline__ double grad_edge_t(const DevGrid &G, const double *f, int l, int e, int fr, int to)
+{
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    double g = (f[to + (i64)l*G.nsh] - f[fr + (i64)l*G.nsh])/G.nd[vi];
+    if (!FLAT && l >= G.L - G.n_oro) {
+        const double *ipw = G.ipw;
+        double c = ipw[8*((i64)l*G.nsh + fr) + 6]*grad_w_raw(G, f, l, fr);
+        c += ipw[8*((i64)l*G.nsh + to) + 6]*grad_w_raw(G, f, l, to);
+        if (l < G.L - 1) {
+            c += ipw[8*((i64)l*G.nsh + fr) + 7]*grad_w_raw(G, f, l + 1, fr);
+            c += ipw[8*((i64)l*G.nsh + to) + 7]*grad_w_raw(G, f, l + 1, to);
+        }
+        c = 0.5*c;
+        g += -G.slope[vi]*c;
+    }
+    return g;
+}
+
+template <bool FLAT>
 __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWork W, int first_step)
 {
-    i64 n = (i64)G.L*G.nvh;
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= n) return;
-    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-    i64 vi = G.nsh + (i64)l*G.vpl + e;
-    if (!first_step) W.pgrad_old[vi] = -W.pgrad_nl[vi] - W.pgrad_l[vi];
-    i64 sf = (i64)l*G.nsh + G.from[e], st = (i64)l*G.nsh + G.to[e];
-    i64 gi = (i64)G.n_cond*G.s;
-    double g = grad_edge(G, s.ep, l, e);
-    double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
-    double nl = (0.5*(p1_t + p1_f))*g;
-    double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
-    double ll = (0.5*(p2_t + p2_f))*G.exbg_grad[vi];
-    double d_t = s.rho[gi + st]/density_total(G, s.rho, st), d_f = s.rho[gi + sf]/density_total(G, s.rho, sf);
-    nl = (0.5*(d_t + d_f))*nl;
-    ll = (0.5*(d_t + d_f))*ll;
-    W.pgrad_nl[vi] = nl;
-    W.pgrad_l[vi] = ll;
-    if (first_step) W.pgrad_old[vi] = -nl - ll;
+    int e = blockIdx.x*TPB + threadIdx.x;
+    if (e >= G.nvh) return;
+    const int fr = G.from[e], to = G.to[e];
+    const i64 gi = (i64)G.n_cond*G.s;
+    for (int l = 0; l < G.L; ++l) {
+        i64 vi = G.nsh + (i64)l*G.vpl + e;
+        if (!first_step) W.pgrad_old[vi] = -W.pgrad_nl[vi] - W.pgrad_l[vi];
+        i64 sf = (i64)l*G.nsh + fr, st = (i64)l*G.nsh + to;
+        double g = grad_edge_t<FLAT>(G, s.ep, l, e, fr, to);
+        double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
+        double nl = (0.5*(p1_t + p1_f))*g;
+        double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
+        double ll = (0.5*(p2_t + p2_f))*G.exbg_grad[vi];
+        double d_t = s.rho[gi + st]/density_total(G, s.rho, st), d_f = s.rho[gi + sf]/density_total(G, s.rho, sf);
+        nl = (0.5*(d_t + d_f))*nl;
+        ll = (0.5*(d_t + d_f))*ll;
+        W.pgrad_nl[vi] = nl;
+        W.pgrad_l[vi] = ll;
+        if (first_step) W.pgrad_old[vi] = -nl - ll;
+    }
 }
 
 /* scalar_times_vector(rho_gas, wind) into the mass flux density (multiplications.c:35-117) */
@@ -274,45 +296,62 @@ __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_
     }
 }
 
-/* vorticities.c:167-222 */
+/* vorticities.c:167-222, one thread per triangle walking all layers */
 __global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, double *res)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= (i64)G.L*G.nt) return;
-    int l = (int)(t/G.nt), h = (int)(t - (i64)l*G.nt);
-    double r = 0;
-    for (int j = 0; j < 3; ++j) {
-        i64 vi = G.nsh + (i64)l*G.vpl + G.vit[3*h + j];
-        double vel = w[vi], lrf = 1;
-        if (l >= G.L - G.n_oro) {
-            i64 di = G.nvh + (i64)l*G.dvpl + h;
-            lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
-            double delta_z = G.dzv[di] - G.zv[vi];
-            i64 ivg;
-            if (delta_z > 0) ivg = vi - G.vpl;
-            else if (l == G.L - 1) ivg = vi - G.vpl;
-            else ivg = vi + G.vpl;
-            double vg = (w[vi] - w[ivg])/(G.zv[vi] - G.zv[ivg]);
-            vel += delta_z*vg;
+    int h = blockIdx.x*TPB + threadIdx.x;
+    if (h >= G.nt) return;
+    int vit[3], vst[3];
+    for (int j = 0; j < 3; ++j) { vit[j] = G.vit[3*h + j]; vst[j] = G.vst[3*h + j]; }
+    for (int l = 0; l < G.L; ++l) {
+        double r = 0;
+        for (int j = 0; j < 3; ++j) {
+            i64 vi = G.nsh + (i64)l*G.vpl + vit[j];
+            double vel = w[vi], lrf = 1;
+            if (l >= G.L - G.n_oro) {
+                i64 di = G.nvh + (i64)l*G.dvpl + h;
+                lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
+                double delta_z = G.dzv[di] - G.zv[vi];
+                i64 ivg;
+                if (delta_z > 0) ivg = vi - G.vpl;
+                else if (l == G.L - 1) ivg = vi - G.vpl;
+                else ivg = vi + G.vpl;
+                double vg = (w[vi] - w[ivg])/(G.zv[vi] - G.zv[ivg]);
+                vel += delta_z*vg;
+            }
+            r += lrf*G.nd[vi]*vst[j]*vel;
         }
-        r += lrf*G.nd[vi]*G.vst[3*h + j]*vel;
+        res[(i64)l*G.nt + h] = r/G.darea[G.nvh + (i64)l*G.dvpl + h];
     }
-    res[t] = r/G.darea[G.nvh + (i64)l*G.dvpl + h];
 }
 
-/* tangential relative vorticity of edge e at level l (vorticities.c:131-156) */
-__device__ __forceinline__ double rel_vort_tangential(const DevGrid &G, const double *w, int l, int e)
+/* averaging.c:97-111 */
+template <bool FLAT>
+__device__ __forceinline__ double horizontal_covariant_t(const DevGrid &G, const double *w, int l, int e)
+{
+    i64 vi = G.nsh + (i64)l*G.vpl + e;
+    double r = w[vi];
+    if (!FLAT && l >= G.L - G.n_oro) {
+        double vc = remap_verpri2horpri(G, w, l, e);
+        r += G.slope[vi]*vc;
+    }
+    return r;
+}
+
+/* tangential relative vorticity of edge e at level l >= 1 (vorticities.c:131-156) */
+template <bool FLAT>
+__device__ __forceinline__ double rel_vort_tangential(const DevGrid &G, const double *w, int l, int e, int fr, int to)
 {
     i64 base = (i64)l*G.vpl;
     const double *nd = G.nd;
     if (l == G.L) {
         i64 i2 = base - G.nvh + e;
-        double cv2 = horizontal_covariant(G, w, l - 1, e);
+        double cv2 = horizontal_covariant_t<FLAT>(G, w, l - 1, e);
         return 1/G.darea[e + (i64)l*G.dvpl]*nd[i2]*cv2;
     }
-    i64 i0 = base + G.nsh + e, i1 = base + G.from[e], i2 = base - G.nvh + e, i3 = base + G.to[e];
-    double cv0 = horizontal_covariant(G, w, l, e);
-    double cv2 = horizontal_covariant(G, w, l - 1, e);
+    i64 i0 = base + G.nsh + e, i1 = base + fr, i2 = base - G.nvh + e, i3 = base + to;
+    double cv0 = horizontal_covariant_t<FLAT>(G, w, l, e);
+    double cv2 = horizontal_covariant_t<FLAT>(G, w, l - 1, e);
     return 1/G.darea[e + (i64)l*G.dvpl]*(
         - nd[i0]*cv0
         + nd[i1]*w[i1]
@@ -321,134 +360,155 @@ __device__ __forceinline__ double rel_vort_tangential(const DevGrid &G, const do
 }
 
 /* calc_pot_vort = calc_rel_vort + add_f_to_rel_vort + division by the
-   interpolated density (vorticities.c:18-165), one thread per curl element */
+   interpolated density (vorticities.c:18-165).  One thread per edge walks all
+   levels and writes the tangential element of each level and the rhombus
+   element of each layer. */
+template <bool FLAT>
 __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, const double *rho, const double *rvt,
                                                   double *pv)
 {
-    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (i >= G.curl) return;
-    int nvh = G.nvh, nsh = G.nsh;
-    int l = (int)(i/(2*nvh)), h = (int)(i - (i64)l*2*nvh);
-    double rv, dv;
-    const int *fr = G.from, *to = G.to;
-    if (h >= nvh) {
-        int e = h - nvh;
-        i64 base = nvh + (i64)l*G.dvpl;
-        int fd = G.dfrom[e], td = G.dto[e];
-        rv = (G.darea[base + fd]*rvt[(i64)l*G.nt + fd] + G.darea[base + td]*rvt[(i64)l*G.nt + td])
-             /(G.darea[base + fd] + G.darea[base + td]);
-        dv = 0;
-        for (int j = 0; j < 4; ++j) dv += G.drw[4*e + j]*rho[(i64)l*nsh + G.dri[4*e + j]];
-    } else {
-        /* the top level copies the tangential vorticity of the level below (vorticities.c:158-163) */
-        rv = rel_vort_tangential(G, w, l == 0 ? 1 : l, h);
+    int e = blockIdx.x*TPB + threadIdx.x;
+    if (e >= G.nvh) return;
+    const int nvh = G.nvh, nsh = G.nsh, L = G.L;
+    const int fr = G.from[e], to = G.to[e], fd = G.dfrom[e], td = G.dto[e];
+    int dri[4];
+    double drw[4];
+    for (int j = 0; j < 4; ++j) { dri[j] = G.dri[4*e + j]; drw[j] = G.drw[4*e + j]; }
+    const double f_t = G.fvec[e], f_r = G.fvec[nvh + e];
+    double rv1 = rel_vort_tangential<FLAT>(G, w, 1, e, fr, to);
+    for (int l = 0; l <= L; ++l) {
+        /* tangential element at level l; the top copies level 1 (vorticities.c:158-163) */
+        double rv = (l <= 1) ? rv1 : rel_vort_tangential<FLAT>(G, w, l, e, fr, to);
+        double dv;
         if (l == 0) {
-            dv = 0.5*(rho[fr[h]] + rho[to[h]])
-                 + (0.5*(rho[fr[h]] + rho[to[h]]) - 0.5*(rho[fr[h] + nsh] + rho[to[h] + nsh]))
-                 /G.pv_top_dz1[h]*G.pv_top_dz2[h];
-        } else if (l == G.L) {
+            dv = 0.5*(rho[fr] + rho[to])
+                 + (0.5*(rho[fr] + rho[to]) - 0.5*(rho[fr + nsh] + rho[to + nsh]))
+                 /G.pv_top_dz1[e]*G.pv_top_dz2[e];
+        } else if (l == L) {
             i64 a = (i64)(l - 1)*nsh, b = (i64)(l - 2)*nsh;
-            dv = 0.5*(rho[a + fr[h]] + rho[a + to[h]])
-                 + (0.5*(rho[b + fr[h]] + rho[b + to[h]]) - 0.5*(rho[a + fr[h]] + rho[a + to[h]]))
-                 /G.pv_sfc_dz1[h]*G.pv_sfc_dz2[h];
+            dv = 0.5*(rho[a + fr] + rho[a + to])
+                 + (0.5*(rho[b + fr] + rho[b + to]) - 0.5*(rho[a + fr] + rho[a + to]))
+                 /G.pv_sfc_dz1[e]*G.pv_sfc_dz2[e];
         } else {
-            i64 uf = (i64)(l - 1)*nsh + fr[h], ut = (i64)(l - 1)*nsh + to[h];
+            i64 uf = (i64)(l - 1)*nsh + fr, ut = (i64)(l - 1)*nsh + to;
             dv = 0.25*(rho[uf] + rho[ut] + rho[uf + nsh] + rho[ut + nsh]);
         }
+        pv[(i64)l*2*nvh + e] = (rv + f_t)/dv;
+        if (l == L) break;
+        /* rhombus element of layer l */
+        i64 base = nvh + (i64)l*G.dvpl;
+        double af = G.darea[base + fd], at = G.darea[base + td];
+        double rr = (af*rvt[(i64)l*G.nt + fd] + at*rvt[(i64)l*G.nt + td])/(af + at);
+        double dr = 0;
+        for (int j = 0; j < 4; ++j) dr += drw[j]*rho[(i64)l*nsh + dri[j]];
+        pv[(i64)l*2*nvh + nvh + e] = (rr + f_r)/dr;
     }
-    pv[i] = (rv + G.fvec[h])/dv;
 }
 
-/* vorticity_flux.c:16-187, horizontal part (one thread per edge and layer) */
+/* vorticity_flux.c:16-187, horizontal part: one thread per edge, all layers */
 __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const double *F, const double *pv, double *out)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= (i64)G.L*G.nvh) return;
-    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-    int nsh = G.nsh, nvh = G.nvh;
-    i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
-    const double *tw = G.tw + 10*(i64)e;
-    const int *ti = G.ti + 10*(i64)e, *tm = G.tmc + 10*(i64)e;
-    int fr = G.from[e], to = G.to[e];
-    double o = 0;
-    if (fr < GAME_N_PENT) {
-        for (int j = 0; j < 4; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
-    } else {
-        for (int j = 0; j < 5; ++j) {
-            if (j == 2) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
-            else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
-        }
-    }
-    if (to < GAME_N_PENT) {
-        for (int j = 5; j < 9; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
-    } else {
-        for (int j = 5; j < 10; ++j) {
-            if (j == 7) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
-            else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
-        }
-    }
+    int e = blockIdx.x*TPB + threadIdx.x;
+    if (e >= G.nvh) return;
+    const int nsh = G.nsh, nvh = G.nvh;
+    double tw[10];
+    int ti[10], tm[10];
+    for (int j = 0; j < 10; ++j) { tw[j] = G.tw[10*(i64)e + j]; ti[j] = G.ti[10*(i64)e + j]; tm[j] = G.tmc[10*(i64)e + j]; }
+    const int fr = G.from[e], to = G.to[e];
+    const bool pent_f = fr < GAME_N_PENT, pent_t = to < GAME_N_PENT;
     const double *ipw = G.ipw;
-    i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
-    double q0 = pv[e + (i64)l*2*nvh], q1 = pv[e + (i64)(l + 1)*2*nvh];
-    o -= 0.5*ipw[8*sf + 6]*F[(i64)l*G.vpl + fr]*q0;
-    o -= 0.5*ipw[8*st + 6]*F[(i64)l*G.vpl + to]*q0;
-    o -= 0.5*ipw[8*sf + 7]*F[(i64)(l + 1)*G.vpl + fr]*q1;
-    o -= 0.5*ipw[8*st + 7]*F[(i64)(l + 1)*G.vpl + to]*q1;
-    out[G.nsh + (i64)l*G.vpl + e] = o;
+    for (int l = 0; l < G.L; ++l) {
+        i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
+        double o = 0;
+        if (pent_f) {
+            for (int j = 0; j < 4; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+        } else {
+            for (int j = 0; j < 5; ++j) {
+                if (j == 2) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
+                else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+            }
+        }
+        if (pent_t) {
+            for (int j = 5; j < 9; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+        } else {
+            for (int j = 5; j < 10; ++j) {
+                if (j == 7) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
+                else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+            }
+        }
+        i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
+        double q0 = pv[e + (i64)l*2*nvh], q1 = pv[e + (i64)(l + 1)*2*nvh];
+        o -= 0.5*ipw[8*sf + 6]*F[(i64)l*G.vpl + fr]*q0;
+        o -= 0.5*ipw[8*st + 6]*F[(i64)l*G.vpl + to]*q0;
+        o -= 0.5*ipw[8*sf + 7]*F[(i64)(l + 1)*G.vpl + fr]*q1;
+        o -= 0.5*ipw[8*st + 7]*F[(i64)(l + 1)*G.vpl + to]*q1;
+        out[nsh + (i64)l*G.vpl + e] = o;
+    }
 }
 
-/* vorticity_flux.c:373-417, vertical part (levels 0..L) */
+/* vorticity_flux.c:373-417, vertical part: one thread per cell, levels 0..L */
 __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const double *F, const double *pv, double *out)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= (i64)(G.L + 1)*G.nsh) return;
-    int l = (int)(t/G.nsh), h = (int)(t - (i64)l*G.nsh);
-    int nsh = G.nsh, nvh = G.nvh;
-    int ne = h < GAME_N_PENT ? 5 : 6;
-    double vw = 0.5;
-    if (l == 0 || l == G.L) vw = 1;
-    double o = 0;
-    const int *adj = G.adj + 6*(i64)h;
-    if (l >= 1)
-        for (int j = 0; j < ne; ++j)
-            o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*F[nsh + (i64)(l - 1)*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
-    if (l <= G.L - 1)
-        for (int j = 0; j < ne; ++j)
-            o += vw*G.ipw[8*((i64)l*nsh + h) + j]*F[nsh + (i64)l*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
-    out[(i64)l*G.vpl + h] = o;
+    int h = blockIdx.x*TPB + threadIdx.x;
+    if (h >= G.nsh) return;
+    const int nsh = G.nsh, nvh = G.nvh;
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    int adj[6];
+    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
+    for (int l = 0; l <= G.L; ++l) {
+        double vw = 0.5;
+        if (l == 0 || l == G.L) vw = 1;
+        double o = 0;
+        if (l >= 1)
+            for (int j = 0; j < ne; ++j)
+                o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*F[nsh + (i64)(l - 1)*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
+        if (l <= G.L - 1)
+            for (int j = 0; j < ne; ++j)
+                o += vw*G.ipw[8*((i64)l*nsh + h) + j]*F[nsh + (i64)l*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
+        out[(i64)l*G.vpl + h] = o;
+    }
 }
 
-/* inner_product.c:13-43 */
+/* inner_product.c:13-43, one thread per cell, all layers */
 __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *a, const double *b, double *out)
 {
-    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (i >= G.s) return;
-    int l = (int)(i/G.nsh), h = (int)(i - (i64)l*G.nsh);
-    int ne = h < GAME_N_PENT ? 5 : 6;
-    const double *w = G.ipw + 8*i;
-    double o = 0;
-    for (int j = 0; j < ne; ++j) {
-        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
-        o += w[j]*a[vi]*b[vi];
+    int h = blockIdx.x*TPB + threadIdx.x;
+    if (h >= G.nsh) return;
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    int adj[6];
+    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
+    for (int l = 0; l < G.L; ++l) {
+        i64 i = (i64)l*G.nsh + h;
+        const double *w = G.ipw + 8*i;
+        double o = 0;
+        for (int j = 0; j < ne; ++j) {
+            i64 vi = G.nsh + (i64)l*G.vpl + adj[j];
+            o += w[j]*a[vi]*b[vi];
+        }
+        o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
+        o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
+        out[i] = o;
     }
-    o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
-    o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
-    out[i] = o;
 }
 
-/* grad (gradient_operators.c:56-74) of a scalar field: edges and interior w points */
-__global__ void __launch_bounds__(TPB) k_grad(DevGrid G, const double *f, double *out)
+/* grad (gradient_operators.c:56-74) of a scalar field: edges (one thread per
+   edge, all layers) */
+template <bool FLAT>
+__global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const double *f, double *out)
+{
+    int e = blockIdx.x*TPB + threadIdx.x;
+    if (e >= G.nvh) return;
+    const int fr = G.from[e], to = G.to[e];
+    for (int l = 0; l < G.L; ++l) out[G.nsh + (i64)l*G.vpl + e] = grad_edge_t<FLAT>(G, f, l, e, fr, to);
+}
+
+/* grad, interior w points */
+__global__ void __launch_bounds__(TPB) k_grad_w(DevGrid G, const double *f, double *out)
 {
     i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
-    if (t < ne) {
-        int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-        out[G.nsh + (i64)l*G.vpl + e] = grad_edge(G, f, l, e);
-    } else if (t < ne + nw) {
-        t -= ne;
-        int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
-        out[h + (i64)lev*G.vpl] = grad_w_raw(G, f, lev, h);
-    }
+    if (t >= (i64)(G.L - 1)*G.nsh) return;
+    int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
+    out[h + (i64)lev*G.vpl] = grad_w_raw(G, f, lev, h);
 }
 
 /* vector_tendencies_expl.c:74-131 fused with the horizontal wind update of
@@ -481,77 +541,157 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWork W, doubl
     }
 }
 
-/* scalar_tendencies_expl.c:102-111, 141-150 at the edges: centred mass flux and
-   the rhotheta_v flux of the main gas */
-__global__ void __launch_bounds__(TPB) k_scalar_flux_main(DevGrid G, DevState s, double *flux, double *flux2)
+/* Vertical contravariant corrections (averaging.c:41-95) of the three fields
+   divv_h and divv_h_tracer need in the terrain-following levels L-n_oro..L-1:
+   the wind (tracers, divergences.c:94-145), the gas mass flux and its
+   rhotheta_v flux (divergences.c:41-59); the fluxes are formed on the fly
+   exactly as scalar_times_vector_h forms them (multiplications.c:45-67). */
+__global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
 {
     i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= (i64)G.L*G.nvh) return;
-    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-    i64 vi = G.nsh + (i64)l*G.vpl + e;
-    i64 sf = (i64)l*G.nsh + G.from[e], st = (i64)l*G.nsh + G.to[e];
+    const i64 n = (i64)G.n_oro*G.nsh;
+    if (t >= n) return;
+    const int nsh = G.nsh, L = G.L;
+    int lev = L - G.n_oro + (int)(t/nsh), h = (int)(t % nsh);
+    int ne = h < GAME_N_PENT ? 5 : 6;
     const double *rg = s.rho + (i64)G.n_cond*G.s;
-    double f1 = (0.5*(rg[st] + rg[sf]))*s.wind[vi];
-    double th_t = s.rt[st]/rg[st], th_f = s.rt[sf]/rg[sf];
-    flux[vi] = f1;
-    flux2[vi] = (0.5*(th_t + th_f))*f1;
-}
-
-/* scalar_tendencies_expl.c:119-177 for the main gas: rho and rhotheta_v tendencies */
-__global__ void __launch_bounds__(TPB) k_scalar_tend_main(DevGrid G, DevState s, DevWork W, double *tend_rho,
-                                                          double *tend_rt, double new_w)
-{
-    i64 j = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (j >= G.s) return;
-    int l = (int)(j/G.nsh), h = (int)(j - (i64)l*G.nsh);
-    double old_w = 1.0 - new_w;
-    i64 shift = (i64)G.n_cond*G.s;
-    double div1 = divv_h_cell(G, W.flux, l, h);
-    double div2 = divv_h_cell(G, W.flux2, l, h);
-    double rg = s.rho[shift + j];
-    double th = s.rt[j]/rg;
-    if (!G.moist) {
-        /* dry: every source term of the reference sum is a zero array */
-        tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div1);
-        tend_rt[j] = old_w*tend_rt[j] + new_w*(-div2);
-        return;
+    double rw = 0, r1 = 0, r2 = 0;
+    for (int pass = (lev == L - G.n_oro) ? 1 : 0; pass < 2; ++pass) {
+        int l = pass == 0 ? lev - 1 : lev;
+        for (int i = 0; i < ne; ++i) {
+            int e = G.adj[6*h + i];
+            i64 si = (i64)l*nsh + h, vi = nsh + (i64)l*G.vpl + e;
+            i64 st = (i64)l*nsh + G.to[e], sf = (i64)l*nsh + G.from[e];
+            double u = s.wind[vi];
+            double f1 = (0.5*(rg[st] + rg[sf]))*u;
+            double f2 = (0.5*(s.rt[st]/rg[st] + s.rt[sf]/rg[sf]))*f1;
+            double c = -0.5*G.ipw[8*si + i]*G.slope[vi];
+            rw += c*u;
+            r1 += c*f1;
+            r2 += c*f2;
+        }
     }
-    /* the moist-air gas takes the vapour phase-transition rate (index n_cond of
-       phase_trans_rates, scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
-    double ptr_v = W.ptr[shift + j];
-    tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div1 + ptr_v);
-    double cv = c_v_mass_weighted(G, s.rho, W.temp[j], j);
-    double heat = W.phr[j] + W.sed_heat[j];
-    tend_rt[j] = old_w*tend_rt[j] + new_w*(
-        -div2
-        + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[j] + s.ep[j]))
-        + ptr_v*th);
+    vcc[t] = rw;
+    vcc[n + t] = r1;
+    vcc[2*n + t] = r2;
 }
 
-/* upstream tracer flux (multiplications.c:69-96) */
-__global__ void __launch_bounds__(TPB) k_tracer_flux(DevGrid G, const double *rho_k, const double *wind, double *flux)
+/* scalar_tendencies_expl.c:89-177 for every constituent (diffusion off): the
+   centred gas mass flux and its rhotheta_v flux, the upstream tracer fluxes,
+   their divergences (divergences.c:14-148) and the RK-weighted tendencies.
+   One thread per cell walks all layers; the edge fluxes are formed on the fly
+   from the reference's own formulas, so no flux array is written. */
+template <bool FLAT>
+__global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevWork W, double *td_rho,
+                                                     double *td_rt, int rk)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (t >= (i64)G.L*G.nvh) return;
-    int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-    i64 vi = G.nsh + (i64)l*G.vpl + e;
-    double u = wind[vi];
-    double sv = (u >= 0.0) ? rho_k[G.from[e] + (i64)l*G.nsh] : rho_k[G.to[e] + (i64)l*G.nsh];
-    flux[vi] = sv*u;
-}
-
-/* tracer tendency (scalar_tendencies_expl.c:113-133) */
-__global__ void __launch_bounds__(TPB) k_tracer_tend(DevGrid G, DevState s, DevWork W, double *tend_rho, int k,
-                                                     double new_w)
-{
-    i64 j = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (j >= G.s) return;
-    int l = (int)(j/G.nsh), h = (int)(j - (i64)l*G.nsh);
-    double old_w = 1.0 - new_w;
-    i64 shift = (i64)k*G.s;
-    i64 shift_pt = (k == G.n_cond + 1) ? shift - G.s : shift;
-    double div = divv_h_tracer_cell(G, W.flux, s.rho + shift, s.wind, l, h);
-    tend_rho[shift + j] = old_w*tend_rho[shift + j] + new_w*(-div + W.ptr[shift_pt + j]);
+    int h = blockIdx.x*TPB + threadIdx.x;
+    if (h >= G.nsh) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro, ncc = G.n_cond, nc = G.n_const;
+    const i64 S = G.s, nvcc = (i64)no*nsh;
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    int ej[6], fj[6], tj[6];
+    double sj[6];
+    for (int j = 0; j < 6; ++j) {
+        ej[j] = G.adj[6*h + j];
+        sj[j] = G.sgn[6*h + j];
+        fj[j] = G.from[ej[j]];
+        tj[j] = G.to[ej[j]];
+    }
+    for (int l = 0; l < L; ++l) {
+        const i64 i = (i64)l*nsh + h;
+        double a[6], u[6];
+        for (int j = 0; j < ne; ++j) {
+            i64 vi = nsh + (i64)l*vpl + ej[j];
+            a[j] = G.area[vi];
+            u[j] = s.wind[vi];
+        }
+        /* which faces carry a terrain correction (divergences.c:41-59) */
+        int has_u = 0, has_l = 0;
+        if (!FLAT) {
+            if (l == L - no - 1) has_l = 1;
+            else if (l == L - 1) has_u = 1;
+            else if (l > L - no - 1) { has_u = 1; has_l = 1; }
+        }
+        const double vol = G.vol[i];
+        const double area_u = has_u ? G.area[h + (i64)l*vpl] : 0.0;
+        const double area_l = has_l ? G.area[h + (i64)(l + 1)*vpl] : 0.0;
+        const i64 vu = (i64)(l - (L - no))*nsh + h, vl = (i64)(l + 1 - (L - no))*nsh + h;
+        for (int k = 0; k < nc; ++k) {
+            const i64 shift = (i64)k*S;
+            const double new_w = (rk == 1 && k != ncc) ? 0.5 : 1.0, old_w = 1.0 - new_w;
+            const double *rk_ = s.rho + shift;
+            if (k == ncc) {
+                double c1 = 0.0, c2 = 0.0;
+                for (int j = 0; j < ne; ++j) {
+                    i64 st = (i64)l*nsh + tj[j], sf = (i64)l*nsh + fj[j];
+                    double f1 = (0.5*(rk_[st] + rk_[sf]))*u[j];
+                    double f2 = (0.5*(s.rt[st]/rk_[st] + s.rt[sf]/rk_[sf]))*f1;
+                    c1 += f1*sj[j]*a[j];
+                    c2 += f2*sj[j]*a[j];
+                }
+                double v1 = 0.0, v2 = 0.0;
+                if (!FLAT) {
+                    if (has_u && has_l) {
+                        v1 = W.vcc[nvcc + vu]*area_u - W.vcc[nvcc + vl]*area_l;
+                        v2 = W.vcc[2*nvcc + vu]*area_u - W.vcc[2*nvcc + vl]*area_l;
+                    } else if (has_l) {
+                        v1 = -W.vcc[nvcc + vl]*area_l;
+                        v2 = -W.vcc[2*nvcc + vl]*area_l;
+                    } else if (has_u) {
+                        v1 = W.vcc[nvcc + vu]*area_u;
+                        v2 = W.vcc[2*nvcc + vu]*area_u;
+                    }
+                }
+                double div1 = 1.0/vol*(c1 + v1);
+                double div2 = 1.0/vol*(c2 + v2);
+                double rg = rk_[i];
+                if (!G.moist) {
+                    /* dry: every source term of the reference sum is a zero array */
+                    td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1);
+                    td_rt[i] = old_w*td_rt[i] + new_w*(-div2);
+                } else {
+                    /* the moist-air gas takes the vapour phase-transition rate
+                       (scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
+                    double ptr_v = W.ptr[shift + i];
+                    double th = s.rt[i]/rg;
+                    td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
+                    double cv = c_v_mass_weighted(G, s.rho, W.temp[i], i);
+                    double heat = W.phr[i] + W.sed_heat[i];
+                    td_rt[i] = old_w*td_rt[i] + new_w*(
+                        -div2
+                        + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[i] + s.ep[i]))
+                        + ptr_v*th);
+                }
+            } else {
+                double c = 0.0;
+                for (int j = 0; j < ne; ++j) {
+                    double sv = (u[j] >= 0.0) ? rk_[(i64)l*nsh + fj[j]] : rk_[(i64)l*nsh + tj[j]];
+                    c += (sv*u[j])*sj[j]*a[j];
+                }
+                double v = 0.0;
+                if (!FLAT) {
+                    if (has_u && has_l) {
+                        double cu = W.vcc[vu], cl = W.vcc[vl];
+                        double du = (cu <= 0.0) ? rk_[i - nsh] : rk_[i];
+                        double dl = (cl <= 0.0) ? rk_[i] : rk_[i + nsh];
+                        v = du*cu*area_u - dl*cl*area_l;
+                    } else if (has_l) {
+                        double cl = W.vcc[vl];
+                        double dl = (cl <= 0.0) ? rk_[i] : rk_[i + nsh];
+                        v = -dl*cl*area_l;
+                    } else if (has_u) {
+                        double cu = W.vcc[vu];
+                        double du = (cu <= 0.0) ? rk_[i - nsh] : rk_[i];
+                        v = du*cu*area_u;
+                    }
+                }
+                double div = 1.0/vol*(c + v);
+                i64 shift_pt = (k == ncc + 1) ? shift - S : shift;
+                td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div + W.ptr[shift_pt + i]);
+            }
+        }
+    }
 }
 
 /* three_band_solver_ver_waves (column_solvers.c:19-383), one column per lane.
@@ -712,11 +852,10 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
 /* three_band_solver_gen_densities (column_solvers.c:385-639) for tracer k,
    one column per lane */
 template <int LT>
-__global__ void __launch_bounds__(128) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td, DevWork W,
-                                                       StepConsts K, int rk, int k)
+__device__ __forceinline__ void gen_densities_column(const DevGrid &G, const DevState &so, const DevState &sn,
+                                                     const DevState &td, const DevWork &W, const StepConsts &K,
+                                                     int rk, int k, int i)
 {
-    int i = blockIdx.x*128 + threadIdx.x;
-    if (i >= G.nsh) return;
     const int L = LT > 0 ? LT : G.L;
     const int nsh = G.nsh, vpl = G.vpl, ncc = G.n_cond;
     const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
@@ -803,6 +942,18 @@ __global__ void __launch_bounds__(128) k_gen_densities(DevGrid G, DevState so, D
         if (v < 0.0) v = 0.0;
         rho_n[i + (i64)j*nsh] = v;
     }
+}
+
+/* all tracers of a column in the reference's constituent order, so the
+   sediment heat accumulates in the same order (column_solvers.c:393-637) */
+template <int LT>
+__global__ void __launch_bounds__(128) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td, DevWork W,
+                                                       StepConsts K, int rk)
+{
+    int i = blockIdx.x*128 + threadIdx.x;
+    if (i >= G.nsh) return;
+    for (int k = 0; k < G.n_const; ++k)
+        if (k != G.n_cond) gen_densities_column<LT>(G, so, sn, td, W, K, rk, k, i);
 }
 
 /* phase_trans.c:16-311 (surface phase transitions off), one thread per grid box */
@@ -1016,13 +1167,19 @@ __global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *
 /* ------------------------------------------------------------------ */
 namespace fp64 {
 
+#define FLAT_DISPATCH(NAME, KERNEL, GRIDDIM, ...)                                                 \
+    do {                                                                                      \
+        if (G.flat) LAUNCH(NAME, KERNEL<true>, GRIDDIM, dim3(TPB), st, __VA_ARGS__);          \
+        else LAUNCH(NAME, KERNEL<false>, GRIDDIM, dim3(TPB), st, __VA_ARGS__);                \
+    } while (0)
+
 void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st)
 { LAUNCH("k_temperature", k_temperature, GRID1(G.s), dim3(TPB), st, G, s, T); }
 
 void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
 {
     LAUNCH("k_pgrad_w", k_pgrad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-    LAUNCH("k_pgrad_e", k_pgrad_e, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s, W, first);
+    FLAT_DISPATCH("k_pgrad_e", k_pgrad_e, GRID1(G.nvh), G, s, W, first);
 }
 
 void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st)
@@ -1031,16 +1188,17 @@ void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)
 {
     LAUNCH("k_mass_flux", k_mass_flux, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas, wind, W.flux);
-    LAUNCH("k_rel_vort_tri", k_rel_vort_tri, GRID1((i64)G.L*G.nt), dim3(TPB), st, G, wind, W.rvt);
-    LAUNCH("k_pot_vort", k_pot_vort, GRID1(G.curl), dim3(TPB), st, G, wind, rho_gas, W.rvt, W.pv);
-    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID1((i64)(G.L + 1)*G.nsh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_inner_product", k_inner_product, GRID1(G.s), dim3(TPB), st, G, wind, wind, W.vsq);
-    LAUNCH("k_grad", k_grad, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, W.vsq, W.vsq_grad);
+    LAUNCH("k_rel_vort_tri", k_rel_vort_tri, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
+    FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID1(G.nvh), G, wind, rho_gas, (const double *)W.rvt, W.pv);
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e, GRID1(G.nvh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
+    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID1(G.nsh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
+    LAUNCH("k_inner_product", k_inner_product, GRID1(G.nsh), dim3(TPB), st, G, wind, wind, W.vsq);
+    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, (const double *)W.vsq, W.vsq_grad);
+    LAUNCH("k_grad_w", k_grad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, W.vsq, W.vsq_grad);
 }
 
 void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
-{ LAUNCH("k_inner_product", k_inner_product, GRID1(G.s), dim3(TPB), st, G, a, b, out); }
+{ LAUNCH("k_inner_product", k_inner_product, GRID1(G.nsh), dim3(TPB), st, G, a, b, out); }
 
 void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
                  double new_w, double itw, double dt, hipStream_t st)
@@ -1051,16 +1209,9 @@ void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const doubl
 
 void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st)
 {
-    for (int k = 0; k < G.n_const; ++k) {
-        double new_w = (rk == 1 && k != G.n_cond) ? 0.5 : 1.0;
-        if (k == G.n_cond) {
-            LAUNCH("k_scalar_flux_main", k_scalar_flux_main, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s, W.flux, W.flux2);
-            LAUNCH("k_scalar_tend_main", k_scalar_tend_main, GRID1(G.s), dim3(TPB), st, G, s, W, td.rho, td.rt, new_w);
-        } else {
-            LAUNCH("k_tracer_flux", k_tracer_flux, GRID1((i64)G.L*G.nvh), dim3(TPB), st, G, s.rho + (i64)k*G.s, s.wind, W.flux);
-            LAUNCH("k_tracer_tend", k_tracer_tend, GRID1(G.s), dim3(TPB), st, G, s, W, td.rho, k, new_w);
-        }
-    }
+    if (!G.flat)
+        LAUNCH("k_vcc", k_vcc, GRID1((i64)G.n_oro*G.nsh), dim3(TPB), st, G, s, W.vcc);
+    FLAT_DISPATCH("k_scalar_tend", k_scalar_tend, GRID1(G.nsh), G, s, W, td.rho, td.rt, rk);
 }
 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
@@ -1076,12 +1227,9 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
                    const StepConsts &K, int rk, hipStream_t st)
 {
     dim3 g((G.nsh + 127)/128), b(128);
-    for (int k = 0; k < G.n_const; ++k) {
-        if (k == G.n_cond) continue;
-        if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities<26>, g, b, st, G, so, sn, td, W, K, rk, k);
-        else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities<55>, g, b, st, G, so, sn, td, W, K, rk, k);
-        else LAUNCH("k_gen_densities", k_gen_densities<0>, g, b, st, G, so, sn, td, W, K, rk, k);
-    }
+    if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities<26>, g, b, st, G, so, sn, td, W, K, rk);
+    else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities<55>, g, b, st, G, so, sn, td, W, K, rk);
+    else LAUNCH("k_gen_densities", k_gen_densities<0>, g, b, st, G, so, sn, td, W, K, rk);
 }
 
 void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st)

@@ -212,6 +212,11 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     G.nsh = d.nsh; G.nvh = d.nvh; G.nt = d.nt; G.vpl = d.vpl; G.dvpl = d.dvpl; G.L = d.n_layers;
     G.n_oro = g->no_of_oro_layers; G.n_const = d.n_const; G.n_cond = d.n_cond; G.moist = d.moisture_on;
     G.s = d.s; G.v = d.v; G.curl = d.curl; G.radius = g->radius; G.toa = g->toa;
+    /* a flat surface gives slope == 0 exactly (grad_hor_cov of identical heights),
+       so every terrain-correction term of the reference is an exact zero */
+    G.flat = 1;
+    for (size_t k = 0; k < (size_t)d.v && G.flat; ++k) if (g->slope[k] != 0.0) G.flat = 0;
+    if (getenv("GAME_GPU_NO_FLAT_SHORTCUT")) G.flat = 0;
     size_t S = (size_t)d.s, V = (size_t)d.v, NVH = (size_t)d.nvh, NSH = (size_t)d.nsh, NT = (size_t)d.nt;
 #define UP(dst, src, n) do { if ((rc = upload(&(dst), (src), (n)))) return rc; } while (0)
     UP(G.nd, g->normal_distance, V); UP(G.vol, g->volume, S); UP(G.area, g->area, V); UP(G.zs, g->z_scalar, S);
@@ -273,6 +278,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     double **sfields[] = {&W.flux_divv, &W.temp, &W.vsq, &W.phr, &W.sed_heat, &M->tmp_s};
     for (double **f : sfields) if ((rc = dzeros(f, S))) return rc;
     if ((rc = dzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
+    if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
     if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
     if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
     if ((rc = dzeros(&M->partials, 2*4096))) return rc;
@@ -425,6 +431,19 @@ int game_gpu_temperature_diagnostics(int k)
     if (!s) return GAME_ERR_ARG;
     fp64::temperature(M->G, *s, M->W.temp, M->stream);
     HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int game_gpu_inner_product(const double *a, const double *b, double *out)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    size_t V = (size_t)M->d.v, S = (size_t)M->d.s;
+    double *da = M->s[GAME_STATE_OLD].wind, *db = M->s[GAME_STATE_NEW].wind;
+    HIP_OK(hipMemcpyAsync(da, a, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    if (b != a) HIP_OK(hipMemcpyAsync(db, b, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    fp64::inner_product(M->G, da, b != a ? db : da, M->tmp_s, M->stream);
+    HIP_OK(hipMemcpyAsync(out, M->tmp_s, S*sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
     return 0;
 }
 

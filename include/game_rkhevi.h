@@ -238,6 +238,11 @@ int game_gpu_linear_combine_two_states(int slot_0, int slot_1, int slot_out,
    for n steps: copy new->old, manage_rkhevi, clear the first-step flag. */
 int game_gpu_run_steps(int n_steps, double delta_t, int first_step_bool);
 
+/* inner_product (src/spatial_operators/inner_product.c:13-43) of two host
+   vector fields (V each) into a host scalar field (S); uses the device state
+   slots as staging, so it is meant for the driver's set-up calls */
+int game_gpu_inner_product(const double *in_field_0, const double *in_field_1, double *out_field);
+
 /* temperature_diagnostics (src/constituents/derived_quantities.c:15-40) of a slot */
 int game_gpu_temperature_diagnostics(int slot);
 

@@ -1,0 +1,60 @@
+"""The C ABI: struct layout identical to the reference's, and every entry
+point declared in include/*.h exported by the libraries (no device needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import refharness as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "game_archived_amd", "lib")
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:unsigned\s+)?(?:int|long long|long|double|void|char)\s*\**\s*"
+                       r"(\w+)\s*\(", text, flags=re.M)
+    return [n for n in names if not n.startswith("game_dims_derive")]
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_dropin_layout_equals_reference(tmp_path):
+    exe = tmp_path / "abi_layout"
+    subprocess.run(["gcc", "-O0", "-I", os.path.join(ROOT, "include"), "-DGAME_RES_ID=5", "-DGAME_NO_OF_LAYERS=26",
+                    "-DGAME_MOISTURE_ON=1", os.path.join(ROOT, "tests", "abi_layout.c"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout
+    n = 0
+    for line in out.splitlines():
+        s, m, v = line.split()
+        if m == "sizeof":
+            assert R.lib().ref_sizeof(s.encode()) == int(v), (s, v)
+        else:
+            assert R.lib().ref_offsetof(s.encode(), m.encode()) == int(v), (s, m, v)
+        n += 1
+    assert n > 60
+
+
+def test_rkhevi_library_exports_every_declared_symbol():
+    lib = C.CDLL(os.path.join(LIB, "libgame_rkhevi.so"))
+    names = declared("game_rkhevi.h")
+    assert "game_gpu_manage_rkhevi" in names and len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("layout", ["r5l26m1", "r4l26m0", "r6l26m0"])
+def test_dropin_library_exports_reference_entry_points(layout):
+    C.CDLL(os.path.join(LIB, "libgame_rkhevi.so"), mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(os.path.join(LIB, f"libgame_dropin_{layout}.so"))
+    names = declared("game_abi.h")
+    for n in ("manage_rkhevi", "linear_combine_two_states", "temperature_diagnostics", "inner_product",
+              "game_abi_fingerprint"):
+        assert n in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.game_abi_fingerprint.restype = C.c_ulonglong
+    assert lib.game_abi_fingerprint() != 0
