@@ -109,6 +109,19 @@ def cpu_baseline(grid, state0, dt, steps):
                       f"OMP_NUM_THREADS={threads}, {cpu_info()}"}
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: the reference point that
+    scales the profiler's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * 2**20 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,11 +135,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--lloyd", type=int, default=2000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is horizontal domain decomposition, not built here)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     from game_archived_amd.grid import Grid
     from game_archived_amd.rkhevi import DeviceModel, NEW

@@ -43,6 +43,11 @@ struct DevGrid {
     const double *pv_top_dz1, *pv_top_dz2, *pv_sfc_dz1, *pv_sfc_dz2;
     /* Klemp damping coefficient of every interior interface, (L-1)*NSH */
     const double *damp;
+    /* number of layer chunks the per-cell kernels split a column into
+       (gridDim.y); more chunks give more waves to hide gather latency */
+    int split_cell;
+    /* 1: sound-wave solver streams each column through registers; 0: LDS-staged */
+    int vw_streaming;
 };
 
 struct DevState {
@@ -79,11 +84,12 @@ struct KernelTimer {
 };
 extern KernelTimer *g_kernel_timer;
 
-#define LAUNCH(NAME, KERNEL, GRID, BLOCK, STREAM, ...)                              \
+#define LAUNCH_SHM(NAME, KERNEL, GRID, BLOCK, SHM, STREAM, ...)                     \
     do {                                                                            \
         if (g_kernel_timer) g_kernel_timer->begin(NAME, STREAM);                    \
-        hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);            \
+        hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);          \
         if (g_kernel_timer) g_kernel_timer->end(STREAM);                            \
     } while (0)
+#define LAUNCH(NAME, KERNEL, GRID, BLOCK, STREAM, ...) LAUNCH_SHM(NAME, KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__)
 
 #endif

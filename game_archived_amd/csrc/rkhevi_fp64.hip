@@ -14,6 +14,14 @@
 
 #define TPB 256
 #define GRID1(n) dim3((unsigned)(((n) + TPB - 1)/TPB))
+#define GRID2(n, chunks) dim3((unsigned)(((n) + TPB - 1)/TPB), (unsigned)(chunks))
+
+/* layers [l0, l1) of chunk blockIdx.y when n layers are split into gridDim.y chunks */
+__device__ __forceinline__ void layer_chunk(int n, int &l0, int &l1)
+{
+    l0 = (int)(((i64)n*blockIdx.y)/gridDim.y);
+    l1 = (int)(((i64)n*(blockIdx.y + 1))/gridDim.y);
+}
 
 /* ------------------------------------------------------------------ */
 /* small device helpers restating reference functions                   */
@@ -455,7 +463,9 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const doubl
     const int ne = h < GAME_N_PENT ? 5 : 6;
     int adj[6];
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
-    for (int l = 0; l <= G.L; ++l) {
+    int l0, l1;
+    layer_chunk(G.L + 1, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         double vw = 0.5;
         if (l == 0 || l == G.L) vw = 1;
         double o = 0;
@@ -477,7 +487,9 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
     const int ne = h < GAME_N_PENT ? 5 : 6;
     int adj[6];
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
-    for (int l = 0; l < G.L; ++l) {
+    int l0, l1;
+    layer_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         i64 i = (i64)l*G.nsh + h;
         const double *w = G.ipw + 8*i;
         double o = 0;
@@ -598,7 +610,9 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
         fj[j] = G.from[ej[j]];
         tj[j] = G.to[ej[j]];
     }
-    for (int l = 0; l < L; ++l) {
+    int l0, l1;
+    layer_chunk(L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         const i64 i = (i64)l*nsh + h;
         double a[6], u[6];
         for (int j = 0; j < ne; ++j) {
@@ -849,111 +863,286 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
     }
 }
 
-/* three_band_solver_gen_densities (column_solvers.c:385-639) for tracer k,
-   one column per lane */
-template <int LT>
-__device__ __forceinline__ void gen_densities_column(const DevGrid &G, const DevState &so, const DevState &sn,
-                                                     const DevState &td, const DevWork &W, const StepConsts &K,
-                                                     int rk, int k, int i)
+/* The same sound-wave solve in three phases over LDS: a block takes VW_NCOL
+   columns; (1) every thread builds rows (column, interface) of the system and
+   the new-state interface theta, (2) one lane per column eliminates, back
+   substitutes and damps, (3a) every thread writes the new scalars of one
+   (column, layer), (3b) then the new w of one (column, interface).  Every
+   value is formed with the same arithmetic as in the streaming kernel above;
+   no thread reads a state element another thread of phase 3 writes. */
+#define VW_NCOL 64
+#define VW_THREADS 256
+
+__global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevState so, DevState sn, DevState td,
+                                                              StepConsts K, int rk)
 {
-    const int L = LT > 0 ? LT : G.L;
+    extern __shared__ double lds[];
+    const int L = G.L, nsh = G.nsh, vpl = G.vpl;
+    const int i0 = blockIdx.x*VW_NCOL;
+    const i64 gi = (i64)G.n_cond*G.s;
+    const double dt = K.dt, iw = K.impl_weight;
+    const double dt2 = dt*dt;
+    const double *zs = G.zs, *exbg = G.exbg, *vol = G.vol, *area = G.area;
+    const int P = L*VW_NCOL;
+    double *cc = lds, *dd = lds + P, *ee = lds + 2*P, *rr = lds + 3*P, *tt = lds + 4*P;
+
+    /* phase 1: row j couples the mass fluxes through interfaces j-1, j, j+1
+       (column_solvers.c:141-189) */
+    for (int idx = threadIdx.x; idx < (L - 1)*VW_NCOL; idx += VW_THREADS) {
+        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh, lo = b + nsh, wi = i + (i64)(j + 1)*vpl;
+        const ColumnLayer ca = column_layer(G, so, sn, td, dt, rk, i, j);
+        const ColumnLayer cb = column_layer(G, so, sn, td, dt, rk, i, j + 1);
+        const double tin = theta_int_new(G, sn, i, j), rio = rho_int_old(G, so, i, j);
+        const double w_old = so.wind[wi];
+        const double rie = 0.5*(ca.re + cb.re);
+        const double d = -(tin*tin)*(ca.ga + cb.ga)
+                         + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
+                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
+                           *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
+        const double r = -(w_old + dt*td.wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+                         + tin*(ca.epe - cb.epe)/dt
+                         + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
+                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
+        double c = 0.0, e = 0.0;
+        if (j < L - 2) {
+            const i64 l2 = lo + nsh;
+            const double tin_n = theta_int_new(G, sn, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
+            c = tin_n*cb.ga*tin
+                + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
+                - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
+            e = tin*cb.ga*tin_n
+                - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
+                + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
+        }
+        cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; tt[idx] = tin;
+    }
+    __syncthreads();
+    /* phase 2: Thomas (column_solvers.c:641-667) and Klemp damping (:284-298) */
+    if (threadIdx.x < VW_NCOL && i0 + (int)threadIdx.x < nsh) {
+        const int col = threadIdx.x, i = i0 + col, n = L - 1;
+        double ep = 0.0, x = rr[col]/dd[col];
+        if (n > 1) ep = ee[col]/dd[col];
+        ee[col] = ep; rr[col] = x;
+        for (int j = 1; j < n; ++j) {
+            const int o = j*VW_NCOL + col;
+            const double cp = cc[o - VW_NCOL];
+            const double den = dd[o] - ep*cp;
+            if (j < n - 1) ep = ee[o]/den;
+            x = (rr[o] - x*cp)/den;
+            ee[o] = ep; rr[o] = x;
+        }
+        for (int j = n - 2; j >= 0; --j) {
+            const int o = j*VW_NCOL + col;
+            x = rr[o] - ee[o]*x;
+            rr[o] = x;
+        }
+        for (int j = 0; j < n; ++j) {
+            const int o = j*VW_NCOL + col;
+            rr[o] = rr[o]/(1.0 + dt*G.damp[(i64)j*nsh + i]);
+        }
+    }
+    __syncthreads();
+    /* phase 3a: rho, rhotheta_v, theta_v_pert, exner_pert of layer j
+       (column_solvers.c:300-345, 357-370); the new gas density goes to LDS */
+    for (int idx = threadIdx.x; idx < L*VW_NCOL; idx += VW_THREADS) {
+        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh;
+        const ColumnLayer cl = column_layer(G, so, sn, td, dt, rk, i, j);
+        const double v = vol[b];
+        double rho_n, rt_n;
+        if (j == 0) {
+            const double x = rr[idx];
+            rho_n = cl.re + dt*(x)/v;
+            rt_n = cl.rte + dt*(tt[idx]*x)/v;
+        } else if (j == L - 1) {
+            const double xm = rr[idx - VW_NCOL];
+            rho_n = cl.re + dt*(-xm)/v;
+            rt_n = cl.rte + dt*(-tt[idx - VW_NCOL]*xm)/v;
+        } else {
+            const double x = rr[idx], xm = rr[idx - VW_NCOL];
+            rho_n = cl.re + dt*(-xm + x)/v;
+            rt_n = cl.rte + dt*(-tt[idx - VW_NCOL]*xm + tt[idx]*x)/v;
+        }
+        const double rt_o = so.rt[b];
+        sn.rho[gi + b] = rho_n;
+        sn.rt[b] = rt_n;
+        sn.tp[b] = rt_n/rho_n - G.thbg[b];
+        sn.ep[b] = so.ep[b] + v*cl.ga*(rt_n - rt_o);
+        cc[idx] = rho_n;
+    }
+    __syncthreads();
+    /* phase 3b: w at the interior interfaces (column_solvers.c:346-355) */
+    for (int idx = threadIdx.x; idx < (L - 1)*VW_NCOL; idx += VW_THREADS) {
+        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        const i64 wi = i + (i64)(j + 1)*vpl;
+        const double din = 0.5*(cc[idx] + cc[idx + VW_NCOL]);
+        sn.wind[wi] = (2.0*rr[idx]/area[wi] - din*so.wind[wi])/rho_int_old(G, so, i, j);
+    }
+}
+
+/* three_band_solver_gen_densities (column_solvers.c:385-639).
+
+   The tracers of a column are independent of each other except through the
+   sediment heat, which k_sed_heat forms separately, so one block solves NCOL
+   columns of one tracer (blockIdx.y) in three phases:
+     1. every thread builds rows (column, layer) of the tridiagonal system
+        into LDS -- all loads independent, fully coalesced;
+     2. one lane per column runs the Thomas elimination on LDS;
+     3. every thread writes the clipped solution back, coalesced.
+   Each coefficient is formed with the reference's arithmetic. */
+#define GD_NCOL 64
+#define GD_THREADS 256
+
+/* vertical fluxes of tracer k through interface jj of column i
+   (column_solvers.c:414-465): fi with the old wind, fr with the new wind
+   times the upstream density */
+__device__ __forceinline__ void tracer_iface_flux(const DevGrid &G, const DevState &so, const DevState &sn,
+                                                  const StepConsts &K, const double *rho_o, int k, int i, int jj,
+                                                  double &fi, double &fr)
+{
     const int nsh = G.nsh, vpl = G.vpl, ncc = G.n_cond;
+    fi = so.wind[i + (i64)(jj + 1)*vpl];
+    fr = sn.wind[i + (i64)(jj + 1)*vpl];
+    if (k < ncc/4) { fi -= K.snow_v; fr -= K.snow_v; }
+    else if (k < ncc/2) { fi -= K.rain_v; fr -= K.rain_v; }
+    else if (k < ncc) { fi -= K.cloud_v; fr -= K.cloud_v; }
+    double a = G.area[i + (i64)(jj + 1)*vpl];
+    fi = a*fi;
+    fr = a*fr;
+    i64 b = i + (i64)jj*nsh;
+    double doi = (fr >= 0.0) ? rho_o[b + nsh] : rho_o[b];
+    fr = doi*fr;
+}
+
+__global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td,
+                                                              StepConsts K)
+{
+    extern __shared__ double lds[];
+    const int L = G.L, nsh = G.nsh, ncc = G.n_cond;
+    const int k = (int)blockIdx.y < ncc ? (int)blockIdx.y : (int)blockIdx.y + 1;   /* skip the gas */
+    const int i0 = blockIdx.x*GD_NCOL;
     const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
-    const double *vol = G.vol, *area = G.area, *T = W.temp;
-    double *sh = W.sed_heat;
+    const double *vol = G.vol, *area = G.area;
     const double *rho_o = so.rho + (i64)k*G.s;
+    const double *trho = td.rho + (i64)k*G.s;
     double vsink = 0.0;
     if (k < ncc/4) vsink = K.snow_v;
     else if (k < ncc/2) vsink = K.rain_v;
     else if (k < ncc) vsink = K.cloud_v;
-    double ep[LT > 0 ? LT : GAME_MAX_LAYERS], x[LT > 0 ? LT : GAME_MAX_LAYERS];
-    i64 sfc_area = i + G.v - nsh, low = i + G.s - nsh;
-    double fi_m = 0.0, fr_m = 0.0, fe_m = 0.0, c_prev = 0.0;
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-        i64 b = i + (i64)j*nsh;
-        double fi = 0.0, fr = 0.0, fe = 0.0;
-        if (j < L - 1) {
-            /* vertical fluxes through interface j (column_solvers.c:414-465) */
-            if (rk == 0 && k == 0) sh[b] = 0.0;
-            fi = so.wind[i + (i64)(j + 1)*vpl];
-            fr = sn.wind[i + (i64)(j + 1)*vpl];
-            if (k < ncc/4) { fi -= K.snow_v; fr -= K.snow_v; }
-            else if (k < ncc/2) { fi -= K.rain_v; fr -= K.rain_v; }
-            else if (k < ncc) { fi -= K.cloud_v; fr -= K.cloud_v; }
-            double a = area[i + (i64)(j + 1)*vpl];
-            fi = a*fi;
-            fr = a*fr;
-            double doi, toi;
-            if (fr >= 0.0) { doi = rho_o[b + nsh]; toi = T[b + nsh]; }
-            else { doi = rho_o[b]; toi = T[b]; }
-            fr = doi*fr;
-            if (k < ncc) fe = c_p_cond(k, toi)*toi*fr;
-        } else if (rk == 0 && k == 0) sh[b] = 0.0;
-        /* row j (column_solvers.c:474-615) */
+    double *cc = lds, *dd = lds + L*GD_NCOL, *ee = lds + 2*L*GD_NCOL, *rr = lds + 3*L*GD_NCOL;
+
+    /* phase 1: rows (column_solvers.c:466-615) */
+    for (int idx = threadIdx.x; idx < L*GD_NCOL; idx += GD_THREADS) {
+        const int j = idx/GD_NCOL, col = idx - j*GD_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh;
+        double fi = 0.0, fr = 0.0, fi_m = 0.0, fr_m = 0.0;
+        if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
+        if (j > 0) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j - 1, fi_m, fr_m);
         double d;
         if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - iw*dt/vol[b]*fi;
         else if (j == L - 1) {
             d = (fi_m >= 0.0) ? 1.0 + iw*dt/vol[b]*fi_m : 1.0;
-            if (k < ncc) d += iw*vsink*dt*area[sfc_area]/vol[b];
+            if (k < ncc) d += iw*vsink*dt*area[i + G.v - nsh]/vol[b];
         } else {
             d = 1.0;
             if (fi_m >= 0.0) d += iw*dt/vol[b]*fi_m;
             if (fi < 0.0) d -= iw*dt/vol[b]*fi;
         }
-        double r = rho_o[b] + dt*td.rho[(i64)k*G.s + b];
-        if (j == 0) {
-            r += ew*dt*fr/vol[b];
-            if (rk == 0 && k < ncc) sh[b] += fe/vol[b];
-        } else if (j == L - 1) {
+        double r = rho_o[b] + dt*trho[b];
+        if (j == 0) r += ew*dt*fr/vol[b];
+        else if (j == L - 1) {
             r += -ew*dt*fr_m/vol[b];
-            if (rk == 0 && k < ncc) sh[b] += -fe_m/vol[b];
             if (k < ncc) {
-                r += -ew*vsink*dt*rho_o[low]*area[sfc_area]/vol[b];
-                if (rk == 0) sh[b] += -vsink*T[low]*c_p_cond(k, T[low])*rho_o[low]*area[sfc_area]/vol[b];
+                const i64 low = i + G.s - nsh;
+                r += -ew*vsink*dt*rho_o[low]*area[i + G.v - nsh]/vol[b];
             }
-        } else {
-            r += ew*dt*(-fr_m + fr)/vol[b];
-            if (rk == 0 && k < ncc) sh[b] += (-fe_m + fe)/vol[b];
-        }
+        } else r += ew*dt*(-fr_m + fr)/vol[b];
         /* off-diagonals of interface j (column_solvers.c:475-488) */
         double c = 0.0, e = 0.0;
         if (j < L - 1) {
-            if (fi >= 0.0) { c = 0.0; e = -iw*dt/vol[b]*fi; }
-            else { c = iw*dt/vol[b + nsh]*fi; e = 0.0; }
+            if (fi >= 0.0) e = -iw*dt/vol[b]*fi;
+            else c = iw*dt/vol[b + nsh]*fi;
         }
-        if (j == 0) {
-            ep[0] = e/d;
-            x[0] = r/d;
-        } else {
-            double den = d - ep[j - 1]*c_prev;
-            if (j < L - 1) ep[j] = e/den;
-            x[j] = (r - x[j - 1]*c_prev)/den;
-        }
-        c_prev = c;
-        fi_m = fi; fr_m = fr; fe_m = fe;
+        cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
     }
-#pragma unroll
-    for (int j = L - 2; j >= 0; --j) x[j] = x[j] - ep[j]*x[j + 1];
+    __syncthreads();
+    /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
+       the multipliers overwrite e, the solution overwrites r */
+    if (threadIdx.x < GD_NCOL && i0 + (int)threadIdx.x < nsh) {
+        const int col = threadIdx.x;
+        double ep = ee[col]/dd[col], x = rr[col]/dd[col];
+        ee[col] = ep; rr[col] = x;
+        for (int j = 1; j < L; ++j) {
+            const int o = j*GD_NCOL + col;
+            const double cp = cc[o - GD_NCOL];
+            const double den = dd[o] - ep*cp;
+            if (j < L - 1) ep = ee[o]/den;
+            x = (rr[o] - x*cp)/den;
+            ee[o] = ep; rr[o] = x;
+        }
+        for (int j = L - 2; j >= 0; --j) {
+            const int o = j*GD_NCOL + col;
+            x = rr[o] - ee[o]*x;
+            rr[o] = x;
+        }
+    }
+    __syncthreads();
+    /* phase 3: clipped solution (column_solvers.c:616-637) */
     double *rho_n = sn.rho + (i64)k*G.s;
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-        double v = x[j];
+    for (int idx = threadIdx.x; idx < L*GD_NCOL; idx += GD_THREADS) {
+        const int j = idx/GD_NCOL, col = idx - j*GD_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        double v = rr[idx];
         if (v < 0.0) v = 0.0;
         rho_n[i + (i64)j*nsh] = v;
     }
 }
 
-/* all tracers of a column in the reference's constituent order, so the
-   sediment heat accumulates in the same order (column_solvers.c:393-637) */
-template <int LT>
-__global__ void __launch_bounds__(128) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td, DevWork W,
-                                                       StepConsts K, int rk)
+/* sediment heat of the condensates at rk 0 (column_solvers.c:414-465 and the
+   sh updates of :500-611): one thread per grid box adds the contributions of
+   the condensates in the reference's constituent order, starting from 0 */
+__global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, DevWork W, StepConsts K)
 {
-    int i = blockIdx.x*128 + threadIdx.x;
-    if (i >= G.nsh) return;
-    for (int k = 0; k < G.n_const; ++k)
-        if (k != G.n_cond) gen_densities_column<LT>(G, so, sn, td, W, K, rk, k, i);
+    const i64 b = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (b >= G.s) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, ncc = G.n_cond;
+    const int j = (int)(b/nsh), i = (int)(b - (i64)j*nsh);
+    const double *vol = G.vol, *area = G.area, *T = W.temp;
+    double sh = 0.0;
+    for (int k = 0; k < ncc; ++k) {
+        const double *rho_o = so.rho + (i64)k*G.s;
+        double vsink;
+        if (k < ncc/4) vsink = K.snow_v;
+        else if (k < ncc/2) vsink = K.rain_v;
+        else vsink = K.cloud_v;
+        double fe = 0.0, fe_m = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int jj = pass == 0 ? j : j - 1;
+            if (jj < 0 || jj > L - 2) continue;
+            double fr = sn.wind[i + (i64)(jj + 1)*vpl];
+            fr -= vsink;
+            fr = area[i + (i64)(jj + 1)*vpl]*fr;
+            const i64 bb = i + (i64)jj*nsh;
+            double doi, toi;
+            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = T[bb + nsh]; }
+            else { doi = rho_o[bb]; toi = T[bb]; }
+            fr = doi*fr;
+            const double f = c_p_cond(k, toi)*toi*fr;
+            if (pass == 0) fe = f; else fe_m = f;
+        }
+        if (j == 0) sh += fe/vol[b];
+        else if (j == L - 1) {
+            const i64 low = i + G.s - nsh, sfc = i + G.v - nsh;
+            sh += -fe_m/vol[b];
+            sh += -vsink*T[low]*c_p_cond(k, T[low])*rho_o[low]*area[sfc]/vol[b];
+        } else sh += (-fe_m + fe)/vol[b];
+    }
+    W.sed_heat[b] = sh;
 }
 
 /* phase_trans.c:16-311 (surface phase transitions off), one thread per grid box */
@@ -1191,14 +1380,14 @@ void advection(const DevGrid &G, const double *wind, const double *rho_gas, cons
     LAUNCH("k_rel_vort_tri", k_rel_vort_tri, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
     FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID1(G.nvh), G, wind, rho_gas, (const double *)W.rvt, W.pv);
     LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e, GRID1(G.nvh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID1(G.nsh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_inner_product", k_inner_product, GRID1(G.nsh), dim3(TPB), st, G, wind, wind, W.vsq);
+    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
+    LAUNCH("k_inner_product", k_inner_product, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, wind, wind, W.vsq);
     FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, (const double *)W.vsq, W.vsq_grad);
     LAUNCH("k_grad_w", k_grad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, W.vsq, W.vsq_grad);
 }
 
 void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
-{ LAUNCH("k_inner_product", k_inner_product, GRID1(G.nsh), dim3(TPB), st, G, a, b, out); }
+{ LAUNCH("k_inner_product", k_inner_product, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, a, b, out); }
 
 void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
                  double new_w, double itw, double dt, hipStream_t st)
@@ -1211,12 +1400,24 @@ void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const 
 {
     if (!G.flat)
         LAUNCH("k_vcc", k_vcc, GRID1((i64)G.n_oro*G.nsh), dim3(TPB), st, G, s, W.vcc);
-    FLAT_DISPATCH("k_scalar_tend", k_scalar_tend, GRID1(G.nsh), G, s, W, td.rho, td.rt, rk);
+    FLAT_DISPATCH("k_scalar_tend", k_scalar_tend, GRID2(G.nsh, G.split_cell), G, s, W, td.rho, td.rt, rk);
 }
 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
                int rk, hipStream_t st)
 {
+    if (!G.vw_streaming) {
+        dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
+        size_t lds = (size_t)5*G.L*VW_NCOL*sizeof(double);   /* 65 KiB at 26 layers, 160 KiB at 64 */
+        static size_t lds_allowed = 65536;
+        if (lds > lds_allowed) {
+            (void)hipFuncSetAttribute((const void *)k_ver_waves_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+            lds_allowed = lds;
+        }
+        LAUNCH_SHM("k_ver_waves", k_ver_waves_lds, g, b, lds, st, G, so, sn, td, K, rk);
+        return;
+    }
     dim3 g((G.nsh + 127)/128), b(128);
     if (G.L == 26) LAUNCH("k_ver_waves", k_ver_waves<26>, g, b, st, G, so, sn, td, K, rk);
     else if (G.L == 55) LAUNCH("k_ver_waves", k_ver_waves<55>, g, b, st, G, so, sn, td, K, rk);
@@ -1226,10 +1427,15 @@ void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const D
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const DevWork &W,
                    const StepConsts &K, int rk, hipStream_t st)
 {
-    dim3 g((G.nsh + 127)/128), b(128);
-    if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities<26>, g, b, st, G, so, sn, td, W, K, rk);
-    else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities<55>, g, b, st, G, so, sn, td, W, K, rk);
-    else LAUNCH("k_gen_densities", k_gen_densities<0>, g, b, st, G, so, sn, td, W, K, rk);
+    if (rk == 0 && G.n_cond > 0) LAUNCH("k_sed_heat", k_sed_heat, GRID1(G.s), dim3(TPB), st, G, so, sn, W, K);
+    dim3 g((G.nsh + GD_NCOL - 1)/GD_NCOL, G.n_const - 1), b(GD_THREADS);
+    size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
+    static size_t lds_allowed = 65536;
+    if (lds > lds_allowed) {
+        (void)hipFuncSetAttribute((const void *)k_gen_densities, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_allowed = lds;
+    }
+    LAUNCH_SHM("k_gen_densities", k_gen_densities, g, b, lds, st, G, so, sn, td, K);
 }
 
 void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st)

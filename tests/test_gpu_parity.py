@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind")
 
 
-def run_pair(grid, state0, n_steps, dt):
+def run_pair(grid, state0, n_steps, dt, with_irrev=False):
     import pyoracle
     from game_archived_amd.rkhevi import DeviceModel
     om = pyoracle.OracleModel(grid, state0, dt)
@@ -27,9 +27,26 @@ def run_pair(grid, state0, n_steps, dt):
         dm.run(n_steps, dt)
         dev = dm.get_state()
         forc = dm.get_forcings()
+        irrev = dm.get_irrev() if with_irrev else None
     finally:
         dm.close()
+    if with_irrev:
+        return om, dev, forc, irrev
     return om, dev, forc
+
+
+def seeded_condensates(grid, state, seed=7):
+    """condensates up to 2e-3 of the gas density in half of the boxes and a
+    third of the boxes super-saturated, so every branch of the phase
+    transitions and the sedimentation runs"""
+    d = grid.dims
+    rng = np.random.default_rng(seed)
+    gas = state.rho[4 * d.s:5 * d.s]
+    for k in range(4):
+        state.rho[k * d.s:(k + 1) * d.s] = gas * rng.uniform(0, 2e-3, d.s) * (rng.random(d.s) < 0.5)
+    vap = state.rho[5 * d.s:6 * d.s]
+    vap *= np.where(rng.random(d.s) < 0.33, 1.6, 1.0)
+    return state
 
 
 @pytest.mark.parametrize("orography", ["flat", "gaussian"])
@@ -57,6 +74,42 @@ def test_moist_fp64_matches_oracle():
         assert e <= 1e-10, (f, e)
 
 
+def test_moist_condensates_mountain_matches_oracle():
+    """Moist with seeded condensates over the Gaussian mountain: tracer
+    transport with terrain corrections, sedimentation, sediment heat and all
+    saturation-adjustment branches, compared in every carried field."""
+    print("case: res 4 moist gaussian seeded condensates 3 steps dt 300", flush=True)
+    g = get_grid(4, "gaussian", moist=1)
+    s0 = seeded_condensates(g, oracle_state(g, moist=True))
+    om, dev, forc, irrev = run_pair(g, s0, 3, 300.0, with_irrev=True)
+    for f in FIELDS:
+        e = rel_err(getattr(dev, f), getattr(om.new, f))
+        print(f, e)
+        assert e <= 1e-10, (f, e)
+    for f in ("pot_vort_tend", "v_squared_grad", "pgrad_acc_old"):
+        e = rel_err(getattr(forc, f), getattr(om.forc, f))
+        print(f, e)
+        assert e <= 1e-10, (f, e)
+    for f in ("phase_trans_rates", "phase_trans_heating_rate", "condensates_sediment_heat"):
+        e = rel_err(getattr(irrev, f), getattr(om.irrev, f))
+        print(f, e)
+        assert e <= 1e-10, (f, e)
+
+
+@pytest.mark.parametrize("split", ["1", "7", "26"])
+def test_results_independent_of_layer_split(monkeypatch, split):
+    """How many layer chunks the per-cell kernels use changes the launch
+    shape only: the bits are those of the oracle for every split."""
+    monkeypatch.setenv("GAME_GPU_SPLIT_CELL", split)
+    print(f"case: res 4 moist flat, split {split}, 2 steps", flush=True)
+    g = get_grid(4, "flat", moist=1)
+    s0 = seeded_condensates(g, oracle_state(g, moist=True))
+    om, dev, forc = run_pair(g, s0, 2, 300.0)
+    for f in FIELDS:
+        e = rel_err(getattr(dev, f), getattr(om.new, f))
+        assert e <= 1e-10, (f, e)
+
+
 def test_config1_res6_mountain_100_steps():
     """The parity workload: res 6 x 26, dry, Gaussian mountain (2000 m,
     1500 km, 30N 90E), 100 steps at dt = 120 s, against the CPU oracle (which
@@ -80,6 +133,19 @@ def test_general_terrain_path_on_flat_grid(monkeypatch):
     g = get_grid(4, "flat")
     s0 = oracle_state(g)
     om, dev, forc = run_pair(g, s0, 5, 300.0)
+    for f in FIELDS:
+        e = rel_err(getattr(dev, f), getattr(om.new, f))
+        assert e <= 1e-12, (f, e)
+
+
+def test_streaming_sound_wave_solver_matches_oracle(monkeypatch):
+    """The register-streaming variant of the sound-wave solver (selected by
+    GAME_GPU_VW_STREAMING) gives the oracle's bits as the LDS-staged one does."""
+    monkeypatch.setenv("GAME_GPU_VW_STREAMING", "1")
+    print("case: res 4 dry gaussian, streaming solver, 3 steps", flush=True)
+    g = get_grid(4, "gaussian")
+    s0 = oracle_state(g)
+    om, dev, forc = run_pair(g, s0, 3, 300.0)
     for f in FIELDS:
         e = rel_err(getattr(dev, f), getattr(om.new, f))
         assert e <= 1e-12, (f, e)

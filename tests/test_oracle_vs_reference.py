@@ -8,49 +8,35 @@ geos95.h (averaging.c, vorticities.c, vorticity_flux.c and manage_rkhevi.c)
 cannot be built, so composite calls reach the oracle's restatement of
 vector_field_hor_cov_to_con, vertical_contravariant_corr, calc_pot_vort and
 vorticity_flux, and the few lines of manage_rkhevi.c (the call order and the
-wind update, src/time_stepping/manage_rkhevi.c:20-100) are restated below.
-Every comparison is bit for bit.
+wind update, src/time_stepping/manage_rkhevi.c:20-100) are restated in
+refharness.reference_composite.  Every comparison is bit for bit.
+
+The same cases are committed as golden digests (tests/golden/, written by
+tests/golden/make_golden.py) so that the oracle is also pinned where the
+reference sources are absent (test_oracle_golden.py).
 """
 import ctypes as C
 
 import numpy as np
 import pytest
 
+import parity_cases as P
 import refharness as R
-from conftest import get_grid, oracle_state
 
 pytestmark = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs the reference sources)")
-
-VP = C.c_void_p
-
-
-def ref_fn(name, *argtypes, restype=C.c_int):
-    f = getattr(R.lib(), name)
-    f.argtypes = list(argtypes)
-    f.restype = restype
-    return f
 
 
 @pytest.fixture(scope="module")
 def setup():
     sz = R.sizes()
-    grid = get_grid(sz["res_id"], "gaussian", moist=1, n_lloyd=2000)
+    grid = P.case_grid()
     assert grid.dims.s == sz["s"] and grid.dims.v == sz["v"]
-    state = oracle_state(grid, moist=True)
-    # exercise every branch of the moist physics: seeded condensates and
-    # super-saturated vapour in a third of the boxes
-    rng = np.random.default_rng(7)
-    d = grid.dims
-    gas = state.rho[4 * d.s:5 * d.s]
-    for k in range(4):
-        state.rho[k * d.s:(k + 1) * d.s] = gas * rng.uniform(0, 2e-3, d.s) * (rng.random(d.s) < 0.5)
-    vap = state.rho[5 * d.s:6 * d.s]
-    vap *= np.where(rng.random(d.s) < 0.33, 1.6, 1.0)
-    return grid, state
+    return grid, P.case_state(grid)
 
 
 def compare(a, b, what):
     a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
     bad = np.flatnonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))
     assert bad.size == 0, f"{what}: {bad.size} elements differ, first at {bad[:5]}, " \
                           f"{a.ravel()[bad[:3]]} vs {b.ravel()[bad[:3]]}"
@@ -63,107 +49,22 @@ def test_struct_layout_matches_views():
 
 
 def test_operators_bitwise(setup):
-    import pyoracle
     grid, st = setup
-    d = grid.dims
-    g, dg = R.grid_structs(grid)
-    ctx = pyoracle.ctx(grid)
-    O = pyoracle.lib()
-    from game_archived_amd import _native as N
-    rng = np.random.default_rng(1)
-    # grad (gradient_operators.c:66-74), divv_h (divergences.c:14-64)
-    scal = rng.standard_normal(d.s)
-    out_r, out_o = np.zeros(d.v), np.zeros(d.v)
-    ref_fn("grad", VP, VP, VP)(VP(scal.ctypes.data), VP(out_r.ctypes.data), g.ptr)
-    O.or_grad(ctx, N.ptr(scal), N.ptr(out_o))
-    compare(out_o, out_r, "grad")
-    vec = rng.standard_normal(d.v)
-    div_r, div_o = np.zeros(d.s), np.zeros(d.s)
-    ref_fn("divv_h", VP, VP, VP)(VP(vec.ctypes.data), VP(div_r.ctypes.data), g.ptr)
-    O.or_divv_h(ctx, N.ptr(vec), N.ptr(div_o))
-    compare(div_o, div_r, "divv_h")
-    dens = np.abs(rng.standard_normal(d.s))
-    ref_fn("divv_h_tracer", VP, VP, VP, VP, VP)(VP(vec.ctypes.data), VP(dens.ctypes.data), VP(st.wind.ctypes.data),
-                                                VP(div_r.ctypes.data), g.ptr)
-    O.or_divv_h_tracer(ctx, N.ptr(vec), N.ptr(dens), N.ptr(st.wind), N.ptr(div_o))
-    compare(div_o, div_r, "divv_h_tracer")
-    for name in ("scalar_times_vector", "scalar_times_vector_h", "scalar_times_vector_h_upstream",
-                 "scalar_times_vector_v"):
-        out_r[:] = 0
-        out_o[:] = 0
-        ref_fn(name, VP, VP, VP, VP)(VP(scal.ctypes.data), VP(vec.ctypes.data), VP(out_r.ctypes.data), g.ptr)
-        getattr(O, "or_" + name)(ctx, N.ptr(scal), N.ptr(vec), N.ptr(out_o))
-        compare(out_o, out_r, name)
-    ip_r, ip_o = np.zeros(d.s), np.zeros(d.s)
-    ref_fn("inner_product", VP, VP, VP, VP)(VP(vec.ctypes.data), VP(st.wind.ctypes.data), VP(ip_r.ctypes.data), g.ptr)
-    O.or_inner_product(ctx, N.ptr(vec), N.ptr(st.wind), N.ptr(ip_o))
-    compare(ip_o, ip_r, "inner_product")
-
-
-def run_reference_steps(grid, state0, dt, n_steps):
-    """The driver loop (src/coordinator.c:292-324) around the call sequence of
-    manage_rkhevi (src/time_stepping/manage_rkhevi.c:20-100), on the reference's
-    own functions."""
-    from game_archived_amd import _native as N
-    d = grid.dims
-    g, dg = R.grid_structs(grid)
-    so, sn, td = R.state_struct(state0), R.state_struct(state0), R.RefStruct("State")
-    diag, forc, irrev = R.RefStruct("Diagnostics"), R.RefStruct("Forcings"), R.RefStruct("Irreversible_quantities")
-    cfg_py = N.default_config()
-    cfg = R.config_struct(cfg_py)
-    lin = ref_fn("linear_combine_two_states", VP, VP, VP, C.c_double, C.c_double, VP)
-    tdiag = ref_fn("temperature_diagnostics", VP, VP, VP)
-    h2o = ref_fn("calc_h2otracers_source_rates", VP, VP, VP, VP, VP, C.c_double)
-    mpg = ref_fn("manage_pressure_gradient", VP, VP, VP, VP, VP, VP, VP)
-    pgc = ref_fn("calc_pressure_grad_condensates_v", VP, VP, VP, VP)
-    vte = ref_fn("vector_tendencies_expl", VP, VP, VP, VP, VP, VP, VP, VP, C.c_int, C.c_double)
-    ste = ref_fn("scalar_tendencies_expl", VP, VP, VP, VP, VP, C.c_double, VP, VP, VP, VP, C.c_int)
-    vw = ref_fn("three_band_solver_ver_waves", VP, VP, VP, VP, VP, VP, C.c_double, VP, C.c_int)
-    gd = ref_fn("three_band_solver_gen_densities", VP, VP, VP, VP, VP, VP, C.c_double, C.c_int, VP)
-    lin(so.ptr, so.ptr, sn.ptr, 1.0, 0.0, g.ptr)
-    first = cfg.member("totally_first_step_bool", np.int32, 1)
-    first[0] = 1
-    wind_o, wind_n, wind_t = (s.member("wind", np.float64, d.v) for s in (so, sn, td))
-    edges = (d.nsh + np.arange(d.n_layers)[:, None] * d.vpl + np.arange(d.nvh)[None, :]).ravel()
-    for _ in range(n_steps):
-        lin(sn.ptr, so.ptr, so.ptr, 1.0, 0.0, g.ptr)
-        tdiag(so.ptr, g.ptr, diag.ptr)
-        if d.moisture_on:
-            h2o(so.ptr, diag.ptr, g.ptr, cfg.ptr, irrev.ptr, dt)
-        for rk in range(2):
-            if rk == 0:
-                mpg(sn.ptr, g.ptr, dg.ptr, diag.ptr, forc.ptr, irrev.ptr, cfg.ptr)
-            pgc(sn.ptr, g.ptr, forc.ptr, irrev.ptr)
-            vte(sn.ptr, td.ptr, g.ptr, dg.ptr, diag.ptr, forc.ptr, irrev.ptr, cfg.ptr, rk, dt)
-            wind_n[edges] = wind_o[edges] + dt * wind_t[edges]
-            ste(so.ptr, sn.ptr, td.ptr, g.ptr, dg.ptr, dt, diag.ptr, forc.ptr, irrev.ptr, cfg.ptr, rk)
-            vw(so.ptr, sn.ptr, td.ptr, diag.ptr, forc.ptr, cfg.ptr, dt, g.ptr, rk)
-            if d.n_const > 1:
-                gd(so.ptr, sn.ptr, td.ptr, diag.ptr, irrev.ptr, cfg.ptr, dt, rk, g.ptr)
-        first[0] = 0
-    return sn, td, forc, irrev, diag
+    ref = R.reference_operators(grid, st)
+    ora = P.oracle_operators(grid, st)
+    assert ref.keys() == ora.keys()
+    for k in ref:
+        compare(ora[k], ref[k], k)
 
 
 def test_composite_step_bitwise(setup):
     """Three RKHEVI steps (moist, Gaussian mountain, res 5 x 26): the oracle
-    reproduces the reference bit for bit in every prognostic field and in the
-    carried forcings."""
-    import pyoracle
+    reproduces the reference bit for bit in every prognostic field, in the
+    carried forcings and tendencies, the phase-transition rates, the sediment
+    heat and the temperature."""
     grid, st = setup
-    d = grid.dims
-    dt = 300.0
-    sn, td, forc, irrev, diag = run_reference_steps(grid, st, dt, 3)
-    om = pyoracle.OracleModel(grid, st, dt)
-    om.run(3)
-    ref_state = R.state_fields(sn, d)
-    for f in ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind"):
-        compare(getattr(om.new, f), ref_state[f], f)
-    for f in ("pgrad_acc_old", "pressure_gradient_acc_neg_nl", "pressure_gradient_acc_neg_l",
-              "pressure_grad_condensates_v", "v_squared_grad", "pot_vort_tend"):
-        compare(getattr(om.forc, f), forc.member(f, np.float64, d.v), f)
-    compare(om.tend.wind, td.member("wind", np.float64, d.v), "tendency wind")
-    compare(om.tend.rho, td.member("rho", np.float64, d.n_const * d.s), "tendency rho")
-    compare(om.irrev.phase_trans_rates, irrev.member("phase_trans_rates", np.float64, 5 * d.s), "phase_trans_rates")
-    compare(om.irrev.condensates_sediment_heat, irrev.member("condensates_sediment_heat", np.float64, d.s),
-            "condensates_sediment_heat")
-    compare(om.diag.temperature, diag.member("temperature", np.float64, d.s), "temperature")
+    ref = R.reference_composite(grid, st)
+    ora = P.oracle_composite(grid, st)
+    assert set(ref) == set(P.COMPOSITE_FIELDS) == set(ora)
+    for k in P.COMPOSITE_FIELDS:
+        compare(ora[k], ref[k], k)
