@@ -177,6 +177,8 @@ def host_lib():
     lib.game_grid_generate.restype = C.c_int
     lib.game_grid_generators.argtypes = [C.c_int, C.c_int, DP]
     lib.game_grid_generators.restype = C.c_int
+    lib.game_grid_lloyd_state.argtypes = [C.c_int, C.c_int, DP, DP, IP, IP, IP]
+    lib.game_grid_lloyd_state.restype = C.c_int
     lib.game_grid_last_error.restype = C.c_char_p
     lib.game_init_default_params.argtypes = [C.POINTER(InitParams)]
     for fn in (lib.game_init_phase1,):

@@ -767,7 +767,8 @@ static int horizontal_generators(hgrid_t *hg, int n_iter, const int *adj, const 
     return 0;
 }
 
-int game_grid_generators(int res_id, int n_lloyd_iterations, double *latlon_out)
+int game_grid_lloyd_state(int res_id, int n_lloyd_iterations, double *latlon_out, double *latlon_dual_out,
+                          int *adj_out, int *from_dual_out, int *to_dual_out)
 {
     hgrid_t hg;
     int rc = hgrid_setup(&hg, res_id);
@@ -781,11 +782,25 @@ int game_grid_generators(int res_id, int n_lloyd_iterations, double *latlon_out)
     set_from_to_index_dual(&hg, from_d, to_d);
     rc = find_adjacent_vector_indices(&hg, from, to, sg, adj);
     if (!rc) rc = horizontal_generators(&hg, n_lloyd_iterations, adj, from_d, to_d, lat, lon, lat_d, lon_d);
-    if (!rc)
+    if (!rc) {
         for (int i = 0; i < hg.nsh; ++i) { latlon_out[2*i] = lat[i]; latlon_out[2*i + 1] = lon[i]; }
+        if (latlon_dual_out) {
+            /* the Voronoi vertices the next iteration starts from (optimize.c:26) */
+            set_dual_coords(&hg, lat, lon, lat_d, lon_d);
+            for (int i = 0; i < hg.nt; ++i) { latlon_dual_out[2*i] = lat_d[i]; latlon_dual_out[2*i + 1] = lon_d[i]; }
+        }
+        if (adj_out) memcpy(adj_out, adj, sizeof(int)*6*(size_t)hg.nsh);
+        if (from_dual_out) memcpy(from_dual_out, from_d, sizeof(int)*(size_t)hg.nvh);
+        if (to_dual_out) memcpy(to_dual_out, to_d, sizeof(int)*(size_t)hg.nvh);
+    }
     free(from); free(to); free(from_d); free(to_d); free(adj); free(sg);
     free(lat); free(lon); free(lat_d); free(lon_d); free(hg.tri_pts);
     return rc;
+}
+
+int game_grid_generators(int res_id, int n_lloyd_iterations, double *latlon_out)
+{
+    return game_grid_lloyd_state(res_id, n_lloyd_iterations, latlon_out, NULL, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------ */

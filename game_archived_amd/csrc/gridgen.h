@@ -59,6 +59,14 @@ int game_grid_generate(const game_grid_params_t *params, const game_dims_t *dims
    (2*NSH doubles, lat/lon interleaved); lets callers cache the expensive part. */
 int game_grid_generators(int res_id, int n_lloyd_iterations, double *latlon_out);
 
+/* The state one Lloyd iteration (grid_generator/src/optimize.c:17-31) starts
+   from after n_lloyd_iterations: generators (NSH x 2 lat/lon), their Voronoi
+   vertices (NT x 2, may be NULL), the adjacency with -1 in the pentagons'
+   sixth slot (6 x NSH, may be NULL) and the dual from/to indices before their
+   reorientation (NVH each, may be NULL). */
+int game_grid_lloyd_state(int res_id, int n_lloyd_iterations, double *latlon_out, double *latlon_dual_out,
+                          int *adj_out, int *from_dual_out, int *to_dual_out);
+
 const char *game_grid_last_error(void);
 
 #ifdef __cplusplus
