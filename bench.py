@@ -1,12 +1,17 @@
 """Benchmark of the device RKHEVI step (one MI355X).
 
 Workload: icosahedral grid res 7 (163842 cells, 491520 edges) x 26 layers,
-moist (6 constituents), flat surface, the analytic balanced state with seeded
-noise, dt = 120 s -- the configuration the simulated-days-per-wall-hour metric
-is quoted on.  A step is one RKHEVI time step (manage_rkhevi plus the driver's
-state copy) on inputs resident in HBM.  Prints one JSON line.
+moist (6 constituents), flat surface, 2000 Lloyd iterations, the analytic
+balanced state with seeded noise, at the reference's automatic time step
+(1.61e-3 x effective resolution, 90.1 s at res 7).  A step is one RKHEVI
+time step (manage_rkhevi plus the driver's state copy) on inputs resident in
+HBM.  Both precision forms run from the same state for one simulated day
+(warm-up plus timed steps); the JSON value is the mixed form's simulated days
+per wall-hour, the FP64 form's numbers and the mixed form's difference from
+it after the day sit in config.forms.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 7] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 7] [--precision both|fp64|mixed] [--no-cpu]
+  python bench.py --copy-probe 1024     # plain 1 GiB device copy, for counter calibration
 """
 import argparse
 import json
@@ -29,17 +34,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(dims, moist, orography):
+def algorithmic_bytes(dims, moist, orography, mixed=False):
     """Minimal-pass traffic of one step in bytes: the phase-schedule model
     (FP64 doubles per cell-layer per step: 172 dry flat, 202 dry with
-    orography, 262 moist flat) times the number of cell-layers."""
+    orography, 262 moist flat) times the number of cell-layers.  The mixed
+    form stores the work arrays in FP32: of the flat dry 172 doubles, 49 are
+    work arrays (pressure gradients, momentum tendency, vorticities, PV,
+    kinetic energy and its gradient), of the moist additions 6.5 (rates,
+    heating, temperature, sediment heat); half their bytes go."""
     per_cell_layer = 262 if moist else (202 if orography != "flat" else 172)
+    if mixed:
+        per_cell_layer -= 0.5 * (49 + (6.5 if moist else 0.0))
     return per_cell_layer * 8.0 * dims.s
 
 
-def kernel_bytes(name, dims):
+def kernel_bytes(name, dims, wb=8):
     """Algorithmic HBM bytes of one launch of a kernel: every distinct array
-    element the kernel must read or write once (double = 8 B, int = 4 B;
+    element the kernel must read or write once (double = 8 B, int = 4 B,
+    work arrays wb = 8 B in the FP64 form and 4 B in the mixed form;
     neighbour tables shared by all layers counted once per launch)."""
     S, V, NSH, NVH, L, NT = dims.s, dims.v, dims.nsh, dims.nvh, dims.n_layers, dims.nt
     E = L * NVH                     # edge values per vector field
@@ -47,33 +59,38 @@ def kernel_bytes(name, dims):
     nc = dims.n_const
     moist = dims.n_const > 1
     table = {
-        # per column: reads rho_gas, rhotheta_v, theta_v_pert, exner_pert of the old
-        # and new state, the tendencies of rho_gas, rhotheta_v and w, w old, z_scalar,
-        # exner_bg, volume, area at w, damping; writes rho_gas, rhotheta_v,
-        # theta_v_pert, exner_pert and w
-        "k_ver_waves": 8 * (S * (4 + 4 + 2 + 4) + W_ * (1 + 1 + 1 + 1) + W_ * 1 + S * 4 + W_ * 1),
-        # all tracers of a column: rho_k old, tendency and new (5 x 3 S), w old and new,
-        # area at w, volume, temperature, sediment heat
-        "k_gen_densities": 8 * ((nc - 1) * 3 * S + 3 * W_ + 3 * S),
-        # TRSK flux: mass flux (edges + w), PV (tangential + rhombus), TRSK tables once, ipw slots 6/7, output
-        "k_vorticity_flux_e": 8 * (E + (L + 1) * NSH + (L + 1) * NVH + L * NVH + 2 * S + E) + 4 * 20 * NVH + 8 * 10 * NVH,
-        "k_vorticity_flux_w": 8 * (E + dims.curl // 2 + 6 * S + (L + 1) * NSH) + 4 * 6 * NSH,
-        "k_pot_vort": 8 * (dims.curl + L * NT + E + (L + 1) * NSH + S + dims.curl // 2) + 4 * 8 * NVH + 8 * 4 * NVH,
-        "k_rel_vort_tri": 8 * (E + E + L * NT + L * NT) + 4 * 6 * NT,
-        # all constituents: rho_k, rhotheta_v, wind at edges, area, volume, tendencies
-        # (read + write); moist adds phase rates, temperature, heating terms, exner
-        "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (S * (nc + 6) if moist else 0)) + 4 * 12 * NSH,
-        "k_vector_tend": 8 * (V * 6 + V + 2 * E),
-        "k_pgrad_e": 8 * (E * 5 + S * (3 + nc)) + 4 * 2 * NVH,
-        "k_pgrad_w": 8 * (W_ * 6 + S * (3 + nc)),
-        "k_grad_e": 8 * (S + E + E) + 4 * 2 * NVH,
-        "k_grad_w": 8 * (S + 2 * W_),
-        "k_inner_product": 8 * (V + 8 * S + S) + 4 * 6 * NSH,
-        "k_mass_flux": 8 * (S + 2 * (E + W_)) + 4 * 2 * NVH,
-        "k_phase_trans": 8 * (S * (nc + 1) + S * 6),
-        "k_temperature": 8 * (S * 5 + S),
-        "k_lincomb_scalars": 8 * (S * (nc + 3) * 2 + S * (nc + 3)),
-        "k_lincomb_vector": 8 * (2 * V + V),
+        # LDS sound-wave solver: old and new state and the tendencies of the
+        # column (14 S), grid columns (4 S: z, exner_bg, volume, theta_bg), w old,
+        # area at w, damping (3 W), the w tendency (work), writes 4 S + W
+        "k_ver_waves": 8 * (14 * S + 3 * W_ + 4 * S + W_) + wb * W_,
+        # every tracer of a column: rho_k old, tendency and new; w old and new,
+        # area at w, volume (shared)
+        "k_gen_densities": 8 * ((nc - 1) * 3 * S + 3 * W_ + S),
+        "k_sed_heat": 8 * (4 * S + 2 * W_ + S) + wb * 2 * S,
+        # TRSK flux: mass flux (edges, w), PV (tangential, rhombus), output (work
+        # arrays); ipw slots 6/7; TRSK tables once
+        "k_vorticity_flux_e": wb * (E + (L + 1) * NSH + (L + 1) * NVH + L * NVH + E) + 8 * 2 * S
+                              + 4 * 20 * NVH + 8 * 10 * NVH,
+        "k_vorticity_flux_w": wb * (E + dims.curl // 2 + (L + 1) * NSH) + 8 * 6 * S + 4 * 6 * NSH,
+        "k_pot_vort": 8 * (E + (L + 1) * NSH + S + dims.curl // 2) + wb * (dims.curl + L * NT)
+                      + 4 * 8 * NVH + 8 * 4 * NVH,
+        "k_rel_vort_tri": 8 * (2 * E + L * NT) + wb * L * NT + 4 * 6 * NT,
+        # all constituents: rho_k, rhotheta_v, wind and area at edges, volume,
+        # tendencies (read + write); moist: rates, temperature, heating, sediment
+        # heat (work) and exner (state, background)
+        "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (2 * S if moist else 0))
+                         + (wb * 8 * S if moist else 0) + 4 * 12 * NSH,
+        "k_vector_tend": wb * (7 * V + W_) + 8 * 2 * E,
+        "k_pgrad_e": 8 * (S * (3 + nc) + 2 * E) + wb * 5 * E + 4 * 2 * NVH,
+        "k_pgrad_w": 8 * (S * (3 + nc) + 3 * W_) + wb * 6 * W_,
+        "k_grad_e": wb * (S + E) + 8 * E + 4 * 2 * NVH,
+        "k_grad_w": wb * (S + W_) + 8 * W_,
+        "k_inner_product": 8 * (V + 8 * S) + wb * S + 4 * 6 * NSH,
+        "k_mass_flux": 8 * (S + E + W_) + wb * (E + W_) + 4 * 2 * NVH,
+        "k_phase_trans": 8 * nc * S + wb * 7 * S,
+        "k_temperature": 8 * 6 * S + wb * S,
+        "k_lincomb_scalars": 8 * (2 * (nc + 2) * S + (nc + 3) * S),
+        "k_lincomb_vector": 8 * 3 * V,
     }
     return table.get(name)
 
@@ -122,10 +139,54 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
 
 
+def print_kernel_table(stats, d, wb):
+    total_ms = sum(ms for _, ms in stats.values())
+    rows = sorted(stats.items(), key=lambda kv: -kv[1][1])
+    log(f"{'kernel':24s} {'launches':>8s} {'avg us':>9s} {'share':>6s} {'GB/s':>8s}")
+    for name, (n, ms) in rows:
+        b = kernel_bytes(name, d, wb)
+        gbs = (b / (ms / n * 1e-3) / 1e9) if b else float("nan")
+        log(f"{name:24s} {n:8d} {ms / n * 1e3:9.1f} {ms / total_ms * 100:5.1f}% {gbs:8.1f}")
+
+
+def time_form(grid, state0, precision, steps, warmup, dt, moist):
+    """warm-up steps, then `steps` timed steps of one precision form with the
+    state resident in HBM; per-kernel times from HIP events on the library's
+    stream"""
+    from game_archived_amd.rkhevi import DeviceModel, NEW
+    d = grid.dims
+    dm = DeviceModel(grid, precision=precision)
+    try:
+        dm.start(state0)
+        i0 = dm.integrals()
+        dm.run(warmup, dt)
+        dm.synchronize()
+        dm.reset_timing()
+        dm.enable_timing(True)
+        t0 = time.perf_counter()
+        dm.run(steps, dt)
+        dm.synchronize()
+        wall = time.perf_counter() - t0
+        dm.enable_timing(False)
+        stats = dm.kernel_stats()
+        i1 = dm.integrals()
+        final = dm.get_state(NEW)
+        nbytes = dm.bytes_allocated()
+    finally:
+        dm.close()
+    dry0 = i0[d.n_cond] - (i0[d.n_cond + 1] if moist else 0.0)
+    dry1 = i1[d.n_cond] - (i1[d.n_cond + 1] if moist else 0.0)
+    return {"ms_per_step": wall / steps * 1e3, "stats": stats, "final": final, "bytes_allocated": nbytes,
+            "mass_drift": float((i1[d.n_cond] - i0[d.n_cond]) / i0[d.n_cond]),
+            "dry_drift": float((dry1 - dry0) / dry0),
+            "finite": bool(np.isfinite(final.rho).all() and np.isfinite(final.wind).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: warm-up plus timed steps cover one simulated day)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--res", type=int, default=7)
     ap.add_argument("--layers", type=int, default=26)
@@ -133,8 +194,10 @@ def main():
     ap.add_argument("--dt", default="auto",
                     help="seconds, or 'auto' for the reference rule 1.61e-3*eff_hor_res (src/coordinator.c:155-171)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--lloyd", type=int, default=2000)
+    ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
+                    help="precision forms to time; the JSON value is the mixed form's when it runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -148,7 +211,7 @@ def main():
         return
 
     from game_archived_amd.grid import Grid
-    from game_archived_amd.rkhevi import DeviceModel, NEW
+    from game_archived_amd.rkhevi import DeviceModel
     from game_archived_amd.state import initial_state
 
     moist = not args.dry
@@ -166,75 +229,75 @@ def main():
         eff_hor_res = math.sqrt(a[idx].sum() / (d.n_levels * d.nsh))
         args.dt = 1.61e-3 * eff_hor_res
     args.dt = float(args.dt)
-    dm = DeviceModel(grid)
+    warmup = args.warmup
+    steps = args.steps if args.steps is not None else max(1, math.ceil(86400.0 / args.dt) - warmup)
+    from game_archived_amd.rkhevi import FP64, MIXED
     t0 = time.time()
+    dm = DeviceModel(grid, precision=FP64)
     state0 = initial_state(grid, dm.balance_terms(), moist=moist)
-    log(f"initial state in {time.time() - t0:.1f} s; device memory {dm.bytes_allocated() / 2**30:.2f} GiB")
-    dm.start(state0)
-    i0 = dm.integrals()
-    dm.run(args.warmup, args.dt)
-    dm.synchronize()
-    dm.reset_timing()
-    dm.enable_timing(True)
-    t0 = time.perf_counter()
-    dm.run(args.steps, args.dt)
-    dm.synchronize()
-    wall = time.perf_counter() - t0
-    dm.enable_timing(False)
-    stats = dm.kernel_stats()
-    i1 = dm.integrals()
-    final = dm.get_state(NEW)
     dm.close()
-    finite = bool(np.isfinite(final.rho).all() and np.isfinite(final.wind).all())
-    ms_per_step = wall / args.steps * 1e3
-    sdph = (args.dt / 86400.0) / (ms_per_step / 1e3 / 3600.0)
-    d = grid.dims
-    # per-kernel table and the dominant kernel's roofline
-    total_ms = sum(ms for _, ms in stats.values())
-    rows = sorted(stats.items(), key=lambda kv: -kv[1][1])
-    log(f"{'kernel':24s} {'launches':>8s} {'avg us':>9s} {'share':>6s} {'GB/s':>8s}")
-    for name, (n, ms) in rows:
-        b = kernel_bytes(name, d)
-        gbs = (b / (ms / n * 1e-3) / 1e9) if b else float("nan")
-        log(f"{name:24s} {n:8d} {ms / n * 1e3:9.1f} {ms / total_ms * 100:5.1f}% {gbs:8.1f}")
+    log(f"initial state in {time.time() - t0:.1f} s")
+    forms = {"fp64": FP64, "mixed": MIXED} if args.precision == "both" else {args.precision: {"fp64": FP64, "mixed": MIXED}[args.precision]}
+    res = {}
+    for name, prec in forms.items():
+        res[name] = time_form(grid, state0, prec, steps, warmup, args.dt, moist)
+        r = res[name]
+        log(f"[{name}] {r['ms_per_step']:.4f} ms/step, device memory {r['bytes_allocated'] / 2**30:.2f} GiB")
+        print_kernel_table(r["stats"], d, 4 if prec == MIXED else 8)
+    main_name = "mixed" if "mixed" in res else "fp64"
+    r = res[main_name]
+    wb = 4 if main_name == "mixed" else 8
+    rows = sorted(r["stats"].items(), key=lambda kv: -kv[1][1])
     dom_name, (dom_n, dom_ms) = rows[0]
-    dom_bytes = kernel_bytes(dom_name, d)
+    dom_bytes = kernel_bytes(dom_name, d, wb)
     dom_avg_s = dom_ms / dom_n * 1e-3
     achieved = dom_bytes / dom_avg_s / 1e9
-    step_bytes = algorithmic_bytes(d, moist, "flat")
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
-    mass_drift = (i1[d.n_cond] - i0[d.n_cond]) / i0[d.n_cond]
-    dry_drift = ((i1[d.n_cond] - (i1[d.n_cond + 1] if moist else 0.0))
-                 - (i0[d.n_cond] - (i0[d.n_cond + 1] if moist else 0.0))) / (i0[d.n_cond] - (i0[d.n_cond + 1] if moist else 0.0))
+    sdph = lambda ms: (args.dt / 86400.0) / (ms / 1e3 / 3600.0)
+    form_cfg = {}
+    for name, x in res.items():
+        step_bytes = algorithmic_bytes(d, moist, "flat", mixed=(name == "mixed"))
+        gbs = step_bytes / (x["ms_per_step"] * 1e-3) / 1e9
+        form_cfg[name] = {
+            "sim_days_per_wall_hour": round(sdph(x["ms_per_step"]), 2), "ms_per_step": round(x["ms_per_step"], 4),
+            "column_updates_per_s": round(d.nsh * 1e3 / x["ms_per_step"], 1),
+            "step_algorithmic_bytes": int(step_bytes), "step_algorithmic_GBps": round(gbs, 1),
+            "step_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "gas_mass_rel_drift": x["mass_drift"], "dry_mass_rel_drift": x["dry_drift"], "finite": x["finite"]}
+    if "mixed" in res and "fp64" in res:
+        a_, b_ = res["mixed"]["final"], res["fp64"]["final"]
+        gas = slice(d.n_cond * d.s, (d.n_cond + 1) * d.s)
+        rel = lambda u, v: float(np.linalg.norm(u - v) / np.linalg.norm(v))
+        form_cfg["mixed"]["rel_l2_vs_fp64"] = {
+            "rho_gas": rel(a_.rho[gas], b_.rho[gas]), "rhotheta_v": rel(a_.rhotheta_v, b_.rhotheta_v),
+            "wind": rel(a_.wind, b_.wind), "after_steps": warmup + steps,
+            "after_sim_days": round((warmup + steps) * args.dt / 86400.0, 4)}
+        form_cfg["mixed"]["speedup_vs_fp64"] = round(res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"], 3)
+    finite = all(x["finite"] for x in res.values())
     out = {
         "metric": "sim-days/wall-hour res7x26L mixed vs FP64 (1 GPU); adjoint/forward cost ratio",
-        "value": round(sdph, 2),
+        "value": round(sdph(r["ms_per_step"]), 2),
         "unit": "sim-days/wall-hour",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if main_name == "fp64" else "f64 state and solvers, f32 work arrays",
         "data": "synthetic",
         "config": {
             "workload": f"RKHEVI step, icosahedral res {args.res} x {args.layers} layers, "
-                        f"{'moist (6 constituents)' if moist else 'dry'}, flat, dt {args.dt:g} s, FP64 form",
+                        f"{'moist (6 constituents)' if moist else 'dry'}, flat, dt {args.dt:g} s, "
+                        f"{main_name} form (value); forms timed: {', '.join(res)}",
             "cells": d.nsh, "edges": d.nvh, "layers": d.n_layers, "dt_s": round(args.dt, 3),
             "dt_rule": "reference auto time step 1.61e-3*eff_hor_res (src/coordinator.c:171)",
-            "precision_form": "fp64", "lloyd_iterations": args.lloyd,
-            "column_updates_per_s": round(d.nsh * args.steps / wall, 1),
-            "step_algorithmic_GBps": round(step_gbs, 1),
-            "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-            "gas_mass_rel_drift": float(mass_drift),
-            "dry_mass_rel_drift": float(dry_drift),
+            "lloyd_iterations": args.lloyd, "value_form": main_name, "forms": form_cfg,
             "finite": finite,
         },
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_us": round(dom_avg_s * 1e6, 2), "algorithmic_bytes": dom_bytes},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "form": main_name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_us": round(dom_avg_s * 1e6, 2), "algorithmic_bytes": dom_bytes},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -54,17 +54,25 @@ struct DevState {
     double *rho, *rt, *tp, *ep, *wind;   /* rho, rhotheta_v, theta_v_pert, exner_pert, wind */
 };
 
-struct DevWork {
+/* work arrays of the step; TW = double in the FP64 form, float in the mixed
+   form (register arithmetic stays FP64 in both) */
+template <typename TW>
+struct DevWorkT {
     /* diagnostics */
-    double *flux, *flux2, *flux_divv, *rvt, *pv, *temp, *vsq;
+    TW *flux, *rvt, *pv, *temp, *vsq;
+    double *flux2, *flux_divv;   /* reference diagnostics the step leaves untouched */
     /* forcings */
-    double *pgrad_old, *pgrad_nl, *pgrad_l, *pgc_v, *vsq_grad, *pvt;
+    TW *pgrad_old, *pgrad_nl, *pgrad_l, *pgc_v, *vsq_grad, *pvt;
     /* irreversible quantities */
-    double *ptr, *phr, *sed_heat;
+    TW *ptr, *phr, *sed_heat;
     /* vertical contravariant corrections of wind, gas mass flux and rhotheta_v
        flux in the terrain-following levels, 3 x n_oro x NSH (unused when flat) */
     double *vcc;
+    /* wind tendency (the FP64 form keeps it in the tendency state) */
+    TW *td_wind;
 };
+typedef DevWorkT<double> DevWork;
+typedef DevWorkT<float> DevWorkF;
 
 struct StepConsts {
     double dt, impl_weight, damping_start, damping_max, snow_v, rain_v, cloud_v;
@@ -74,6 +82,38 @@ __device__ __forceinline__ double density_total(const DevGrid &G, const double *
 {   /* derived_quantities.c:83-95: left-to-right sum from 0 */
     double r = 0.0;
     for (int k = 0; k < G.n_const; ++k) r += rho[k*G.s + i];
+    return r;
+}
+
+/* dictionary.c:111-190 */
+__device__ __forceinline__ double c_p_cond(int k, double t)
+{
+    if ((k & 1) == 0) {
+        if (t > GC_T_0) t = GC_T_0;
+        if (t < 20.0) t = 20.0;
+        double q = t/125.1;
+        double r = -2.0572 + 0.14644*t + 0.06163*t*exp(-(q*q));
+        return r/GC_M_V;
+    }
+    double tc = t - GC_T_0, r;
+    if (tc >= 0.0) {
+        if (tc > 35.0) tc = 35.0;
+        double a = tc - 35.0;
+        r = 0.9979 + 3.1e-6*(a*a) + 3.8e-9*pow(a, 4.0);
+    } else {
+        if (tc < -37.0) tc = -37.0;
+        r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
+    }
+    return 4186.8*r;
+}
+
+/* derived_quantities.c:97-115 */
+__device__ __forceinline__ double c_v_mass_weighted(const DevGrid &G, const double *rho, double T, i64 i)
+{
+    double r = 0.0;
+    for (int k = 0; k < G.n_cond; ++k) r += rho[k*G.s + i]*c_p_cond(k, T);
+    r += rho[(i64)G.n_cond*G.s + i]*GC_C_D_V;
+    if (G.moist == 1) r += rho[(i64)(G.n_cond + 1)*G.s + i]*GC_C_V_V;
     return r;
 }
 

@@ -28,10 +28,11 @@ __device__ __forceinline__ void layer_chunk(int n, int &l0, int &l1)
 /* ------------------------------------------------------------------ */
 
 /* raw vertical covariant gradient at an interior w point (gradient_operators.c:35-54) */
-__device__ __forceinline__ double grad_w_raw(const DevGrid &G, const double *f, int lev, int h)
+template <typename TF>
+__device__ __forceinline__ double grad_w_raw(const DevGrid &G, const TF *f, int lev, int h)
 {
     i64 vi = h + (i64)lev*G.vpl;
-    return (f[h + (i64)(lev - 1)*G.nsh] - f[h + (i64)lev*G.nsh])/G.nd[vi];
+    return ((double)f[h + (i64)(lev - 1)*G.nsh] - (double)f[h + (i64)lev*G.nsh])/G.nd[vi];
 }
 
 /* contravariant horizontal gradient at edge e of layer l (gradient_operators.c:14-33,
@@ -163,56 +164,26 @@ __device__ __forceinline__ double divv_h_tracer_cell(const DevGrid &G, const dou
     return 1.0/G.vol[i]*(comp_h + comp_v);
 }
 
-/* dictionary.c:111-190 */
-__device__ __forceinline__ double c_p_cond(int k, double t)
-{
-    if ((k & 1) == 0) {
-        if (t > GC_T_0) t = GC_T_0;
-        if (t < 20.0) t = 20.0;
-        double q = t/125.1;
-        double r = -2.0572 + 0.14644*t + 0.06163*t*exp(-(q*q));
-        return r/GC_M_V;
-    }
-    double tc = t - GC_T_0, r;
-    if (tc >= 0.0) {
-        if (tc > 35.0) tc = 35.0;
-        double a = tc - 35.0;
-        r = 0.9979 + 3.1e-6*(a*a) + 3.8e-9*pow(a, 4.0);
-    } else {
-        if (tc < -37.0) tc = -37.0;
-        r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
-    }
-    return 4186.8*r;
-}
-
-/* derived_quantities.c:97-115 */
-__device__ __forceinline__ double c_v_mass_weighted(const DevGrid &G, const double *rho, double T, i64 i)
-{
-    double r = 0.0;
-    for (int k = 0; k < G.n_cond; ++k) r += rho[k*G.s + i]*c_p_cond(k, T);
-    r += rho[(i64)G.n_cond*G.s + i]*GC_C_D_V;
-    if (G.moist == 1) r += rho[(i64)(G.n_cond + 1)*G.s + i]*GC_C_V_V;
-    return r;
-}
-
 /* ------------------------------------------------------------------ */
 /* kernels                                                              */
 /* ------------------------------------------------------------------ */
 
 /* derived_quantities.c:15-40 */
-__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, double *T)
+template <typename TO>
+__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *T)
 {
     i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
     if (i >= G.s) return;
     double t = (G.thbg[i] + s.tp[i])*(G.exbg[i] + s.ep[i]);
     if (G.moist)
         t = t/(1.0 + s.rho[(i64)(G.n_cond + 1)*G.s + i]/s.rho[(i64)G.n_cond*G.s + i]*(GC_M_D/GC_M_V - 1.0));
-    T[i] = t;
+    T[i] = (TO)t;
 }
 
 /* manage_pressure_gradient.c:17-87 at the interior w points (layers 1..L-1),
    fused with calc_pressure_grad_condensates_v */
-__global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWork W, int first_step, int with_pgrad)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad)
 {
     i64 n = (i64)(G.L - 1)*G.nsh;
     i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
@@ -223,7 +194,7 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWork 
     double rt_up = density_total(G, s.rho, up), rt_lo = density_total(G, s.rho, lo);
     double d_up = s.rho[gi + up]/rt_up, d_lo = s.rho[gi + lo]/rt_lo;
     if (with_pgrad) {
-        if (!first_step) W.pgrad_old[w] = -W.pgrad_nl[w] - W.pgrad_l[w];
+        if (!first_step) W.pgrad_old[w] = (TW)(-(double)W.pgrad_nl[w] - (double)W.pgrad_l[w]);
         double raw = (s.ep[up] - s.ep[lo])/G.nd[w];
         double p1_up = GC_C_D_P*(G.thbg[up] + s.tp[up]), p1_lo = GC_C_D_P*(G.thbg[lo] + s.tp[lo]);
         double nl = (0.5*(p1_up + p1_lo))*raw;
@@ -231,21 +202,21 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWork 
         double ll = (0.5*(p2_up + p2_lo))*G.exbg_grad[w];
         nl = (0.5*(d_up + d_lo))*nl;
         ll = (0.5*(d_up + d_lo))*ll;
-        W.pgrad_nl[w] = nl;
-        W.pgrad_l[w] = ll;
-        if (first_step) W.pgrad_old[w] = -nl - ll;
+        W.pgrad_nl[w] = (TW)nl;
+        W.pgrad_l[w] = (TW)ll;
+        if (first_step) W.pgrad_old[w] = (TW)(-nl - ll);
     }
     double c_up = s.rho[gi + up]/rt_up - 1.0, c_lo = s.rho[gi + lo]/rt_lo - 1.0;
-    W.pgc_v[w] = (0.5*(c_up + c_lo))*G.gm[w];
+    W.pgc_v[w] = (TW)((0.5*(c_up + c_lo))*G.gm[w]);
 }
 
 /* manage_pressure_gradient.c:17-72 at the edges: one thread per edge walks
    all layers, so the edge's table entries are read once */
-template <bool FLAT>
-__device__ __forceinline__ double grad_edge_t(const DevGrid &G, const double *f, int l, int e, int fr, int to)
+template <bool FLAT, typename TF>
+__device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int l, int e, int fr, int to)
 {
     i64 vi = G.nsh + (i64)l*G.vpl + e;
-    double g = (f[to + (i64)l*G.nsh] - f[fr + (i64)l*G.nsh])/G.nd[vi];
+    double g = ((double)f[to + (i64)l*G.nsh] - (double)f[fr + (i64)l*G.nsh])/G.nd[vi];
     if (!FLAT && l >= G.L - G.n_oro) {
         const double *ipw = G.ipw;
         double c = ipw[8*((i64)l*G.nsh + fr) + 6]*grad_w_raw(G, f, l, fr);
@@ -260,8 +231,8 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const double *f,
     return g;
 }
 
-template <bool FLAT>
-__global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWork W, int first_step)
+template <bool FLAT, typename TW>
+__global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step)
 {
     int e = blockIdx.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
@@ -269,9 +240,9 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWork 
     const i64 gi = (i64)G.n_cond*G.s;
     for (int l = 0; l < G.L; ++l) {
         i64 vi = G.nsh + (i64)l*G.vpl + e;
-        if (!first_step) W.pgrad_old[vi] = -W.pgrad_nl[vi] - W.pgrad_l[vi];
+        if (!first_step) W.pgrad_old[vi] = (TW)(-(double)W.pgrad_nl[vi] - (double)W.pgrad_l[vi]);
         i64 sf = (i64)l*G.nsh + fr, st = (i64)l*G.nsh + to;
-        double g = grad_edge_t<FLAT>(G, s.ep, l, e, fr, to);
+        double g = grad_edge_t<FLAT, double>(G, s.ep, l, e, fr, to);
         double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
         double nl = (0.5*(p1_t + p1_f))*g;
         double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
@@ -279,14 +250,15 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWork 
         double d_t = s.rho[gi + st]/density_total(G, s.rho, st), d_f = s.rho[gi + sf]/density_total(G, s.rho, sf);
         nl = (0.5*(d_t + d_f))*nl;
         ll = (0.5*(d_t + d_f))*ll;
-        W.pgrad_nl[vi] = nl;
-        W.pgrad_l[vi] = ll;
-        if (first_step) W.pgrad_old[vi] = -nl - ll;
+        W.pgrad_nl[vi] = (TW)nl;
+        W.pgrad_l[vi] = (TW)ll;
+        if (first_step) W.pgrad_old[vi] = (TW)(-nl - ll);
     }
 }
 
 /* scalar_times_vector(rho_gas, wind) into the mass flux density (multiplications.c:35-117) */
-__global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_gas, const double *wind, double *flux)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_gas, const double *wind, TW *flux)
 {
     i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
     i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
@@ -294,18 +266,19 @@ __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_
         int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
         i64 vi = G.nsh + (i64)l*G.vpl + e;
         double sv = 0.5*(rho_gas[G.to[e] + (i64)l*G.nsh] + rho_gas[G.from[e] + (i64)l*G.nsh]);
-        flux[vi] = sv*wind[vi];
+        flux[vi] = (TW)(sv*wind[vi]);
     } else if (t < ne + nw) {
         t -= ne;
         int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
         i64 w = (i64)lev*G.vpl + h;
         double sv = 0.5*(rho_gas[h + (i64)(lev - 1)*G.nsh] + rho_gas[h + (i64)lev*G.nsh]);
-        flux[w] = sv*wind[w];
+        flux[w] = (TW)(sv*wind[w]);
     }
 }
 
 /* vorticities.c:167-222, one thread per triangle walking all layers */
-__global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, double *res)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, TW *res)
 {
     int h = blockIdx.x*TPB + threadIdx.x;
     if (h >= G.nt) return;
@@ -329,7 +302,7 @@ __global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w
             }
             r += lrf*G.nd[vi]*vst[j]*vel;
         }
-        res[(i64)l*G.nt + h] = r/G.darea[G.nvh + (i64)l*G.dvpl + h];
+        res[(i64)l*G.nt + h] = (TW)(r/G.darea[G.nvh + (i64)l*G.dvpl + h]);
     }
 }
 
@@ -371,9 +344,9 @@ __device__ __forceinline__ double rel_vort_tangential(const DevGrid &G, const do
    interpolated density (vorticities.c:18-165).  One thread per edge walks all
    levels and writes the tangential element of each level and the rhombus
    element of each layer. */
-template <bool FLAT>
-__global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, const double *rho, const double *rvt,
-                                                  double *pv)
+template <bool FLAT, typename TW>
+__global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, const double *rho, const TW *rvt,
+                                                  TW *pv)
 {
     int e = blockIdx.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
@@ -401,20 +374,21 @@ __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, co
             i64 uf = (i64)(l - 1)*nsh + fr, ut = (i64)(l - 1)*nsh + to;
             dv = 0.25*(rho[uf] + rho[ut] + rho[uf + nsh] + rho[ut + nsh]);
         }
-        pv[(i64)l*2*nvh + e] = (rv + f_t)/dv;
+        pv[(i64)l*2*nvh + e] = (TW)((rv + f_t)/dv);
         if (l == L) break;
         /* rhombus element of layer l */
         i64 base = nvh + (i64)l*G.dvpl;
         double af = G.darea[base + fd], at = G.darea[base + td];
-        double rr = (af*rvt[(i64)l*G.nt + fd] + at*rvt[(i64)l*G.nt + td])/(af + at);
+        double rr = (af*(double)rvt[(i64)l*G.nt + fd] + at*(double)rvt[(i64)l*G.nt + td])/(af + at);
         double dr = 0;
         for (int j = 0; j < 4; ++j) dr += drw[j]*rho[(i64)l*nsh + dri[j]];
-        pv[(i64)l*2*nvh + nvh + e] = (rr + f_r)/dr;
+        pv[(i64)l*2*nvh + nvh + e] = (TW)((rr + f_r)/dr);
     }
 }
 
 /* vorticity_flux.c:16-187, horizontal part: one thread per edge, all layers */
-__global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const double *F, const double *pv, double *out)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F, const TW *pv, TW *out)
 {
     int e = blockIdx.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
@@ -429,33 +403,34 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const doubl
         i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
         double o = 0;
         if (pent_f) {
-            for (int j = 0; j < 4; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+            for (int j = 0; j < 4; ++j) o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
         } else {
             for (int j = 0; j < 5; ++j) {
-                if (j == 2) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
-                else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+                if (j == 2) o += tw[j]*(double)F[fb + ti[j]]*0.5*((double)pv[pb + tm[j]] + (double)pv[pb + e]);
+                else o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
             }
         }
         if (pent_t) {
-            for (int j = 5; j < 9; ++j) o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+            for (int j = 5; j < 9; ++j) o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
         } else {
             for (int j = 5; j < 10; ++j) {
-                if (j == 7) o += tw[j]*F[fb + ti[j]]*0.5*(pv[pb + tm[j]] + pv[pb + e]);
-                else o += tw[j]*F[fb + ti[j]]*pv[pb + tm[j]];
+                if (j == 7) o += tw[j]*(double)F[fb + ti[j]]*0.5*((double)pv[pb + tm[j]] + (double)pv[pb + e]);
+                else o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
             }
         }
         i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
-        double q0 = pv[e + (i64)l*2*nvh], q1 = pv[e + (i64)(l + 1)*2*nvh];
-        o -= 0.5*ipw[8*sf + 6]*F[(i64)l*G.vpl + fr]*q0;
-        o -= 0.5*ipw[8*st + 6]*F[(i64)l*G.vpl + to]*q0;
-        o -= 0.5*ipw[8*sf + 7]*F[(i64)(l + 1)*G.vpl + fr]*q1;
-        o -= 0.5*ipw[8*st + 7]*F[(i64)(l + 1)*G.vpl + to]*q1;
-        out[nsh + (i64)l*G.vpl + e] = o;
+        double q0 = (double)pv[e + (i64)l*2*nvh], q1 = (double)pv[e + (i64)(l + 1)*2*nvh];
+        o -= 0.5*ipw[8*sf + 6]*(double)F[(i64)l*G.vpl + fr]*q0;
+        o -= 0.5*ipw[8*st + 6]*(double)F[(i64)l*G.vpl + to]*q0;
+        o -= 0.5*ipw[8*sf + 7]*(double)F[(i64)(l + 1)*G.vpl + fr]*q1;
+        o -= 0.5*ipw[8*st + 7]*(double)F[(i64)(l + 1)*G.vpl + to]*q1;
+        out[nsh + (i64)l*G.vpl + e] = (TW)o;
     }
 }
 
 /* vorticity_flux.c:373-417, vertical part: one thread per cell, levels 0..L */
-__global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const double *F, const double *pv, double *out)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F, const TW *pv, TW *out)
 {
     int h = blockIdx.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
@@ -471,16 +446,19 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const doubl
         double o = 0;
         if (l >= 1)
             for (int j = 0; j < ne; ++j)
-                o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*F[nsh + (i64)(l - 1)*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
+                o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
+                     *(double)pv[(i64)l*2*nvh + adj[j]];
         if (l <= G.L - 1)
             for (int j = 0; j < ne; ++j)
-                o += vw*G.ipw[8*((i64)l*nsh + h) + j]*F[nsh + (i64)l*G.vpl + adj[j]]*pv[(i64)l*2*nvh + adj[j]];
-        out[(i64)l*G.vpl + h] = o;
+                o += vw*G.ipw[8*((i64)l*nsh + h) + j]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
+                     *(double)pv[(i64)l*2*nvh + adj[j]];
+        out[(i64)l*G.vpl + h] = (TW)o;
     }
 }
 
 /* inner_product.c:13-43, one thread per cell, all layers */
-__global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *a, const double *b, double *out)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *a, const double *b, TW *out)
 {
     int h = blockIdx.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
@@ -499,33 +477,35 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
         }
         o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
         o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
-        out[i] = o;
+        out[i] = (TW)o;
     }
 }
 
 /* grad (gradient_operators.c:56-74) of a scalar field: edges (one thread per
    edge, all layers) */
-template <bool FLAT>
-__global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const double *f, double *out)
+template <bool FLAT, typename TW>
+__global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const TW *f, TW *out)
 {
     int e = blockIdx.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int fr = G.from[e], to = G.to[e];
-    for (int l = 0; l < G.L; ++l) out[G.nsh + (i64)l*G.vpl + e] = grad_edge_t<FLAT>(G, f, l, e, fr, to);
+    for (int l = 0; l < G.L; ++l) out[G.nsh + (i64)l*G.vpl + e] = (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to);
 }
 
 /* grad, interior w points */
-__global__ void __launch_bounds__(TPB) k_grad_w(DevGrid G, const double *f, double *out)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_grad_w(DevGrid G, const TW *f, TW *out)
 {
     i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
     if (t >= (i64)(G.L - 1)*G.nsh) return;
     int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
-    out[h + (i64)lev*G.vpl] = grad_w_raw(G, f, lev, h);
+    out[h + (i64)lev*G.vpl] = (TW)grad_w_raw(G, f, lev, h);
 }
 
 /* vector_tendencies_expl.c:74-131 fused with the horizontal wind update of
    manage_rkhevi.c:67-75 */
-__global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWork W, double *tend_w, const double *wind_old,
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWorkT<TW> W, TW *tend_w, const double *wind_old,
                                                      double *wind_new, double new_w, double chw, double ohw,
                                                      double cvw, double dt)
 {
@@ -534,22 +514,22 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWork W, doubl
     double old_w = 1.0 - new_w;
     int l = (int)(i/G.vpl);
     int h = (int)(i - (i64)l*G.vpl);
-    if (i < G.nsh || i >= G.v - G.nsh) { tend_w[i] = 0.0; return; }
+    if (i < G.nsh || i >= G.v - G.nsh) { tend_w[i] = (TW)0.0; return; }
     /* friction_acc is zero with momentum diffusion and PBL off: x + 0.0 == x */
     if (h >= G.nsh) {
-        double tv = old_w*tend_w[i] + new_w*(
-            ohw*W.pgrad_old[i]
-            - chw*(W.pgrad_nl[i] + W.pgrad_l[i])
-            + W.pvt[i]
-            - 0.5*W.vsq_grad[i]);
-        tend_w[i] = tv;
+        double tv = old_w*(double)tend_w[i] + new_w*(
+            ohw*(double)W.pgrad_old[i]
+            - chw*((double)W.pgrad_nl[i] + (double)W.pgrad_l[i])
+            + (double)W.pvt[i]
+            - 0.5*(double)W.vsq_grad[i]);
+        tend_w[i] = (TW)tv;
         wind_new[i] = wind_old[i] + dt*tv;
     } else {
-        tend_w[i] = old_w*tend_w[i] + new_w*(
-            -cvw*(W.pgrad_nl[i] + W.pgrad_l[i])
-            + W.pvt[i]
-            - 0.5*W.vsq_grad[i]
-            + W.pgc_v[i]);
+        tend_w[i] = (TW)(old_w*(double)tend_w[i] + new_w*(
+            -cvw*((double)W.pgrad_nl[i] + (double)W.pgrad_l[i])
+            + (double)W.pvt[i]
+            - 0.5*(double)W.vsq_grad[i]
+            + (double)W.pgc_v[i]));
     }
 }
 
@@ -593,8 +573,8 @@ __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
    their divergences (divergences.c:14-148) and the RK-weighted tendencies.
    One thread per cell walks all layers; the edge fluxes are formed on the fly
    from the reference's own formulas, so no flux array is written. */
-template <bool FLAT>
-__global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevWork W, double *td_rho,
+template <bool FLAT, typename TW>
+__global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevWorkT<TW> W, double *td_rho,
                                                      double *td_rt, int rk)
 {
     int h = blockIdx.x*TPB + threadIdx.x;
@@ -667,11 +647,11 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 } else {
                     /* the moist-air gas takes the vapour phase-transition rate
                        (scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
-                    double ptr_v = W.ptr[shift + i];
+                    double ptr_v = (double)W.ptr[shift + i];
                     double th = s.rt[i]/rg;
                     td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
-                    double cv = c_v_mass_weighted(G, s.rho, W.temp[i], i);
-                    double heat = W.phr[i] + W.sed_heat[i];
+                    double cv = c_v_mass_weighted(G, s.rho, (double)W.temp[i], i);
+                    double heat = (double)W.phr[i] + (double)W.sed_heat[i];
                     td_rt[i] = old_w*td_rt[i] + new_w*(
                         -div2
                         + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[i] + s.ep[i]))
@@ -702,7 +682,7 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 }
                 double div = 1.0/vol*(c + v);
                 i64 shift_pt = (k == ncc + 1) ? shift - S : shift;
-                td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div + W.ptr[shift_pt + i]);
+                td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div + (double)W.ptr[shift_pt + i]);
             }
         }
     }
@@ -873,8 +853,9 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
 #define VW_NCOL 64
 #define VW_THREADS 256
 
+template <typename TWIND>
 __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevState so, DevState sn, DevState td,
-                                                              StepConsts K, int rk)
+                                                              const TWIND *td_wind, StepConsts K, int rk)
 {
     extern __shared__ double lds[];
     const int L = G.L, nsh = G.nsh, vpl = G.vpl;
@@ -901,7 +882,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
                          + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
                          - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
                            *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
-        const double r = -(w_old + dt*td.wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+        const double r = -(w_old + dt*(double)td_wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
                          + tin*(ca.epe - cb.epe)/dt
                          + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
                          - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
@@ -1106,13 +1087,15 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
 /* sediment heat of the condensates at rk 0 (column_solvers.c:414-465 and the
    sh updates of :500-611): one thread per grid box adds the contributions of
    the condensates in the reference's constituent order, starting from 0 */
-__global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, DevWork W, StepConsts K)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, const TW *T, TW *sed_heat,
+                                                  StepConsts K)
 {
     const i64 b = (i64)blockIdx.x*TPB + threadIdx.x;
     if (b >= G.s) return;
     const int nsh = G.nsh, vpl = G.vpl, L = G.L, ncc = G.n_cond;
     const int j = (int)(b/nsh), i = (int)(b - (i64)j*nsh);
-    const double *vol = G.vol, *area = G.area, *T = W.temp;
+    const double *vol = G.vol, *area = G.area;
     double sh = 0.0;
     for (int k = 0; k < ncc; ++k) {
         const double *rho_o = so.rho + (i64)k*G.s;
@@ -1129,8 +1112,8 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
             fr = area[i + (i64)(jj + 1)*vpl]*fr;
             const i64 bb = i + (i64)jj*nsh;
             double doi, toi;
-            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = T[bb + nsh]; }
-            else { doi = rho_o[bb]; toi = T[bb]; }
+            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = (double)T[bb + nsh]; }
+            else { doi = rho_o[bb]; toi = (double)T[bb]; }
             fr = doi*fr;
             const double f = c_p_cond(k, toi)*toi*fr;
             if (pass == 0) fe = f; else fe_m = f;
@@ -1139,10 +1122,11 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
         else if (j == L - 1) {
             const i64 low = i + G.s - nsh, sfc = i + G.v - nsh;
             sh += -fe_m/vol[b];
-            sh += -vsink*T[low]*c_p_cond(k, T[low])*rho_o[low]*area[sfc]/vol[b];
+            const double tl = (double)T[low];
+            sh += -vsink*tl*c_p_cond(k, tl)*rho_o[low]*area[sfc]/vol[b];
         } else sh += (-fe_m + fe)/vol[b];
     }
-    W.sed_heat[b] = sh;
+    sed_heat[b] = (TW)sh;
 }
 
 /* phase_trans.c:16-311 (surface phase transitions off), one thread per grid box */
@@ -1217,14 +1201,15 @@ __device__ __forceinline__ double phase_trans_heat(int dir, double t)
     return enthalpy_sublimation(t) - enthalpy_evaporation(t);
 }
 
-__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, DevWork W, double dt)
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, const TW *temp, TW *ptr, TW *phr,
+                                                     double dt)
 {
     i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
     if (i >= G.s) return;
     const i64 S = G.s;
     const double *rho = s.rho;
-    double *ptr = W.ptr;
-    double T = W.temp[i];
+    double T = (double)temp[i];
     const double mcwc = 0.2e-3;
     int ncc = G.n_cond;
     double sp = (T >= GC_T_0) ? sat_water(T) : sat_ice(T);
@@ -1290,12 +1275,12 @@ __global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, DevW
         r0 -= r1;
         heat += -r1*phase_trans_heat(2, T);
     }
-    ptr[i] = r0;
-    ptr[S + i] = r1;
-    ptr[2*S + i] = r2;
-    ptr[3*S + i] = r3;
-    ptr[4*S + i] = r4;
-    W.phr[i] = heat;
+    ptr[i] = (TW)r0;
+    ptr[S + i] = (TW)r1;
+    ptr[2*S + i] = (TW)r2;
+    ptr[3*S + i] = (TW)r3;
+    ptr[4*S + i] = (TW)r4;
+    phr[i] = (TW)heat;
 }
 
 /* linear_combine_two_states.c:14-43 */
@@ -1354,68 +1339,124 @@ __global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *
 /* ------------------------------------------------------------------ */
 /* host-side launchers                                                  */
 /* ------------------------------------------------------------------ */
-namespace fp64 {
+/* horizontal operators and tendencies; TW is the work-array type */
+namespace hor {
 
 #define FLAT_DISPATCH(NAME, KERNEL, GRIDDIM, ...)                                                 \
     do {                                                                                      \
-        if (G.flat) LAUNCH(NAME, KERNEL<true>, GRIDDIM, dim3(TPB), st, __VA_ARGS__);          \
-        else LAUNCH(NAME, KERNEL<false>, GRIDDIM, dim3(TPB), st, __VA_ARGS__);                \
+        if (G.flat) LAUNCH(NAME, (KERNEL<true, TW>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);    \
+        else LAUNCH(NAME, (KERNEL<false, TW>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);          \
     } while (0)
 
-void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st)
-{ LAUNCH("k_temperature", k_temperature, GRID1(G.s), dim3(TPB), st, G, s, T); }
-
-void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
+template <typename TW>
+static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, hipStream_t st)
 {
-    LAUNCH("k_pgrad_w", k_pgrad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
+    LAUNCH("k_pgrad_w", k_pgrad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
     FLAT_DISPATCH("k_pgrad_e", k_pgrad_e, GRID1(G.nvh), G, s, W, first);
 }
 
-void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st)
-{ LAUNCH("k_pgrad_w", k_pgrad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
+template <typename TW>
+static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, hipStream_t st)
+{ LAUNCH("k_pgrad_w", k_pgrad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
 
-void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)
+template <typename TW>
+static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
+                        hipStream_t st)
 {
-    LAUNCH("k_mass_flux", k_mass_flux, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas, wind, W.flux);
-    LAUNCH("k_rel_vort_tri", k_rel_vort_tri, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
-    FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID1(G.nvh), G, wind, rho_gas, (const double *)W.rvt, W.pv);
-    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e, GRID1(G.nvh), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, W.flux, W.pv, W.pvt);
-    LAUNCH("k_inner_product", k_inner_product, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, wind, wind, W.vsq);
-    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, (const double *)W.vsq, W.vsq_grad);
-    LAUNCH("k_grad_w", k_grad_w, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, W.vsq, W.vsq_grad);
+    LAUNCH("k_mass_flux", k_mass_flux<TW>, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas,
+           wind, W.flux);
+    LAUNCH("k_rel_vort_tri", k_rel_vort_tri<TW>, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
+    FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID1(G.nvh), G, wind, rho_gas, (const TW *)W.rvt, W.pv);
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<TW>, GRID1(G.nvh), dim3(TPB), st, G, (const TW *)W.flux,
+           (const TW *)W.pv, W.pvt);
+    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G,
+           (const TW *)W.flux, (const TW *)W.pv, W.pvt);
+    LAUNCH("k_inner_product", k_inner_product<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, wind, wind, W.vsq);
+    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, (const TW *)W.vsq, W.vsq_grad);
+    LAUNCH("k_grad_w", k_grad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, (const TW *)W.vsq, W.vsq_grad);
 }
 
-void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
-{ LAUNCH("k_inner_product", k_inner_product, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, a, b, out); }
-
-void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
-                 double new_w, double itw, double dt, hipStream_t st)
+template <typename TW>
+static void vector_tend_t(const DevGrid &G, const DevWorkT<TW> &W, TW *tend_w, const double *w_old, double *w_new,
+                          double new_w, double itw, double dt, hipStream_t st)
 {
     double chw = 0.5 + itw, ohw = 1.0 - chw, cvw = 1.0 - itw;
-    LAUNCH("k_vector_tend", k_vector_tend, GRID1(G.v), dim3(TPB), st, G, W, tend_w, w_old, w_new, new_w, chw, ohw, cvw, dt);
+    LAUNCH("k_vector_tend", k_vector_tend<TW>, GRID1(G.v), dim3(TPB), st, G, W, tend_w, w_old, w_new, new_w, chw, ohw,
+           cvw, dt);
 }
 
-void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st)
+template <typename TW>
+static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkT<TW> &W, int rk,
+                          hipStream_t st)
 {
     if (!G.flat)
         LAUNCH("k_vcc", k_vcc, GRID1((i64)G.n_oro*G.nsh), dim3(TPB), st, G, s, W.vcc);
     FLAT_DISPATCH("k_scalar_tend", k_scalar_tend, GRID2(G.nsh, G.split_cell), G, s, W, td.rho, td.rt, rk);
 }
 
+void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
+{ pgrad_t<double>(G, s, W, first, st); }
+void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, hipStream_t st)
+{ pgrad_t<float>(G, s, W, first, st); }
+void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st) { pgc_v_t<double>(G, s, W, st); }
+void pgc_v(const DevGrid &G, const DevState &s, const DevWorkF &W, hipStream_t st) { pgc_v_t<float>(G, s, W, st); }
+void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)
+{ advection_t<double>(G, wind, rho_gas, W, st); }
+void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st)
+{ advection_t<float>(G, wind, rho_gas, W, st); }
+void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
+                 double new_w, double itw, double dt, hipStream_t st)
+{ vector_tend_t<double>(G, W, tend_w, w_old, w_new, new_w, itw, dt, st); }
+void vector_tend(const DevGrid &G, const DevWorkF &W, float *tend_w, const double *w_old, double *w_new,
+                 double new_w, double itw, double dt, hipStream_t st)
+{ vector_tend_t<float>(G, W, tend_w, w_old, w_new, new_w, itw, dt, st); }
+void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st)
+{ scalar_tend_t<double>(G, s, td, W, rk, st); }
+void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkF &W, int rk, hipStream_t st)
+{ scalar_tend_t<float>(G, s, td, W, rk, st); }
+
+void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
+{ LAUNCH("k_inner_product", k_inner_product<double>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, a, b, out); }
+
+}  // namespace hor
+
+/* the column solvers and point physics, shared by both precision forms (the
+   work arrays they read or write are double in the FP64 form, float in the
+   mixed form) */
+namespace col {
+
+void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<double>, GRID1(G.s), dim3(TPB), st, G, s, T); }
+void temperature(const DevGrid &G, const DevState &s, float *T, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<float>, GRID1(G.s), dim3(TPB), st, G, s, T); }
+
+void phase_trans(const DevGrid &G, const DevState &s, const double *T, double *ptr, double *phr, double dt,
+                 hipStream_t st)
+{ LAUNCH("k_phase_trans", k_phase_trans<double>, GRID1(G.s), dim3(TPB), st, G, s, T, ptr, phr, dt); }
+void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr, float *phr, double dt,
+                 hipStream_t st)
+{ LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, ptr, phr, dt); }
+
+template <typename TWIND>
+static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
+                          const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
+{
+    dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
+    size_t lds = (size_t)5*G.L*VW_NCOL*sizeof(double);   /* 65 KiB at 26 layers, 160 KiB at 64 */
+    static size_t lds_allowed = 65536;
+    if (lds > lds_allowed) {
+        (void)hipFuncSetAttribute((const void *)k_ver_waves_lds<TWIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        lds_allowed = lds;
+    }
+    LAUNCH_SHM("k_ver_waves", k_ver_waves_lds<TWIND>, g, b, lds, st, G, so, sn, td, td_wind, K, rk);
+}
+
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
                int rk, hipStream_t st)
 {
     if (!G.vw_streaming) {
-        dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
-        size_t lds = (size_t)5*G.L*VW_NCOL*sizeof(double);   /* 65 KiB at 26 layers, 160 KiB at 64 */
-        static size_t lds_allowed = 65536;
-        if (lds > lds_allowed) {
-            (void)hipFuncSetAttribute((const void *)k_ver_waves_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds);
-            lds_allowed = lds;
-        }
-        LAUNCH_SHM("k_ver_waves", k_ver_waves_lds, g, b, lds, st, G, so, sn, td, K, rk);
+        ver_waves_lds<double>(G, so, sn, td, td.wind, K, rk, st);
         return;
     }
     dim3 g((G.nsh + 127)/128), b(128);
@@ -1424,10 +1465,13 @@ void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const D
     else LAUNCH("k_ver_waves", k_ver_waves<0>, g, b, st, G, so, sn, td, K, rk);
 }
 
-void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const DevWork &W,
-                   const StepConsts &K, int rk, hipStream_t st)
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *td_wind,
+               const StepConsts &K, int rk, hipStream_t st)
+{ ver_waves_lds<float>(G, so, sn, td, td_wind, K, rk, st); }
+
+static void gen_densities_only(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
+                               const StepConsts &K, hipStream_t st)
 {
-    if (rk == 0 && G.n_cond > 0) LAUNCH("k_sed_heat", k_sed_heat, GRID1(G.s), dim3(TPB), st, G, so, sn, W, K);
     dim3 g((G.nsh + GD_NCOL - 1)/GD_NCOL, G.n_const - 1), b(GD_THREADS);
     size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
     static size_t lds_allowed = 65536;
@@ -1438,8 +1482,21 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
     LAUNCH_SHM("k_gen_densities", k_gen_densities, g, b, lds, st, G, so, sn, td, K);
 }
 
-void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st)
-{ LAUNCH("k_phase_trans", k_phase_trans, GRID1(G.s), dim3(TPB), st, G, s, W, dt); }
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
+                   double *sed_heat, const StepConsts &K, int rk, hipStream_t st)
+{
+    if (rk == 0 && G.n_cond > 0)
+        LAUNCH("k_sed_heat", k_sed_heat<double>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, sed_heat, K);
+    gen_densities_only(G, so, sn, td, K, st);
+}
+
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,
+                   float *sed_heat, const StepConsts &K, int rk, hipStream_t st)
+{
+    if (rk == 0 && G.n_cond > 0)
+        LAUNCH("k_sed_heat", k_sed_heat<float>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, sed_heat, K);
+    gen_densities_only(G, so, sn, td, K, st);
+}
 
 void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
                     hipStream_t st)
@@ -1456,4 +1513,4 @@ int integral_partials(i64 n, const double *x, const double *y, const double *vol
     return blocks;
 }
 
-}  // namespace fp64
+}  // namespace col

@@ -7,20 +7,41 @@
 
 #include "rkhevi_dev.h"
 
-namespace fp64 {
-void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st);
+/* horizontal operators and tendencies: work arrays double (FP64 form) or
+   float (mixed form), register arithmetic FP64 in both */
+namespace hor {
 void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st);
+void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, hipStream_t st);
 void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st);
+void pgc_v(const DevGrid &G, const DevState &s, const DevWorkF &W, hipStream_t st);
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st);
-void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st);
+void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st);
 void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
                  double new_w, double itw, double dt, hipStream_t st);
+void vector_tend(const DevGrid &G, const DevWorkF &W, float *tend_w, const double *w_old, double *w_new,
+                 double new_w, double itw, double dt, hipStream_t st);
 void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st);
+void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkF &W, int rk, hipStream_t st);
+void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st);
+}
+
+/* column solvers, point physics and state arithmetic shared by both forms:
+   FP64 arithmetic; the work arrays are double (FP64 form) or float (mixed) */
+namespace col {
+void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st);
+void temperature(const DevGrid &G, const DevState &s, float *T, hipStream_t st);
+void phase_trans(const DevGrid &G, const DevState &s, const double *T, double *ptr, double *phr, double dt,
+                 hipStream_t st);
+void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr, float *phr, double dt,
+                 hipStream_t st);
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
                int rk, hipStream_t st);
-void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const DevWork &W,
-                   const StepConsts &K, int rk, hipStream_t st);
-void phase_trans(const DevGrid &G, const DevState &s, const DevWork &W, double dt, hipStream_t st);
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *td_wind,
+               const StepConsts &K, int rk, hipStream_t st);
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
+                   double *sed_heat, const StepConsts &K, int rk, hipStream_t st);
+void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,
+                   float *sed_heat, const StepConsts &K, int rk, hipStream_t st);
 void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
                     hipStream_t st);
 int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st);

@@ -15,6 +15,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "../../include/game_rkhevi.h"
 #include "rkhevi_dev.h"
@@ -91,7 +92,8 @@ struct Model {
     int precision = GAME_FP64;
     DevGrid G;
     DevState s[3];
-    DevWork W;
+    DevWork W;      /* FP64 form */
+    DevWorkF WF;    /* mixed form */
     StepConsts K;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
@@ -122,6 +124,14 @@ int upload(const T **dst, const T *src, size_t count)
     if (src) HIP_OK(hipMemcpyAsync(p, src, count*sizeof(T), hipMemcpyHostToDevice, M->stream));
     *dst = (const T *)p;
     return 0;
+}
+
+int fzeros(float **dst, size_t count)
+{
+    void *p;
+    int rc = dalloc(&p, count*sizeof(float));
+    *dst = (float *)p;
+    return rc;
 }
 
 int dzeros(double **dst, size_t count)
@@ -194,7 +204,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
                  d.n_layers, g->no_of_oro_layers);
         return GAME_ERR_ARG;
     }
-    if (precision != GAME_FP64) {
+    if (precision != GAME_FP64 && precision != GAME_MIXED) {
         snprintf(g_err, sizeof g_err, "precision mode %d is not available", precision);
         return GAME_ERR_ARG;
     }
@@ -280,15 +290,35 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if ((rc = dzeros(&s.ep, S))) return rc;
         if ((rc = dzeros(&s.wind, V))) return rc;
     }
-    DevWork &W = M->W;
-    double **vfields[] = {&W.flux, &W.flux2, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad, &W.pvt};
-    for (double **f : vfields) if ((rc = dzeros(f, V))) return rc;
-    double **sfields[] = {&W.flux_divv, &W.temp, &W.vsq, &W.phr, &W.sed_heat, &M->tmp_s};
-    for (double **f : sfields) if ((rc = dzeros(f, S))) return rc;
-    if ((rc = dzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
-    if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
-    if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
-    if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+    if ((rc = dzeros(&M->tmp_s, S))) return rc;
+    if (precision == GAME_FP64) {
+        DevWork &W = M->W;
+        double **vfields[] = {&W.flux, &W.flux2, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad,
+                              &W.pvt};
+        for (double **f : vfields) if ((rc = dzeros(f, V))) return rc;
+        double **sfields[] = {&W.flux_divv, &W.temp, &W.vsq, &W.phr, &W.sed_heat};
+        for (double **f : sfields) if ((rc = dzeros(f, S))) return rc;
+        if ((rc = dzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
+        if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
+        if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
+        if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+        W.td_wind = M->s[GAME_STATE_TENDENCY].wind;
+    } else {
+        /* mixed: every work array FP32; the tendency state keeps FP64 densities
+           and rhotheta_v, the wind tendency is FP32 */
+        DevWorkF &W = M->WF;
+        float **vfields[] = {&W.flux, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad, &W.pvt,
+                             &W.td_wind};
+        for (float **f : vfields) if ((rc = fzeros(f, V))) return rc;
+        float **sfields[] = {&W.temp, &W.vsq, &W.phr, &W.sed_heat};
+        for (float **f : sfields) if ((rc = fzeros(f, S))) return rc;
+        if ((rc = fzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
+        if ((rc = fzeros(&W.pv, (size_t)d.curl))) return rc;
+        if ((rc = fzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+        if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
+        if ((rc = dzeros(&W.flux2, 8))) return rc;
+        if ((rc = dzeros(&W.flux_divv, S))) return rc;
+    }
     if ((rc = dzeros(&M->partials, 2*4096))) return rc;
     HIP_OK(hipHostMalloc((void **)&M->h_partials, sizeof(double)*2*4096, hipHostMallocDefault));
     StepConsts &K = M->K;
@@ -342,71 +372,113 @@ static int d2h(double *host, const double *dev, size_t n)
     return 0;
 }
 
-int game_gpu_get_forcings(game_forcings_view_t *h)
+}  /* extern "C" */
+
+/* FP32 work array of the mixed form, widened to the caller's doubles */
+static int d2h(double *host, const float *dev, size_t n)
 {
-    if (!M) return GAME_ERR_NOT_INIT;
+    if (!host) return 0;
+    std::vector<float> tmp(n);
+    HIP_OK(hipMemcpyAsync(tmp.data(), dev, n*sizeof(float), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    for (size_t i = 0; i < n; ++i) host[i] = (double)tmp[i];
+    return 0;
+}
+
+template <class WK>
+static int get_forcings_t(const WK &W, game_forcings_view_t *h)
+{
     size_t V = (size_t)M->d.v;
     int rc;
-    if ((rc = d2h(h->pgrad_acc_old, M->W.pgrad_old, V))) return rc;
-    if ((rc = d2h(h->pressure_gradient_acc_neg_nl, M->W.pgrad_nl, V))) return rc;
-    if ((rc = d2h(h->pressure_gradient_acc_neg_l, M->W.pgrad_l, V))) return rc;
-    if ((rc = d2h(h->pressure_grad_condensates_v, M->W.pgc_v, V))) return rc;
-    if ((rc = d2h(h->v_squared_grad, M->W.vsq_grad, V))) return rc;
-    if ((rc = d2h(h->pot_vort_tend, M->W.pvt, V))) return rc;
+    if ((rc = d2h(h->pgrad_acc_old, W.pgrad_old, V))) return rc;
+    if ((rc = d2h(h->pressure_gradient_acc_neg_nl, W.pgrad_nl, V))) return rc;
+    if ((rc = d2h(h->pressure_gradient_acc_neg_l, W.pgrad_l, V))) return rc;
+    if ((rc = d2h(h->pressure_grad_condensates_v, W.pgc_v, V))) return rc;
+    if ((rc = d2h(h->v_squared_grad, W.vsq_grad, V))) return rc;
+    if ((rc = d2h(h->pot_vort_tend, W.pvt, V))) return rc;
     if (h->radiation_tendency) memset(h->radiation_tendency, 0, sizeof(double)*(size_t)M->d.s);
     HIP_OK(hipStreamSynchronize(M->stream));
     return 0;
 }
 
+template <class WK>
+static int get_diagnostics_t(const WK &W, game_diag_view_t *h)
+{
+    int rc;
+    if ((rc = d2h(h->flux_density, W.flux, (size_t)M->d.v))) return rc;
+    if ((rc = d2h(h->flux_density_divv, W.flux_divv, (size_t)M->d.s))) return rc;
+    if ((rc = d2h(h->rel_vort_on_triangles, W.rvt, (size_t)M->d.dual_v_vectors))) return rc;
+    if ((rc = d2h(h->pot_vort, W.pv, (size_t)M->d.curl))) return rc;
+    if ((rc = d2h(h->temperature, W.temp, (size_t)M->d.s))) return rc;
+    if ((rc = d2h(h->v_squared, W.vsq, (size_t)M->d.s))) return rc;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+template <class WK>
+static int get_irrev_t(const WK &W, game_irrev_view_t *h)
+{
+    int rc;
+    size_t S = (size_t)M->d.s;
+    if ((rc = d2h(h->condensates_sediment_heat, W.sed_heat, S))) return rc;
+    if ((rc = d2h(h->phase_trans_rates, W.ptr, (size_t)(M->d.n_cond + 1)*S))) return rc;
+    if ((rc = d2h(h->phase_trans_heating_rate, W.phr, S))) return rc;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+/* manage_rkhevi.c:20-100 on the device-resident states; td_wind is the
+   tendency state's wind (FP64 form) or the FP32 wind tendency (mixed) */
+template <class WK, typename TW>
+static int step_t(const WK &W, TW *td_wind, double dt, int first)
+{
+    hipStream_t st = M->stream;
+    const DevGrid &G = M->G;
+    DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
+    M->K.dt = dt;
+    const double *rho_gas_new = sn.rho + (i64)G.n_cond*G.s;
+    col::temperature(G, so, W.temp, st);
+    if (G.moist) col::phase_trans(G, so, W.temp, W.ptr, W.phr, dt, st);
+    for (int rk = 0; rk < 2; ++rk) {
+        if (rk == 0) hor::pgrad(G, sn, W, first, st);
+        else hor::pgc_v(G, sn, W, st);
+        if (rk == 1 || first) hor::advection(G, sn.wind, rho_gas_new, W, st);
+        hor::vector_tend(G, W, td_wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
+        hor::scalar_tend(G, sn, td, W, rk, st);
+        if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, sn, td, M->K, rk, st);
+        else col::ver_waves(G, so, sn, td, (const float *)td_wind, M->K, rk, st);
+        if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.sed_heat, M->K, rk, st);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+extern "C" {
+
+int game_gpu_get_forcings(game_forcings_view_t *h)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    return M->precision == GAME_MIXED ? get_forcings_t(M->WF, h) : get_forcings_t(M->W, h);
+}
+
 int game_gpu_get_diagnostics(game_diag_view_t *h)
 {
     if (!M) return GAME_ERR_NOT_INIT;
-    int rc;
-    if ((rc = d2h(h->flux_density, M->W.flux, (size_t)M->d.v))) return rc;
-    if ((rc = d2h(h->flux_density_divv, M->W.flux_divv, (size_t)M->d.s))) return rc;
-    if ((rc = d2h(h->rel_vort_on_triangles, M->W.rvt, (size_t)M->d.dual_v_vectors))) return rc;
-    if ((rc = d2h(h->pot_vort, M->W.pv, (size_t)M->d.curl))) return rc;
-    if ((rc = d2h(h->temperature, M->W.temp, (size_t)M->d.s))) return rc;
-    if ((rc = d2h(h->v_squared, M->W.vsq, (size_t)M->d.s))) return rc;
-    HIP_OK(hipStreamSynchronize(M->stream));
-    return 0;
+    return M->precision == GAME_MIXED ? get_diagnostics_t(M->WF, h) : get_diagnostics_t(M->W, h);
 }
 
 int game_gpu_get_irrev(game_irrev_view_t *h)
 {
     if (!M) return GAME_ERR_NOT_INIT;
-    int rc;
-    size_t S = (size_t)M->d.s;
-    if ((rc = d2h(h->condensates_sediment_heat, M->W.sed_heat, S))) return rc;
-    if ((rc = d2h(h->phase_trans_rates, M->W.ptr, (size_t)(M->d.n_cond + 1)*S))) return rc;
-    if ((rc = d2h(h->phase_trans_heating_rate, M->W.phr, S))) return rc;
-    HIP_OK(hipStreamSynchronize(M->stream));
-    return 0;
+    return M->precision == GAME_MIXED ? get_irrev_t(M->WF, h) : get_irrev_t(M->W, h);
 }
 
 /* manage_rkhevi.c:20-100 on the device-resident states */
 int game_gpu_manage_rkhevi(double dt, int first)
 {
     if (!M) return GAME_ERR_NOT_INIT;
-    hipStream_t st = M->stream;
-    const DevGrid &G = M->G;
-    DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
-    DevWork &W = M->W;
-    M->K.dt = dt;
-    const double *rho_gas_new = sn.rho + (i64)G.n_cond*G.s;
-    fp64::temperature(G, so, W.temp, st);
-    if (G.moist) fp64::phase_trans(G, so, W, dt, st);
-    for (int rk = 0; rk < 2; ++rk) {
-        if (rk == 0) fp64::pgrad(G, sn, W, first, st);
-        else fp64::pgc_v(G, sn, W, st);
-        if (rk == 1 || first) fp64::advection(G, sn.wind, rho_gas_new, W, st);
-        fp64::vector_tend(G, W, td.wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
-        fp64::scalar_tend(G, sn, td, W, rk, st);
-        fp64::ver_waves(G, so, sn, td, M->K, rk, st);
-        if (G.n_const > 1) fp64::gen_densities(G, so, sn, td, W, M->K, rk, st);
-    }
-    HIP_OK(hipGetLastError());
-    return 0;
+    if (M->precision == GAME_MIXED) return step_t(M->WF, M->WF.td_wind, dt, first);
+    return step_t(M->W, M->s[GAME_STATE_TENDENCY].wind, dt, first);
 }
 
 int game_gpu_linear_combine_two_states(int k0, int k1, int ko, double c0, double c1)
@@ -414,7 +486,7 @@ int game_gpu_linear_combine_two_states(int k0, int k1, int ko, double c0, double
     if (!M) return GAME_ERR_NOT_INIT;
     DevState *a = slot(k0), *b = slot(k1), *o = slot(ko);
     if (!a || !b || !o) { snprintf(g_err, sizeof g_err, "bad state slot"); return GAME_ERR_ARG; }
-    fp64::linear_combine(M->G, *a, *b, *o, c0, c1, M->stream);
+    col::linear_combine(M->G, *a, *b, *o, c0, c1, M->stream);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -437,7 +509,8 @@ int game_gpu_temperature_diagnostics(int k)
     if (!M) return GAME_ERR_NOT_INIT;
     DevState *s = slot(k);
     if (!s) return GAME_ERR_ARG;
-    fp64::temperature(M->G, *s, M->W.temp, M->stream);
+    if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, M->stream);
+    else col::temperature(M->G, *s, M->W.temp, M->stream);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -449,7 +522,7 @@ int game_gpu_inner_product(const double *a, const double *b, double *out)
     double *da = M->s[GAME_STATE_OLD].wind, *db = M->s[GAME_STATE_NEW].wind;
     HIP_OK(hipMemcpyAsync(da, a, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
     if (b != a) HIP_OK(hipMemcpyAsync(db, b, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
-    fp64::inner_product(M->G, da, b != a ? db : da, M->tmp_s, M->stream);
+    hor::inner_product(M->G, da, b != a ? db : da, M->tmp_s, M->stream);
     HIP_OK(hipMemcpyAsync(out, M->tmp_s, S*sizeof(double), hipMemcpyDeviceToHost, M->stream));
     HIP_OK(hipStreamSynchronize(M->stream));
     return 0;
@@ -457,7 +530,7 @@ int game_gpu_inner_product(const double *a, const double *b, double *out)
 
 static int integral(i64 n, const double *x, const double *y, double *out)
 {
-    int blocks = fp64::integral_partials(n, x, y, M->G.vol, M->partials, M->stream);
+    int blocks = col::integral_partials(n, x, y, M->G.vol, M->partials, M->stream);
     HIP_OK(hipMemcpyAsync(M->h_partials, M->partials, sizeof(double)*2*(size_t)blocks, hipMemcpyDeviceToHost,
                           M->stream));
     HIP_OK(hipStreamSynchronize(M->stream));
@@ -485,38 +558,51 @@ int game_gpu_integrals(int k, double *out)
         if ((rc = integral(G.s, s->rho + (i64)c*G.s, nullptr, &out[c]))) return rc;
     if ((rc = integral(G.s, s->rt, nullptr, &out[G.n_const]))) return rc;
     const double *rg = s->rho + (i64)G.n_cond*G.s;
-    fp64::inner_product(G, s->wind, s->wind, M->tmp_s, M->stream);
+    hor::inner_product(G, s->wind, s->wind, M->tmp_s, M->stream);
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 1]))) return rc;
     out[G.n_const + 1] *= 0.5;
     if ((rc = integral(G.s, rg, G.phi, &out[G.n_const + 2]))) return rc;
-    fp64::temperature(G, *s, M->tmp_s, M->stream);
+    col::temperature(G, *s, M->tmp_s, M->stream);
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 3]))) return rc;
     out[G.n_const + 3] *= GC_C_D_V;
     return 0;
 }
+
+}  /* extern "C" */
+
+template <class WK>
+static int balance_terms_t(const WK &W, const double *wind, const double *density, double *v_squared, double *pvt)
+{
+    DevState &t = M->s[GAME_STATE_TENDENCY];   /* used as scratch, cleared after */
+    size_t S = (size_t)M->d.s, V = (size_t)M->d.v;
+    typedef typename std::remove_pointer<decltype(W.vsq)>::type TW;
+    HIP_OK(hipMemcpyAsync(t.wind, wind, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    HIP_OK(hipMemcpyAsync(t.rt, density, S*sizeof(double), hipMemcpyHostToDevice, M->stream));
+    hor::advection(M->G, t.wind, t.rt, W, M->stream);
+    int rc;
+    if ((rc = d2h(v_squared, W.vsq, S))) return rc;
+    if ((rc = d2h(pvt, W.pvt, V))) return rc;
+    HIP_OK(hipMemsetAsync(t.wind, 0, V*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(t.rt, 0, S*sizeof(double), M->stream));
+    HIP_OK(hipMemsetAsync(W.pvt, 0, V*sizeof(TW), M->stream));
+    HIP_OK(hipMemsetAsync(W.vsq, 0, S*sizeof(TW), M->stream));
+    HIP_OK(hipMemsetAsync(W.vsq_grad, 0, V*sizeof(TW), M->stream));
+    HIP_OK(hipMemsetAsync(W.flux, 0, V*sizeof(TW), M->stream));
+    HIP_OK(hipMemsetAsync(W.pv, 0, (size_t)M->d.curl*sizeof(TW), M->stream));
+    HIP_OK(hipMemsetAsync(W.rvt, 0, (size_t)M->d.dual_v_vectors*sizeof(TW), M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+extern "C" {
 
 /* v_squared and pot_vort_tend of a (wind, density) pair, the terms of the
    initial hydrostatic balance (src/io/set_initial_state.c:155-166) */
 int game_gpu_init_balance_terms(const double *wind, const double *density, double *v_squared, double *pvt)
 {
     if (!M) return GAME_ERR_NOT_INIT;
-    DevState &t = M->s[GAME_STATE_TENDENCY];   /* used as scratch, cleared after */
-    size_t S = (size_t)M->d.s, V = (size_t)M->d.v;
-    HIP_OK(hipMemcpyAsync(t.wind, wind, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
-    HIP_OK(hipMemcpyAsync(t.rt, density, S*sizeof(double), hipMemcpyHostToDevice, M->stream));
-    fp64::advection(M->G, t.wind, t.rt, M->W, M->stream);
-    HIP_OK(hipMemcpyAsync(v_squared, M->W.vsq, S*sizeof(double), hipMemcpyDeviceToHost, M->stream));
-    HIP_OK(hipMemcpyAsync(pvt, M->W.pvt, V*sizeof(double), hipMemcpyDeviceToHost, M->stream));
-    HIP_OK(hipMemsetAsync(t.wind, 0, V*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(t.rt, 0, S*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.pvt, 0, V*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.vsq, 0, S*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.vsq_grad, 0, V*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.flux, 0, V*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.pv, 0, (size_t)M->d.curl*sizeof(double), M->stream));
-    HIP_OK(hipMemsetAsync(M->W.rvt, 0, (size_t)M->d.dual_v_vectors*sizeof(double), M->stream));
-    HIP_OK(hipStreamSynchronize(M->stream));
-    return 0;
+    if (M->precision == GAME_MIXED) return balance_terms_t(M->WF, wind, density, v_squared, pvt);
+    return balance_terms_t(M->W, wind, density, v_squared, pvt);
 }
 
 int game_gpu_enable_timing(int on)

@@ -190,7 +190,17 @@ typedef struct game_config {
 /* ------------------------------------------------------------------ */
 /* GPU path (libgame_rkhevi.so)                                         */
 /* ------------------------------------------------------------------ */
-/* precision modes of the device step */
+/* precision forms of the device step.
+   GAME_FP64: every array and every operation in FP64, the reference's
+     operation order (bit-identical to the reference's C on its inputs).
+   GAME_MIXED: the work arrays of the horizontal operators (pressure gradient,
+     mass flux, vorticities, potential vorticity, TRSK flux, kinetic energy
+     and its gradient), the momentum tendency, temperature, phase-transition
+     rates and heating and the sediment heat are stored in FP32; the
+     prognostic state, the density and rhotheta_v tendencies, both vertical
+     implicit solvers, the grid metrics and the integrals stay FP64, so every
+     constituent's mass is conserved as in the FP64 form.  Forcings and
+     diagnostics read back through the getters are widened to double. */
 enum game_precision { GAME_FP64 = 0, GAME_MIXED = 1 };
 
 /* error codes (0 = success) */
