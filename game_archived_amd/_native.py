@@ -214,4 +214,11 @@ def gpu_lib():
     lib.game_gpu_bytes_allocated.restype = C.c_longlong
     lib.game_gpu_init_balance_terms.argtypes = [DP, DP, DP, DP]
     lib.game_gpu_kernel_count.restype = C.c_int
+    VP = C.c_void_p   # device pointers of the operator-level interface
+    for name, n in (("grad", 2), ("divv_h", 2), ("curl_triangles", 2), ("rhombus_density", 2),
+                    ("vorticity_flux", 3), ("inner_product", 3)):
+        getattr(lib, f"game_gpu_op_{name}").argtypes = [VP] * n
+        getattr(lib, f"game_gpu_op_{name}_adjoint").argtypes = [VP] * n
+    lib.game_gpu_op_thomas.argtypes = [C.c_int, C.c_int] + [VP] * 6
+    lib.game_gpu_op_thomas_adjoint.argtypes = [C.c_int, C.c_int] + [VP] * 10
     return lib

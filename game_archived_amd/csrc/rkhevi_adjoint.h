@@ -1,0 +1,36 @@
+/*
+ * rkhevi_adjoint.h -- reverse index tables and launchers of the operator
+ * adjoints (rkhevi_adjoint.hip).
+ */
+#ifndef GAME_RKHEVI_ADJOINT_H
+#define GAME_RKHEVI_ADJOINT_H
+
+#include "rkhevi_dev.h"
+
+/* reverse tables, built once from the forward tables */
+struct AdjTables {
+    const int *slot_from, *slot_to;   /* NVH: slot of edge e in the adjacency of its from/to cell */
+    const int *tri_of_edge;           /* 2 NVH: the two triangles whose curl reads edge e */
+    const int *tri_slot;              /* 2 NVH: the slot of e in those triangles' edge lists */
+    const int *rh_off, *rh_ent;       /* CSR over cells: (edge, slot) pairs of the rhombus weights reading the cell */
+    const int *tr_off, *tr_ent;       /* CSR over edges: (edge, slot) pairs of the TRSK stencils reading the edge */
+};
+
+namespace adj {
+void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st);
+void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st);
+void divv_h_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *Fb, hipStream_t st);
+void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st);
+void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream_t st);
+void rhombus_density_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *rb, hipStream_t st);
+void vorticity_flux_adjoint(const DevGrid &G, const AdjTables &A, const double *pv, const double *ob, double *Fb,
+                            hipStream_t st);
+void inner_product_adjoint(const DevGrid &G, const AdjTables &A, const double *b, const double *ib, double *ab,
+                           hipStream_t st);
+void thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r, double *x,
+            double *work, hipStream_t st);
+void thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
+                    const double *xb, double *rb, double *cb, double *db, double *eb, double *work, hipStream_t st);
+}
+
+#endif

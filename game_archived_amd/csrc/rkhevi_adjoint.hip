@@ -1,0 +1,356 @@
+/*
+ * rkhevi_adjoint.hip -- hand-written adjoints of the linear operators of the
+ * RKHEVI step (FP64), for gfx950.
+ *
+ * Every adjoint is a gather: each thread owns one output element of the
+ * adjoint and sums the contributions of the forward outputs that read it,
+ * through reverse index tables built once on the host (no atomics, so the
+ * result does not depend on scheduling).  Each coefficient is recomputed
+ * exactly as the forward kernel forms it, terrain-following terms included,
+ * so <A x, y> = <x, A^T y> holds to rounding.
+ *
+ *   operator (forward, reference)                    adjoint gather
+ *   grad (gradient_operators.c:56-74)                 cell <- adjacent edges, own column
+ *   divv_h (divergences.c:14-64)                      edge <- from/to cells
+ *   curl on triangles (vorticities.c:167-222)         edge <- its two triangles, layers l-1..l+1
+ *   rhombus density (vorticities.c:74-80)             cell <- rhombi that weight it (CSR)
+ *   TRSK vorticity flux in F (vorticity_flux.c)       edge <- edges whose TRSK stencil holds it (CSR)
+ *                                                     w point <- adjacent edges and own levels
+ *   inner product in a (inner_product.c:13-43)        edge <- from/to cells, w <- own column
+ *   three-band solve (column_solvers.c:641-667)       transposed three-band solve
+ */
+#include "rkhevi_dev.h"
+#include "rkhevi_adjoint.h"
+
+#define ATPB 256
+#define AGRID(n) dim3((unsigned)(((n) + ATPB - 1)/ATPB))
+
+/* ------------------------------------------------------------------ */
+/* grad^T: fbar(c, l) from gbar at the cell's edges and w points        */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double grad_w_cot(const DevGrid &G, const AdjTables &A, const double *gb, int h, int lev,
+                                             int ne)
+{
+    /* cotangent of the vertical covariant gradient gw(h, lev): its own w
+       output plus the terrain corrections of the cell's edges at layers lev
+       (ipw slot 6) and lev - 1 (slot 7) (gradient_operators.c:14-33, averaging.c:14-39) */
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro;
+    double r = gb[h + (i64)lev*vpl];
+    if (!G.flat) {
+        if (lev >= L - no)
+            for (int j = 0; j < ne; ++j) {
+                i64 vi = nsh + (i64)lev*vpl + G.adj[6*h + j];
+                r += -G.slope[vi]*0.5*G.ipw[8*((i64)lev*nsh + h) + 6]*gb[vi];
+            }
+        if (lev - 1 >= L - no)
+            for (int j = 0; j < ne; ++j) {
+                i64 vi = nsh + (i64)(lev - 1)*vpl + G.adj[6*h + j];
+                r += -G.slope[vi]*0.5*G.ipw[8*((i64)(lev - 1)*nsh + h) + 7]*gb[vi];
+            }
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const double *gb, double *fb)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.s) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
+    const int l = (int)(t/nsh), h = (int)(t - (i64)l*nsh);
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    double r = 0.0;
+    for (int j = 0; j < ne; ++j) {
+        i64 vi = nsh + (i64)l*vpl + G.adj[6*h + j];
+        /* g_e = (f_to - f_from)/d_e: the from-cell (sign +1) gets -1/d_e */
+        r += -(double)G.sgn[6*h + j]*gb[vi]/G.nd[vi];
+    }
+    if (l + 1 <= L - 1) r += grad_w_cot(G, A, gb, h, l + 1, ne)/G.nd[h + (i64)(l + 1)*vpl];
+    if (l >= 1) r -= grad_w_cot(G, A, gb, h, l, ne)/G.nd[h + (i64)l*vpl];
+    fb[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* divv_h and divv_h^T                                                   */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ATPB) k_divv(DevGrid G, const double *in, double *out)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.s) return;
+    const int l = (int)(t/G.nsh), h = (int)(t - (i64)l*G.nsh);
+    out[t] = divv_h_cell(G, in, l, h);
+}
+
+/* cotangent of the vertical contravariant correction vcc(c, lev), lev in
+   [L - n_oro, L - 1]: it enters layer lev with +area_w(lev) and layer lev - 1
+   with -area_w(lev) (divergences.c:41-59) */
+__device__ __forceinline__ double vcc_cot(const DevGrid &G, const double *db, int c, int lev)
+{
+    i64 s0 = (i64)lev*G.nsh + c, s1 = s0 - G.nsh;
+    double a = G.area[c + (i64)lev*G.vpl];
+    return db[s0]/G.vol[s0]*a - db[s1]/G.vol[s1]*a;
+}
+
+__global__ void __launch_bounds__(ATPB) k_divv_adj(DevGrid G, AdjTables A, const double *db, double *Fb)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.v) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro;
+    const int l = (int)(t/vpl), h = (int)(t - (i64)l*vpl);
+    if (h < nsh) { Fb[t] = 0.0; return; }   /* divv_h reads horizontal components only */
+    const int e = h - nsh;
+    const int fr = G.from[e], to = G.to[e];
+    const i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
+    double r = G.area[t]*(db[sf]/G.vol[sf] - db[st]/G.vol[st]);
+    if (!G.flat && l >= L - no) {
+        const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
+        for (int p = 0; p < 2; ++p) {
+            const int c = cell[p];
+            double k = -0.5*G.ipw[8*((i64)l*nsh + c) + slot[p]]*G.slope[t];
+            double v = vcc_cot(G, db, c, l);
+            if (l + 1 <= L - 1) v += vcc_cot(G, db, c, l + 1);
+            r += k*v;
+        }
+    }
+    Fb[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* curl on triangles^T                                                  */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const double *zb, double *wb)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.v) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro;
+    const int lp = (int)(t/vpl), h = (int)(t - (i64)lp*vpl);
+    if (h < nsh) { wb[t] = 0.0; return; }
+    const int e = h - nsh;
+    double r = 0.0;
+    for (int p = 0; p < 2; ++p) {
+        const int tri = A.tri_of_edge[2*e + p], k = A.tri_slot[2*e + p];
+        const double sg = (double)G.vst[3*tri + k];
+        for (int l = lp - 1; l <= lp + 1; ++l) {
+            if (l < 0 || l > L - 1) continue;
+            const i64 vi = nsh + (i64)l*vpl + e;
+            double lrf = 1.0, self = 1.0, other = 0.0;
+            int other_layer = -1;
+            if (l >= L - no) {
+                const i64 di = G.nvh + (i64)l*G.dvpl + tri;
+                lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
+                const double dz = G.dzv[di] - G.zv[vi];
+                const bool up = dz > 0 || l == L - 1;
+                const i64 ivg = up ? vi - vpl : vi + vpl;
+                const double q = dz/(G.zv[vi] - G.zv[ivg]);
+                self = 1.0 + q;
+                other = -q;
+                other_layer = up ? l - 1 : l + 1;
+            }
+            const double base = zb[(i64)l*G.nt + tri]*lrf*G.nd[vi]*sg/G.darea[G.nvh + (i64)l*G.dvpl + tri];
+            if (l == lp) r += base*self;
+            if (other_layer == lp) r += base*other;
+        }
+    }
+    wb[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* rhombus density and its adjoint                                      */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ATPB) k_rhombus(DevGrid G, const double *rho, double *out)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= (i64)G.L*G.nvh) return;
+    const int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+    double dr = 0.0;
+    for (int j = 0; j < 4; ++j) dr += G.drw[4*e + j]*rho[(i64)l*G.nsh + G.dri[4*e + j]];
+    out[t] = dr;
+}
+
+__global__ void __launch_bounds__(ATPB) k_rhombus_adj(DevGrid G, AdjTables A, const double *db, double *rb)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.s) return;
+    const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+    double r = 0.0;
+    for (int k = A.rh_off[c]; k < A.rh_off[c + 1]; ++k) {
+        const int e = A.rh_ent[2*k], j = A.rh_ent[2*k + 1];
+        r += G.drw[4*e + j]*db[(i64)l*G.nvh + e];
+    }
+    rb[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* TRSK vorticity flux^T in the mass flux F (pv fixed)                   */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double vflux_w_weight(const DevGrid &G, int lev)
+{   /* vorticity_flux.c:373-417: half weights inside, full at top and surface */
+    return (lev == 0 || lev == G.L) ? 1.0 : 0.5;
+}
+
+__global__ void __launch_bounds__(ATPB) k_vflux_adj(DevGrid G, AdjTables A, const double *pv, const double *ob,
+                                                    double *Fb)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.v) return;
+    const int nsh = G.nsh, nvh = G.nvh, vpl = G.vpl, L = G.L;
+    const int l = (int)(t/vpl), h = (int)(t - (i64)l*vpl);
+    double r = 0.0;
+    if (h >= nsh && l < L) {
+        /* horizontal F of edge e' at layer l */
+        const int ep = h - nsh;
+        const i64 pb = nvh + (i64)l*2*nvh;
+        for (int k = A.tr_off[ep]; k < A.tr_off[ep + 1]; ++k) {
+            const int e = A.tr_ent[2*k], j = A.tr_ent[2*k + 1];
+            const bool half = (j == 2 && G.from[e] >= GAME_N_PENT) || (j == 7 && G.to[e] >= GAME_N_PENT);
+            const double q = half ? 0.5*(pv[pb + G.tmc[10*e + j]] + pv[pb + e]) : pv[pb + G.tmc[10*e + j]];
+            r += G.tw[10*e + j]*q*ob[nsh + (i64)l*vpl + e];
+        }
+        /* the vertical outputs of the edge's two cells at levels l and l + 1 */
+        const int cell[2] = {G.from[ep], G.to[ep]}, slot[2] = {A.slot_from[ep], A.slot_to[ep]};
+        for (int p = 0; p < 2; ++p) {
+            const int c = cell[p];
+            const double w = G.ipw[8*((i64)l*nsh + c) + slot[p]];
+            r += vflux_w_weight(G, l + 1)*w*pv[(i64)(l + 1)*2*nvh + ep]*ob[(i64)(l + 1)*vpl + c];
+            r += vflux_w_weight(G, l)*w*pv[(i64)l*2*nvh + ep]*ob[(i64)l*vpl + c];
+        }
+    } else if (h < nsh) {
+        /* vertical F of cell h at level l: the w terms of the cell's edges at
+           layers l (ipw slot 6) and l - 1 (slot 7) (vorticity_flux.c:110-187) */
+        const int ne = h < GAME_N_PENT ? 5 : 6;
+        for (int j = 0; j < ne; ++j) {
+            const int e = G.adj[6*h + j];
+            const double q = pv[e + (i64)l*2*nvh];
+            if (l <= L - 1)
+                r += -0.5*G.ipw[8*((i64)l*nsh + h) + 6]*q*ob[nsh + (i64)l*vpl + e];
+            if (l >= 1)
+                r += -0.5*G.ipw[8*((i64)(l - 1)*nsh + h) + 7]*q*ob[nsh + (i64)(l - 1)*vpl + e];
+        }
+    }
+    Fb[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* inner product^T in its first argument                                 */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ATPB) k_ip_adj(DevGrid G, AdjTables A, const double *b, const double *ib,
+                                                 double *ab)
+{
+    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    if (t >= G.v) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
+    const int l = (int)(t/vpl), h = (int)(t - (i64)l*vpl);
+    double r = 0.0;
+    if (h >= nsh) {
+        const int e = h - nsh;
+        const int fr = G.from[e], to = G.to[e];
+        const i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
+        r = b[t]*(G.ipw[8*sf + A.slot_from[e]]*ib[sf] + G.ipw[8*st + A.slot_to[e]]*ib[st]);
+    } else {
+        double s = 0.0;
+        if (l >= 1) s += G.ipw[8*((i64)(l - 1)*nsh + h) + 7]*ib[(i64)(l - 1)*nsh + h];
+        if (l <= L - 1) s += G.ipw[8*((i64)l*nsh + h) + 6]*ib[(i64)l*nsh + h];
+        r = b[t]*s;
+    }
+    ab[t] = r;
+}
+
+/* ------------------------------------------------------------------ */
+/* batched three-band solve and its adjoint (one column per lane;        */
+/* element j of column k at j*ncol + k, so lanes read consecutive words) */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ATPB) k_thomas(int n, int ncol, const double *c, const double *d, const double *e,
+                                                 const double *r, double *x, double *ep)
+{
+    const int k = blockIdx.x*ATPB + threadIdx.x;
+    if (k >= ncol) return;
+    /* column_solvers.c:641-667 */
+    double epj = (n > 1) ? e[k]/d[k] : 0.0, xj = r[k]/d[k];
+    ep[k] = epj;
+    x[k] = xj;
+    for (int j = 1; j < n; ++j) {
+        const i64 o = (i64)j*ncol + k, om = o - ncol;
+        const double den = d[o] - epj*c[om];
+        if (j < n - 1) epj = e[o]/den;
+        xj = (r[o] - xj*c[om])/den;
+        ep[o] = epj;
+        x[o] = xj;
+    }
+    for (int j = n - 2; j >= 0; --j) {
+        const i64 o = (i64)j*ncol + k;
+        xj = x[o] - ep[o]*xj;
+        x[o] = xj;
+    }
+}
+
+/* A^T rbar = xbar (A^T has sub-diagonal e and super-diagonal c), then the
+   coefficient cotangents -rbar x^T on the three bands */
+__global__ void __launch_bounds__(ATPB) k_thomas_adj(int n, int ncol, const double *c, const double *d,
+                                                     const double *e, const double *x, const double *xb,
+                                                     double *rb, double *cb, double *db, double *eb, double *ep)
+{
+    const int k = blockIdx.x*ATPB + threadIdx.x;
+    if (k >= ncol) return;
+    double epj = (n > 1) ? c[k]/d[k] : 0.0, yj = xb[k]/d[k];
+    ep[k] = epj;
+    rb[k] = yj;
+    for (int j = 1; j < n; ++j) {
+        const i64 o = (i64)j*ncol + k, om = o - ncol;
+        const double den = d[o] - epj*e[om];
+        if (j < n - 1) epj = c[o]/den;
+        yj = (xb[o] - yj*e[om])/den;
+        ep[o] = epj;
+        rb[o] = yj;
+    }
+    for (int j = n - 2; j >= 0; --j) {
+        const i64 o = (i64)j*ncol + k;
+        yj = rb[o] - ep[o]*yj;
+        rb[o] = yj;
+    }
+    for (int j = 0; j < n; ++j) {
+        const i64 o = (i64)j*ncol + k;
+        db[o] = -rb[o]*x[o];
+        if (j < n - 1) {
+            cb[o] = -rb[o + ncol]*x[o];
+            eb[o] = -rb[o]*x[o + ncol];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+namespace adj {
+
+void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st)
+{ LAUNCH("k_grad_adj", k_grad_adj, AGRID(G.s), dim3(ATPB), st, G, A, gb, fb); }
+
+void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st)
+{ LAUNCH("k_divv", k_divv, AGRID(G.s), dim3(ATPB), st, G, in, out); }
+
+void divv_h_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *Fb, hipStream_t st)
+{ LAUNCH("k_divv_adj", k_divv_adj, AGRID(G.v), dim3(ATPB), st, G, A, db, Fb); }
+
+void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st)
+{ LAUNCH("k_curl_adj", k_curl_adj, AGRID(G.v), dim3(ATPB), st, G, A, zb, wb); }
+
+void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream_t st)
+{ LAUNCH("k_rhombus", k_rhombus, AGRID((i64)G.L*G.nvh), dim3(ATPB), st, G, rho, out); }
+
+void rhombus_density_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *rb, hipStream_t st)
+{ LAUNCH("k_rhombus_adj", k_rhombus_adj, AGRID(G.s), dim3(ATPB), st, G, A, db, rb); }
+
+void vorticity_flux_adjoint(const DevGrid &G, const AdjTables &A, const double *pv, const double *ob, double *Fb,
+                            hipStream_t st)
+{ LAUNCH("k_vflux_adj", k_vflux_adj, AGRID(G.v), dim3(ATPB), st, G, A, pv, ob, Fb); }
+
+void inner_product_adjoint(const DevGrid &G, const AdjTables &A, const double *b, const double *ib, double *ab,
+                           hipStream_t st)
+{ LAUNCH("k_ip_adj", k_ip_adj, AGRID(G.v), dim3(ATPB), st, G, A, b, ib, ab); }
+
+void thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r, double *x,
+            double *work, hipStream_t st)
+{ LAUNCH("k_thomas", k_thomas, AGRID(ncol), dim3(ATPB), st, n, ncol, c, d, e, r, x, work); }
+
+void thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
+                    const double *xb, double *rb, double *cb, double *db, double *eb, double *work, hipStream_t st)
+{ LAUNCH("k_thomas_adj", k_thomas_adj, AGRID(ncol), dim3(ATPB), st, n, ncol, c, d, e, x, xb, rb, cb, db, eb, work); }
+
+}  // namespace adj

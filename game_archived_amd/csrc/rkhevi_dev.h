@@ -117,6 +117,57 @@ __device__ __forceinline__ double c_v_mass_weighted(const DevGrid &G, const doub
     return r;
 }
 
+/* averaging.c:41-95 */
+__device__ __forceinline__ double vertical_contravariant_corr(const DevGrid &G, const double *f, int l, int h)
+{
+    double r = 0;
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    if (l >= G.L - G.n_oro) {
+        if (l == G.L - G.n_oro) {
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+        } else {
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)(l - 1)*G.nsh + h, vi = G.nsh + (i64)(l - 1)*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+            for (int i = 0; i < ne; ++i) {
+                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
+                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
+            }
+        }
+    }
+    return r;
+}
+
+/* divergences.c:14-64 for cell h of layer l */
+__device__ __forceinline__ double divv_h_cell(const DevGrid &G, const double *in, int l, int h)
+{
+    int ne = h < GAME_N_PENT ? 5 : 6;
+    i64 i = (i64)l*G.nsh + h;
+    double comp_h = 0.0;
+    for (int j = 0; j < ne; ++j) {
+        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
+        comp_h += in[vi]*G.sgn[6*h + j]*G.area[vi];
+    }
+    double comp_v = 0.0;
+    int L = G.L, no = G.n_oro;
+    if (l == L - no - 1) {
+        double cl = vertical_contravariant_corr(G, in, l + 1, h);
+        comp_v = -cl*G.area[h + (i64)(l + 1)*G.vpl];
+    } else if (l == L - 1) {
+        double cu = vertical_contravariant_corr(G, in, l, h);
+        comp_v = cu*G.area[h + (i64)l*G.vpl];
+    } else if (l > L - no - 1) {
+        double cu = vertical_contravariant_corr(G, in, l, h);
+        double cl = vertical_contravariant_corr(G, in, l + 1, h);
+        comp_v = cu*G.area[h + (i64)l*G.vpl] - cl*G.area[h + (i64)(l + 1)*G.vpl];
+    }
+    return 1.0/G.vol[i]*(comp_h + comp_v);
+}
+
 /* per-kernel timing with HIP events on the launch stream (off unless enabled) */
 struct KernelTimer {
     virtual void begin(const char *name, hipStream_t st) = 0;

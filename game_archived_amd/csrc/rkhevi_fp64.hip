@@ -35,27 +35,6 @@ __device__ __forceinline__ double grad_w_raw(const DevGrid &G, const TF *f, int 
     return ((double)f[h + (i64)(lev - 1)*G.nsh] - (double)f[h + (i64)lev*G.nsh])/G.nd[vi];
 }
 
-/* contravariant horizontal gradient at edge e of layer l (gradient_operators.c:14-33,
-   averaging.c:14-39, 113-133), vertical gradients recomputed on the fly */
-__device__ __forceinline__ double grad_edge(const DevGrid &G, const double *f, int l, int e)
-{
-    int fr = G.from[e], to = G.to[e];
-    i64 vi = G.nsh + (i64)l*G.vpl + e;
-    double g = (f[to + (i64)l*G.nsh] - f[fr + (i64)l*G.nsh])/G.nd[vi];
-    if (l >= G.L - G.n_oro) {
-        const double *ipw = G.ipw;
-        double c = ipw[8*((i64)l*G.nsh + fr) + 6]*grad_w_raw(G, f, l, fr);
-        c += ipw[8*((i64)l*G.nsh + to) + 6]*grad_w_raw(G, f, l, to);
-        if (l < G.L - 1) {
-            c += ipw[8*((i64)l*G.nsh + fr) + 7]*grad_w_raw(G, f, l + 1, fr);
-            c += ipw[8*((i64)l*G.nsh + to) + 7]*grad_w_raw(G, f, l + 1, to);
-        }
-        c = 0.5*c;
-        g += -G.slope[vi]*c;
-    }
-    return g;
-}
-
 /* averaging.c:14-39 on a stored vector field */
 __device__ __forceinline__ double remap_verpri2horpri(const DevGrid &G, const double *f, int l, int e)
 {
@@ -68,100 +47,6 @@ __device__ __forceinline__ double remap_verpri2horpri(const DevGrid &G, const do
         c += ipw[8*((i64)l*G.nsh + to) + 7]*f[(i64)(l + 1)*G.vpl + to];
     }
     return 0.5*c;
-}
-
-/* averaging.c:97-111 */
-__device__ __forceinline__ double horizontal_covariant(const DevGrid &G, const double *w, int l, int e)
-{
-    i64 vi = G.nsh + (i64)l*G.vpl + e;
-    double r = w[vi];
-    if (l >= G.L - G.n_oro) {
-        double vc = remap_verpri2horpri(G, w, l, e);
-        r += G.slope[vi]*vc;
-    }
-    return r;
-}
-
-/* averaging.c:41-95 */
-__device__ __forceinline__ double vertical_contravariant_corr(const DevGrid &G, const double *f, int l, int h)
-{
-    double r = 0;
-    int ne = h < GAME_N_PENT ? 5 : 6;
-    if (l >= G.L - G.n_oro) {
-        if (l == G.L - G.n_oro) {
-            for (int i = 0; i < ne; ++i) {
-                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
-                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
-            }
-        } else {
-            for (int i = 0; i < ne; ++i) {
-                i64 si = (i64)(l - 1)*G.nsh + h, vi = G.nsh + (i64)(l - 1)*G.vpl + G.adj[6*h + i];
-                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
-            }
-            for (int i = 0; i < ne; ++i) {
-                i64 si = (i64)l*G.nsh + h, vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + i];
-                r += -0.5*G.ipw[8*si + i]*G.slope[vi]*f[vi];
-            }
-        }
-    }
-    return r;
-}
-
-/* divergences.c:14-64 for cell h of layer l */
-__device__ __forceinline__ double divv_h_cell(const DevGrid &G, const double *in, int l, int h)
-{
-    int ne = h < GAME_N_PENT ? 5 : 6;
-    i64 i = (i64)l*G.nsh + h;
-    double comp_h = 0.0;
-    for (int j = 0; j < ne; ++j) {
-        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
-        comp_h += in[vi]*G.sgn[6*h + j]*G.area[vi];
-    }
-    double comp_v = 0.0;
-    int L = G.L, no = G.n_oro;
-    if (l == L - no - 1) {
-        double cl = vertical_contravariant_corr(G, in, l + 1, h);
-        comp_v = -cl*G.area[h + (i64)(l + 1)*G.vpl];
-    } else if (l == L - 1) {
-        double cu = vertical_contravariant_corr(G, in, l, h);
-        comp_v = cu*G.area[h + (i64)l*G.vpl];
-    } else if (l > L - no - 1) {
-        double cu = vertical_contravariant_corr(G, in, l, h);
-        double cl = vertical_contravariant_corr(G, in, l + 1, h);
-        comp_v = cu*G.area[h + (i64)l*G.vpl] - cl*G.area[h + (i64)(l + 1)*G.vpl];
-    }
-    return 1.0/G.vol[i]*(comp_h + comp_v);
-}
-
-/* divergences.c:66-148 for cell h of layer l */
-__device__ __forceinline__ double divv_h_tracer_cell(const DevGrid &G, const double *in, const double *dens,
-                                                     const double *wind, int l, int h)
-{
-    int ne = h < GAME_N_PENT ? 5 : 6;
-    i64 i = (i64)l*G.nsh + h;
-    double comp_h = 0.0;
-    for (int j = 0; j < ne; ++j) {
-        i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + j];
-        comp_h += in[vi]*G.sgn[6*h + j]*G.area[vi];
-    }
-    double comp_v = 0.0;
-    int L = G.L, no = G.n_oro;
-    if (l == L - no - 1) {
-        double cl = vertical_contravariant_corr(G, wind, l + 1, h);
-        double dl = (cl <= 0.0) ? dens[i] : dens[i + G.nsh];
-        comp_v = -dl*cl*G.area[h + (i64)(l + 1)*G.vpl];
-    } else if (l == L - 1) {
-        double cu = vertical_contravariant_corr(G, wind, l, h);
-        double du = (cu <= 0.0) ? dens[i - G.nsh] : dens[i];
-        comp_v = du*cu*G.area[h + (i64)l*G.vpl];
-    } else if (l > L - no - 1) {
-        double cu = vertical_contravariant_corr(G, wind, l, h);
-        double du = (cu <= 0.0) ? dens[i - G.nsh] : dens[i];
-        double cl = vertical_contravariant_corr(G, wind, l + 1, h);
-        double dl = (cl <= 0.0) ? dens[i] : dens[i + G.nsh];
-        comp_v = du*cu*G.area[h + (i64)l*G.vpl] - dl*cl*G.area[h + (i64)(l + 1)*G.vpl];
-    }
-    return 1.0/G.vol[i]*(comp_h + comp_v);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1417,6 +1302,23 @@ void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const 
 
 void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st)
 { LAUNCH("k_inner_product", k_inner_product<double>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, a, b, out); }
+
+/* single operators (FP64), for the operator-level interface and its adjoints */
+void grad(const DevGrid &G, const double *f, double *out, hipStream_t st)
+{
+    typedef double TW;
+    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, f, out);
+    LAUNCH("k_grad_w", k_grad_w<double>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, f, out);
+}
+
+void curl_triangles(const DevGrid &G, const double *wind, double *out, hipStream_t st)
+{ LAUNCH("k_rel_vort_tri", k_rel_vort_tri<double>, GRID1(G.nt), dim3(TPB), st, G, wind, out); }
+
+void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double *out, hipStream_t st)
+{
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<double>, GRID1(G.nvh), dim3(TPB), st, G, F, pv, out);
+    LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w<double>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, F, pv, out);
+}
 
 }  // namespace hor
 

@@ -23,6 +23,9 @@ void vector_tend(const DevGrid &G, const DevWorkF &W, float *tend_w, const doubl
 void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWork &W, int rk, hipStream_t st);
 void scalar_tend(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkF &W, int rk, hipStream_t st);
 void inner_product(const DevGrid &G, const double *a, const double *b, double *out, hipStream_t st);
+void grad(const DevGrid &G, const double *f, double *out, hipStream_t st);
+void curl_triangles(const DevGrid &G, const double *wind, double *out, hipStream_t st);
+void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double *out, hipStream_t st);
 }
 
 /* column solvers, point physics and state arithmetic shared by both forms:

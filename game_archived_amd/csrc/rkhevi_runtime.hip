@@ -20,6 +20,7 @@
 #include "../../include/game_rkhevi.h"
 #include "rkhevi_dev.h"
 #include "rkhevi_kernels.h"
+#include "rkhevi_adjoint.h"
 
 KernelTimer *g_kernel_timer = nullptr;
 
@@ -94,6 +95,8 @@ struct Model {
     DevState s[3];
     DevWork W;      /* FP64 form */
     DevWorkF WF;    /* mixed form */
+    AdjTables A;    /* reverse tables of the operator adjoints, built on first use */
+    int adj_ready = 0;
     StepConsts K;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
@@ -644,6 +647,221 @@ int game_gpu_kernel_stats(int k, const char **name, long long *launches, double 
     *launches = M->timer.stats[k].launches;
     *ms = M->timer.stats[k].ms;
     return 0;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* operator-level interface and adjoints (device pointers, FP64)        */
+/* ------------------------------------------------------------------ */
+}  /* extern "C" */
+
+static int upload_ints(const int **dst, const std::vector<int> &v)
+{
+    return upload<int>(dst, v.data(), v.size());
+}
+
+/* reverse index tables of the adjoint gathers, from the forward tables */
+static int build_adjoint_tables(void)
+{
+    if (M->adj_ready) return 0;
+    const DevGrid &G = M->G;
+    const size_t NSH = (size_t)G.nsh, NVH = (size_t)G.nvh, NT = (size_t)G.nt;
+    std::vector<int> adj(6*NSH), from(NVH), to(NVH), vit(3*NT), dri(4*NVH), ti(10*NVH);
+    struct { int *h; const int *d; size_t n; } cp[] = {
+        {adj.data(), G.adj, 6*NSH}, {from.data(), G.from, NVH}, {to.data(), G.to, NVH},
+        {vit.data(), G.vit, 3*NT}, {dri.data(), G.dri, 4*NVH}, {ti.data(), G.ti, 10*NVH}};
+    for (auto &c : cp) HIP_OK(hipMemcpyAsync(c.h, c.d, c.n*sizeof(int), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    std::vector<int> sf(NVH, -1), stt(NVH, -1);
+    for (size_t c = 0; c < NSH; ++c) {
+        int ne = c < GAME_N_PENT ? 5 : 6;
+        for (int j = 0; j < ne; ++j) {
+            int e = adj[6*c + j];
+            if (from[e] == (int)c) sf[e] = j;
+            else if (to[e] == (int)c) stt[e] = j;
+        }
+    }
+    std::vector<int> toe(2*NVH, -1), tsl(2*NVH, -1), cnt(NVH, 0);
+    for (size_t t = 0; t < NT; ++t)
+        for (int k = 0; k < 3; ++k) {
+            int e = vit[3*t + k];
+            if (e < 0 || (size_t)e >= NVH || cnt[e] >= 2) {
+                snprintf(g_err, sizeof g_err, "triangle %zu lists edge %d a third time", t, e);
+                return GAME_ERR_ARG;
+            }
+            toe[2*e + cnt[e]] = (int)t;
+            tsl[2*e + cnt[e]] = k;
+            ++cnt[e];
+        }
+    for (size_t e = 0; e < NVH; ++e)
+        if (sf[e] < 0 || stt[e] < 0 || cnt[e] != 2) {
+            snprintf(g_err, sizeof g_err, "edge %zu: inconsistent adjacency or triangle tables", e);
+            return GAME_ERR_ARG;
+        }
+    /* CSR lists in ascending (edge, slot) order */
+    std::vector<int> roff(NSH + 1, 0), rent;
+    for (size_t e = 0; e < NVH; ++e) for (int j = 0; j < 4; ++j) ++roff[dri[4*e + j] + 1];
+    for (size_t c = 0; c < NSH; ++c) roff[c + 1] += roff[c];
+    rent.resize(2*(size_t)roff[NSH]);
+    {
+        std::vector<int> fill(roff.begin(), roff.end() - 1);
+        for (size_t e = 0; e < NVH; ++e)
+            for (int j = 0; j < 4; ++j) {
+                int c = dri[4*e + j], k = fill[c]++;
+                rent[2*(size_t)k] = (int)e;
+                rent[2*(size_t)k + 1] = j;
+            }
+    }
+    /* TRSK slots the forward reads: 0..3 / 5..8 beside a pentagon, else 0..4 / 5..9 */
+    auto used = [&](size_t e, int j) {
+        if (j < 5) return from[e] < GAME_N_PENT ? j < 4 : true;
+        return to[e] < GAME_N_PENT ? j < 9 : true;
+    };
+    std::vector<int> toff(NVH + 1, 0), tent;
+    for (size_t e = 0; e < NVH; ++e) for (int j = 0; j < 10; ++j) if (used(e, j)) ++toff[ti[10*e + j] + 1];
+    for (size_t e = 0; e < NVH; ++e) toff[e + 1] += toff[e];
+    tent.resize(2*(size_t)toff[NVH]);
+    {
+        std::vector<int> fill(toff.begin(), toff.end() - 1);
+        for (size_t e = 0; e < NVH; ++e)
+            for (int j = 0; j < 10; ++j) {
+                if (!used(e, j)) continue;
+                int k = fill[ti[10*e + j]]++;
+                tent[2*(size_t)k] = (int)e;
+                tent[2*(size_t)k + 1] = j;
+            }
+    }
+    AdjTables &A = M->A;
+    int rc;
+    if ((rc = upload_ints(&A.slot_from, sf)) || (rc = upload_ints(&A.slot_to, stt))
+        || (rc = upload_ints(&A.tri_of_edge, toe)) || (rc = upload_ints(&A.tri_slot, tsl))
+        || (rc = upload_ints(&A.rh_off, roff)) || (rc = upload_ints(&A.rh_ent, rent))
+        || (rc = upload_ints(&A.tr_off, toff)) || (rc = upload_ints(&A.tr_ent, tent)))
+        return rc;
+    HIP_OK(hipStreamSynchronize(M->stream));
+    M->adj_ready = 1;
+    return 0;
+}
+
+static int op_done(void)
+{
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(M->stream));
+    return 0;
+}
+
+#define OP_PROLOGUE(NEED_TABLES)                                               \
+    if (!M) return GAME_ERR_NOT_INIT;                                          \
+    if (M->precision != GAME_FP64) {                                           \
+        snprintf(g_err, sizeof g_err, "operator interface needs the FP64 form"); \
+        return GAME_ERR_ARG;                                                   \
+    }                                                                          \
+    if (NEED_TABLES) { int rc_ = build_adjoint_tables(); if (rc_) return rc_; }
+
+extern "C" {
+
+int game_gpu_op_grad(const double *f, double *out)
+{
+    OP_PROLOGUE(0)
+    HIP_OK(hipMemsetAsync(out, 0, (size_t)M->d.v*sizeof(double), M->stream));
+    hor::grad(M->G, f, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_grad_adjoint(const double *gbar, double *fbar)
+{
+    OP_PROLOGUE(1)
+    adj::grad_adjoint(M->G, M->A, gbar, fbar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_divv_h(const double *in, double *out)
+{
+    OP_PROLOGUE(0)
+    adj::divv_h(M->G, in, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_divv_h_adjoint(const double *dbar, double *inbar)
+{
+    OP_PROLOGUE(1)
+    adj::divv_h_adjoint(M->G, M->A, dbar, inbar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_curl_triangles(const double *wind, double *out)
+{
+    OP_PROLOGUE(0)
+    hor::curl_triangles(M->G, wind, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_curl_triangles_adjoint(const double *zbar, double *windbar)
+{
+    OP_PROLOGUE(1)
+    adj::curl_triangles_adjoint(M->G, M->A, zbar, windbar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_rhombus_density(const double *rho, double *out)
+{
+    OP_PROLOGUE(0)
+    adj::rhombus_density(M->G, rho, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_rhombus_density_adjoint(const double *dbar, double *rhobar)
+{
+    OP_PROLOGUE(1)
+    adj::rhombus_density_adjoint(M->G, M->A, dbar, rhobar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_vorticity_flux(const double *flux, const double *pot_vort, double *out)
+{
+    OP_PROLOGUE(0)
+    hor::vorticity_flux(M->G, flux, pot_vort, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_vorticity_flux_adjoint(const double *pot_vort, const double *outbar, double *fluxbar)
+{
+    OP_PROLOGUE(1)
+    adj::vorticity_flux_adjoint(M->G, M->A, pot_vort, outbar, fluxbar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_inner_product(const double *a, const double *b, double *out)
+{
+    OP_PROLOGUE(0)
+    hor::inner_product(M->G, a, b, out, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_inner_product_adjoint(const double *b, const double *outbar, double *abar)
+{
+    OP_PROLOGUE(1)
+    adj::inner_product_adjoint(M->G, M->A, b, outbar, abar, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r,
+                       double *x, double *work)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    adj::thomas(n, ncol, c, d, e, r, x, work, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
+                               const double *xbar, double *rbar, double *cbar, double *dbar, double *ebar,
+                               double *work)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    adj::thomas_adjoint(n, ncol, c, d, e, x, xbar, rbar, cbar, dbar, ebar, work, M->stream);
+    return op_done();
 }
 
 }  /* extern "C" */

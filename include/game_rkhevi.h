@@ -279,6 +279,45 @@ int game_gpu_synchronize(void);
 /* bytes of device memory held by the library */
 long long game_gpu_bytes_allocated(void);
 
+/* ------------------------------------------------------------------ */
+/* Single operators and their adjoints (FP64 form only).                */
+/* All array arguments are DEVICE pointers in the layouts above (S, V,   */
+/* curl, L*NT triangle or L*NVH edge fields); each call runs on the      */
+/* library's stream and returns after it has finished.  The adjoints     */
+/* are exact transposes of the forward operators, terrain terms          */
+/* included, computed as gathers over reverse tables (no atomics).       */
+/* The reference has no adjoints; the forward operators replace:         */
+/* ------------------------------------------------------------------ */
+/* grad, gradient_operators.c:56-74: S -> V (top and surface w are 0) */
+int game_gpu_op_grad(const double *f, double *out);
+int game_gpu_op_grad_adjoint(const double *out_bar, double *f_bar);
+/* divv_h, divergences.c:14-64: V (horizontal part read) -> S */
+int game_gpu_op_divv_h(const double *in, double *out);
+int game_gpu_op_divv_h_adjoint(const double *out_bar, double *in_bar);
+/* calc_rel_vort_on_triangles, vorticities.c:167-222: V -> L*NT */
+int game_gpu_op_curl_triangles(const double *wind, double *out);
+int game_gpu_op_curl_triangles_adjoint(const double *out_bar, double *wind_bar);
+/* rhombus density interpolation of calc_pot_vort, vorticities.c:74-80: S -> L*NVH */
+int game_gpu_op_rhombus_density(const double *rho, double *out);
+int game_gpu_op_rhombus_density_adjoint(const double *out_bar, double *rho_bar);
+/* vorticity_flux, vorticity_flux.c:16-187 and :373-417, linear in the mass
+   flux for a fixed potential vorticity (curl layout): V -> V */
+int game_gpu_op_vorticity_flux(const double *flux, const double *pot_vort, double *out);
+int game_gpu_op_vorticity_flux_adjoint(const double *pot_vort, const double *out_bar, double *flux_bar);
+/* inner_product, inner_product.c:13-43, linear in a for a fixed b: V x V -> S */
+int game_gpu_op_inner_product(const double *a, const double *b, double *out);
+int game_gpu_op_inner_product_adjoint(const double *b, const double *out_bar, double *a_bar);
+/* three-band solve, column_solvers.c:641-667, of ncol independent systems of
+   n rows: row j reads c[j-1] x[j-1] + d[j] x[j] + e[j] x[j+1] = r[j];
+   element j of system k at j*ncol + k (c, e: n - 1 rows used);
+   work: n*ncol doubles */
+int game_gpu_op_thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r,
+                       double *x, double *work);
+/* its adjoint: r_bar = A^-T x_bar, c_bar/d_bar/e_bar = -(r_bar x^T) on the bands */
+int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
+                               const double *x_bar, double *r_bar, double *c_bar, double *d_bar, double *e_bar,
+                               double *work);
+
 #ifdef __cplusplus
 }
 #endif
