@@ -627,8 +627,8 @@ __device__ __forceinline__ double rho_int_old(const DevGrid &G, const DevState &
 }
 
 template <int LT>
-__global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevState sn, DevState td, StepConsts K,
-                                                   int rk)
+__global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevState snr, DevState sn, DevState td,
+                                                   StepConsts K, int rk)
 {
     int i = blockIdx.x*128 + threadIdx.x;
     if (i >= G.nsh) return;
@@ -641,9 +641,9 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
     double ep[(LT > 0 ? LT : GAME_MAX_LAYERS) - 1], x[(LT > 0 ? LT : GAME_MAX_LAYERS) - 1];
 
     /* downward sweep: build row j and eliminate it (column_solvers.c:141-189, 641-660) */
-    ColumnLayer ca = column_layer(G, so, sn, td, dt, rk, i, 0);   /* layer j */
-    ColumnLayer cb = column_layer(G, so, sn, td, dt, rk, i, 1);   /* layer j+1 */
-    double tin = theta_int_new(G, sn, i, 0), rio = rho_int_old(G, so, i, 0);
+    ColumnLayer ca = column_layer(G, so, snr, td, dt, rk, i, 0);   /* layer j */
+    ColumnLayer cb = column_layer(G, so, snr, td, dt, rk, i, 1);   /* layer j+1 */
+    double tin = theta_int_new(G, snr, i, 0), rio = rho_int_old(G, so, i, 0);
     double c_prev = 0.0;
 #pragma unroll
     for (int j = 0; j < L - 1; ++j) {
@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
         ColumnLayer cc = cb;
         if (j < L - 2) {
             i64 l2 = lo + nsh;
-            tin_n = theta_int_new(G, sn, i, j + 1);
+            tin_n = theta_int_new(G, snr, i, j + 1);
             rio_n = rho_int_old(G, so, i, j + 1);
             /* upper diagonal e[j] and lower diagonal c[j] (column_solvers.c:173-189) */
             c = tin_n*cb.ga*tin
@@ -672,7 +672,7 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
             e = tin*cb.ga*tin_n
                 - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
                 + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
-            cc = column_layer(G, so, sn, td, dt, rk, i, j + 2);
+            cc = column_layer(G, so, snr, td, dt, rk, i, j + 2);
         }
         if (j == 0) {
             if (L - 1 > 1) ep[0] = e/d;
@@ -697,8 +697,8 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
 #pragma unroll
     for (int j = 0; j < L; ++j) {
         i64 b = i + (i64)j*nsh;
-        ColumnLayer cl = column_layer(G, so, sn, td, dt, rk, i, j);
-        double tin_j = (j < L - 1) ? theta_int_new(G, sn, i, j) : 0.0;
+        ColumnLayer cl = column_layer(G, so, snr, td, dt, rk, i, j);
+        double tin_j = (j < L - 1) ? theta_int_new(G, snr, i, j) : 0.0;
         double rio_j = (j < L - 1) ? rho_int_old(G, so, i, j) : 0.0;
         double v = vol[b];
         double rho_n, rt_n;
@@ -739,8 +739,8 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
 #define VW_THREADS 256
 
 template <typename TWIND>
-__global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevState so, DevState sn, DevState td,
-                                                              const TWIND *td_wind, StepConsts K, int rk)
+__global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevState so, DevState snr, DevState sn,
+                                                              DevState td, const TWIND *td_wind, StepConsts K, int rk)
 {
     extern __shared__ double lds[];
     const int L = G.L, nsh = G.nsh, vpl = G.vpl;
@@ -758,9 +758,9 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
         const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
         if (i >= nsh) continue;
         const i64 b = i + (i64)j*nsh, lo = b + nsh, wi = i + (i64)(j + 1)*vpl;
-        const ColumnLayer ca = column_layer(G, so, sn, td, dt, rk, i, j);
-        const ColumnLayer cb = column_layer(G, so, sn, td, dt, rk, i, j + 1);
-        const double tin = theta_int_new(G, sn, i, j), rio = rho_int_old(G, so, i, j);
+        const ColumnLayer ca = column_layer(G, so, snr, td, dt, rk, i, j);
+        const ColumnLayer cb = column_layer(G, so, snr, td, dt, rk, i, j + 1);
+        const double tin = theta_int_new(G, snr, i, j), rio = rho_int_old(G, so, i, j);
         const double w_old = so.wind[wi];
         const double rie = 0.5*(ca.re + cb.re);
         const double d = -(tin*tin)*(ca.ga + cb.ga)
@@ -774,7 +774,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
         double c = 0.0, e = 0.0;
         if (j < L - 2) {
             const i64 l2 = lo + nsh;
-            const double tin_n = theta_int_new(G, sn, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
+            const double tin_n = theta_int_new(G, snr, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
             c = tin_n*cb.ga*tin
                 + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
                 - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
         const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
         if (i >= nsh) continue;
         const i64 b = i + (i64)j*nsh;
-        const ColumnLayer cl = column_layer(G, so, sn, td, dt, rk, i, j);
+        const ColumnLayer cl = column_layer(G, so, snr, td, dt, rk, i, j);
         const double v = vol[b];
         double rho_n, rt_n;
         if (j == 0) {
@@ -1340,8 +1340,8 @@ void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr
 { LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, ptr, phr, dt); }
 
 template <typename TWIND>
-static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
-                          const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
+static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,
+                          const DevState &td, const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
 {
     dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
     size_t lds = (size_t)5*G.L*VW_NCOL*sizeof(double);   /* 65 KiB at 26 layers, 160 KiB at 64 */
@@ -1351,25 +1351,25 @@ static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &
                                   (int)lds);
         lds_allowed = lds;
     }
-    LAUNCH_SHM("k_ver_waves", k_ver_waves_lds<TWIND>, g, b, lds, st, G, so, sn, td, td_wind, K, rk);
+    LAUNCH_SHM("k_ver_waves", k_ver_waves_lds<TWIND>, g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
 }
 
-void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
-               int rk, hipStream_t st)
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
+               const StepConsts &K, int rk, hipStream_t st)
 {
     if (!G.vw_streaming) {
-        ver_waves_lds<double>(G, so, sn, td, td.wind, K, rk, st);
+        ver_waves_lds<double>(G, so, snr, sn, td, td.wind, K, rk, st);
         return;
     }
     dim3 g((G.nsh + 127)/128), b(128);
-    if (G.L == 26) LAUNCH("k_ver_waves", k_ver_waves<26>, g, b, st, G, so, sn, td, K, rk);
-    else if (G.L == 55) LAUNCH("k_ver_waves", k_ver_waves<55>, g, b, st, G, so, sn, td, K, rk);
-    else LAUNCH("k_ver_waves", k_ver_waves<0>, g, b, st, G, so, sn, td, K, rk);
+    if (G.L == 26) LAUNCH("k_ver_waves", k_ver_waves<26>, g, b, st, G, so, snr, sn, td, K, rk);
+    else if (G.L == 55) LAUNCH("k_ver_waves", k_ver_waves<55>, g, b, st, G, so, snr, sn, td, K, rk);
+    else LAUNCH("k_ver_waves", k_ver_waves<0>, g, b, st, G, so, snr, sn, td, K, rk);
 }
 
-void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *td_wind,
-               const StepConsts &K, int rk, hipStream_t st)
-{ ver_waves_lds<float>(G, so, sn, td, td_wind, K, rk, st); }
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
+               const float *td_wind, const StepConsts &K, int rk, hipStream_t st)
+{ ver_waves_lds<float>(G, so, snr, sn, td, td_wind, K, rk, st); }
 
 static void gen_densities_only(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
                                const StepConsts &K, hipStream_t st)

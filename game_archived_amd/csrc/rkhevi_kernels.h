@@ -37,10 +37,12 @@ void phase_trans(const DevGrid &G, const DevState &s, const double *T, double *p
                  hipStream_t st);
 void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr, float *phr, double dt,
                  hipStream_t st);
-void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const StepConsts &K,
-               int rk, hipStream_t st);
-void ver_waves(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *td_wind,
+/* snr: the new state as read before this substep writes it (at rk 0 the old
+   state, which equals it by the step's entry condition) */
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
                const StepConsts &K, int rk, hipStream_t st);
+void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
+               const float *td_wind, const StepConsts &K, int rk, hipStream_t st);
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
                    double *sed_heat, const StepConsts &K, int rk, hipStream_t st);
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,

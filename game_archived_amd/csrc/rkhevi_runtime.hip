@@ -431,7 +431,12 @@ static int get_irrev_t(const WK &W, game_irrev_view_t *h)
 }
 
 /* manage_rkhevi.c:20-100 on the device-resident states; td_wind is the
-   tendency state's wind (FP64 form) or the FP32 wind tendency (mixed) */
+   tendency state's wind (FP64 form) or the FP32 wind tendency (mixed).
+   The step's entry condition is state_new == state_old (src/coordinator.c:305),
+   so whatever rk 0 reads of state_new before writing it is read from
+   state_old: the step then also runs on a state_new buffer that holds an
+   older state, which lets game_gpu_run_steps swap the two buffers instead of
+   copying. */
 template <class WK, typename TW>
 static int step_t(const WK &W, TW *td_wind, double dt, int first)
 {
@@ -439,17 +444,19 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
     const DevGrid &G = M->G;
     DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
     M->K.dt = dt;
-    const double *rho_gas_new = sn.rho + (i64)G.n_cond*G.s;
     col::temperature(G, so, W.temp, st);
     if (G.moist) col::phase_trans(G, so, W.temp, W.ptr, W.phr, dt, st);
     for (int rk = 0; rk < 2; ++rk) {
-        if (rk == 0) hor::pgrad(G, sn, W, first, st);
-        else hor::pgc_v(G, sn, W, st);
-        if (rk == 1 || first) hor::advection(G, sn.wind, rho_gas_new, W, st);
+        const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
+        DevState sns = snr;                                 /* ... with the horizontal wind just updated */
+        sns.wind = sn.wind;
+        if (rk == 0) hor::pgrad(G, snr, W, first, st);
+        else hor::pgc_v(G, snr, W, st);
+        if (rk == 1 || first) hor::advection(G, snr.wind, snr.rho + (i64)G.n_cond*G.s, W, st);
         hor::vector_tend(G, W, td_wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
-        hor::scalar_tend(G, sn, td, W, rk, st);
-        if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, sn, td, M->K, rk, st);
-        else col::ver_waves(G, so, sn, td, (const float *)td_wind, M->K, rk, st);
+        hor::scalar_tend(G, sns, td, W, rk, st);
+        if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
+        else col::ver_waves(G, so, snr, sn, td, (const float *)td_wind, M->K, rk, st);
         if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.sed_heat, M->K, rk, st);
     }
     HIP_OK(hipGetLastError());
@@ -498,9 +505,13 @@ int game_gpu_run_steps(int n, double dt, int first)
 {
     if (!M) return GAME_ERR_NOT_INIT;
     for (int k = 0; k < n; ++k) {
-        int rc = game_gpu_linear_combine_two_states(GAME_STATE_NEW, GAME_STATE_OLD, GAME_STATE_OLD, 1.0, 0.0);
-        if (rc) return rc;
-        rc = game_gpu_manage_rkhevi(dt, first);
+        /* src/coordinator.c:305 copies state_new into state_old; exchanging the
+           two buffers gives the step the same inputs (see step_t): the fields
+           the step never writes (top and surface w) are equal in both */
+        DevState t = M->s[GAME_STATE_OLD];
+        M->s[GAME_STATE_OLD] = M->s[GAME_STATE_NEW];
+        M->s[GAME_STATE_NEW] = t;
+        int rc = game_gpu_manage_rkhevi(dt, first);
         if (rc) return rc;
         first = 0;
     }
