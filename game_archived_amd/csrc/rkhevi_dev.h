@@ -46,6 +46,8 @@ struct DevGrid {
     /* number of layer chunks the per-cell kernels split a column into
        (gridDim.y); more chunks give more waves to hide gather latency */
     int split_cell;
+    /* the same for the per-edge kernels */
+    int split_edge;
     /* 1: sound-wave solver streams each column through registers; 0: LDS-staged */
     int vw_streaming;
 };

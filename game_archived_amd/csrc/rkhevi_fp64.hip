@@ -123,7 +123,9 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
     if (e >= G.nvh) return;
     const int fr = G.from[e], to = G.to[e];
     const i64 gi = (i64)G.n_cond*G.s;
-    for (int l = 0; l < G.L; ++l) {
+    int l0, l1;
+    layer_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         i64 vi = G.nsh + (i64)l*G.vpl + e;
         if (!first_step) W.pgrad_old[vi] = (TW)(-(double)W.pgrad_nl[vi] - (double)W.pgrad_l[vi]);
         i64 sf = (i64)l*G.nsh + fr, st = (i64)l*G.nsh + to;
@@ -241,8 +243,10 @@ __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, co
     double drw[4];
     for (int j = 0; j < 4; ++j) { dri[j] = G.dri[4*e + j]; drw[j] = G.drw[4*e + j]; }
     const double f_t = G.fvec[e], f_r = G.fvec[nvh + e];
-    double rv1 = rel_vort_tangential<FLAT>(G, w, 1, e, fr, to);
-    for (int l = 0; l <= L; ++l) {
+    int l0, l1;
+    layer_chunk(L + 1, l0, l1);
+    double rv1 = (l0 <= 1) ? rel_vort_tangential<FLAT>(G, w, 1, e, fr, to) : 0.0;
+    for (int l = l0; l < l1; ++l) {
         /* tangential element at level l; the top copies level 1 (vorticities.c:158-163) */
         double rv = (l <= 1) ? rv1 : rel_vort_tangential<FLAT>(G, w, l, e, fr, to);
         double dv;
@@ -284,7 +288,9 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F
     const int fr = G.from[e], to = G.to[e];
     const bool pent_f = fr < GAME_N_PENT, pent_t = to < GAME_N_PENT;
     const double *ipw = G.ipw;
-    for (int l = 0; l < G.L; ++l) {
+    int l0, l1;
+    layer_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
         double o = 0;
         if (pent_f) {
@@ -374,7 +380,9 @@ __global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const TW *f, TW *out)
     int e = blockIdx.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int fr = G.from[e], to = G.to[e];
-    for (int l = 0; l < G.L; ++l) out[G.nsh + (i64)l*G.vpl + e] = (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to);
+    int l0, l1;
+    layer_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) out[G.nsh + (i64)l*G.vpl + e] = (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to);
 }
 
 /* grad, interior w points */
@@ -467,13 +475,17 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
     const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro, ncc = G.n_cond, nc = G.n_const;
     const i64 S = G.s, nvcc = (i64)no*nsh;
     const int ne = h < GAME_N_PENT ? 5 : 6;
-    int ej[6], fj[6], tj[6];
-    double sj[6];
+    /* each edge as (edge, the cell across it, whether h is its from-cell);
+       the centred fluxes are symmetric in the two cells and IEEE addition
+       commutes, so they come out bit for bit as the reference forms them
+       from from/to */
+    int ej[6], nb[6];
+    unsigned from_bits = 0;
     for (int j = 0; j < 6; ++j) {
-        ej[j] = G.adj[6*h + j];
-        sj[j] = G.sgn[6*h + j];
-        fj[j] = G.from[ej[j]];
-        tj[j] = G.to[ej[j]];
+        const int e = G.adj[6*h + j], f = G.from[e], t = G.to[e];
+        ej[j] = e;
+        nb[j] = (f == h) ? t : f;
+        if (G.sgn[6*h + j] > 0) from_bits |= 1u << j;
     }
     int l0, l1;
     layer_chunk(L, l0, l1);
@@ -500,14 +512,18 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
             const i64 shift = (i64)k*S;
             const double new_w = (rk == 1 && k != ncc) ? 0.5 : 1.0, old_w = 1.0 - new_w;
             const double *rk_ = s.rho + shift;
+            const double rho_h = rk_[i];
             if (k == ncc) {
+                const double th_h = s.rt[i]/rho_h;
                 double c1 = 0.0, c2 = 0.0;
                 for (int j = 0; j < ne; ++j) {
-                    i64 st = (i64)l*nsh + tj[j], sf = (i64)l*nsh + fj[j];
-                    double f1 = (0.5*(rk_[st] + rk_[sf]))*u[j];
-                    double f2 = (0.5*(s.rt[st]/rk_[st] + s.rt[sf]/rk_[sf]))*f1;
-                    c1 += f1*sj[j]*a[j];
-                    c2 += f2*sj[j]*a[j];
+                    const i64 sb = (i64)l*nsh + nb[j];
+                    const double rho_nb = rk_[sb];
+                    const double sg = ((from_bits >> j) & 1u) ? 1.0 : -1.0;
+                    double f1 = (0.5*(rho_h + rho_nb))*u[j];
+                    double f2 = (0.5*(th_h + s.rt[sb]/rho_nb))*f1;
+                    c1 += f1*sg*a[j];
+                    c2 += f2*sg*a[j];
                 }
                 double v1 = 0.0, v2 = 0.0;
                 if (!FLAT) {
@@ -524,7 +540,7 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 }
                 double div1 = 1.0/vol*(c1 + v1);
                 double div2 = 1.0/vol*(c2 + v2);
-                double rg = rk_[i];
+                const double rg = rho_h;
                 if (!G.moist) {
                     /* dry: every source term of the reference sum is a zero array */
                     td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1);
@@ -533,7 +549,7 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                     /* the moist-air gas takes the vapour phase-transition rate
                        (scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
                     double ptr_v = (double)W.ptr[shift + i];
-                    double th = s.rt[i]/rg;
+                    const double th = th_h;
                     td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
                     double cv = c_v_mass_weighted(G, s.rho, (double)W.temp[i], i);
                     double heat = (double)W.phr[i] + (double)W.sed_heat[i];
@@ -545,8 +561,11 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
             } else {
                 double c = 0.0;
                 for (int j = 0; j < ne; ++j) {
-                    double sv = (u[j] >= 0.0) ? rk_[(i64)l*nsh + fj[j]] : rk_[(i64)l*nsh + tj[j]];
-                    c += (sv*u[j])*sj[j]*a[j];
+                    /* upstream density: the from-cell for u >= 0 */
+                    const bool is_from = (from_bits >> j) & 1u;
+                    const double rho_nb = rk_[(i64)l*nsh + nb[j]];
+                    const double sv = ((u[j] >= 0.0) == is_from) ? rho_h : rho_nb;
+                    c += (sv*u[j])*(is_from ? 1.0 : -1.0)*a[j];
                 }
                 double v = 0.0;
                 if (!FLAT) {
@@ -1237,7 +1256,7 @@ template <typename TW>
 static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, hipStream_t st)
 {
     LAUNCH("k_pgrad_w", k_pgrad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-    FLAT_DISPATCH("k_pgrad_e", k_pgrad_e, GRID1(G.nvh), G, s, W, first);
+    FLAT_DISPATCH("k_pgrad_e", k_pgrad_e, GRID2(G.nvh, G.split_edge), G, s, W, first);
 }
 
 template <typename TW>
@@ -1251,13 +1270,13 @@ static void advection_t(const DevGrid &G, const double *wind, const double *rho_
     LAUNCH("k_mass_flux", k_mass_flux<TW>, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas,
            wind, W.flux);
     LAUNCH("k_rel_vort_tri", k_rel_vort_tri<TW>, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
-    FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID1(G.nvh), G, wind, rho_gas, (const TW *)W.rvt, W.pv);
-    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<TW>, GRID1(G.nvh), dim3(TPB), st, G, (const TW *)W.flux,
+    FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID2(G.nvh, G.split_edge), G, wind, rho_gas, (const TW *)W.rvt, W.pv);
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<TW>, GRID2(G.nvh, G.split_edge), dim3(TPB), st, G, (const TW *)W.flux,
            (const TW *)W.pv, W.pvt);
     LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G,
            (const TW *)W.flux, (const TW *)W.pv, W.pvt);
     LAUNCH("k_inner_product", k_inner_product<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, wind, wind, W.vsq);
-    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, (const TW *)W.vsq, W.vsq_grad);
+    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID2(G.nvh, G.split_edge), G, (const TW *)W.vsq, W.vsq_grad);
     LAUNCH("k_grad_w", k_grad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, (const TW *)W.vsq, W.vsq_grad);
 }
 
@@ -1307,7 +1326,7 @@ void inner_product(const DevGrid &G, const double *a, const double *b, double *o
 void grad(const DevGrid &G, const double *f, double *out, hipStream_t st)
 {
     typedef double TW;
-    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID1(G.nvh), G, f, out);
+    FLAT_DISPATCH("k_grad_e", k_grad_e, GRID2(G.nvh, G.split_edge), G, f, out);
     LAUNCH("k_grad_w", k_grad_w<double>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, f, out);
 }
 
@@ -1316,7 +1335,7 @@ void curl_triangles(const DevGrid &G, const double *wind, double *out, hipStream
 
 void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double *out, hipStream_t st)
 {
-    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<double>, GRID1(G.nvh), dim3(TPB), st, G, F, pv, out);
+    LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<double>, GRID2(G.nvh, G.split_edge), dim3(TPB), st, G, F, pv, out);
     LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w<double>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, F, pv, out);
 }
 

@@ -237,6 +237,11 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         int v = atoi(e);
         if (v >= 1 && v <= d.n_layers) G.split_cell = v;
     }
+    G.split_edge = 2;
+    if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
+        int v = atoi(e);
+        if (v >= 1 && v <= d.n_layers) G.split_edge = v;
+    }
     G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
     size_t S = (size_t)d.s, V = (size_t)d.v, NVH = (size_t)d.nvh, NSH = (size_t)d.nsh, NT = (size_t)d.nt;
 #define UP(dst, src, n) do { if ((rc = upload(&(dst), (src), (n)))) return rc; } while (0)

@@ -98,9 +98,10 @@ def test_moist_condensates_mountain_matches_oracle():
 
 @pytest.mark.parametrize("split", ["1", "7", "26"])
 def test_results_independent_of_layer_split(monkeypatch, split):
-    """How many layer chunks the per-cell kernels use changes the launch
-    shape only: the bits are those of the oracle for every split."""
+    """How many layer chunks the per-cell and per-edge kernels use changes
+    the launch shape only: the bits are those of the oracle for every split."""
     monkeypatch.setenv("GAME_GPU_SPLIT_CELL", split)
+    monkeypatch.setenv("GAME_GPU_SPLIT_EDGE", split)
     print(f"case: res 4 moist flat, split {split}, 2 steps", flush=True)
     g = get_grid(4, "flat", moist=1)
     s0 = seeded_condensates(g, oracle_state(g, moist=True))
