@@ -66,7 +66,9 @@ def kernel_bytes(name, dims, wb=8):
         # every tracer of a column: rho_k old, tendency and new; w old and new,
         # area at w, volume (shared)
         "k_gen_densities": 8 * ((nc - 1) * 3 * S + 3 * W_ + S),
-        "k_sed_heat": 8 * (4 * S + 2 * W_ + S) + wb * 2 * S,
+        # condensate densities, w and area at two interfaces, volume; temperature,
+        # the condensates' c_p table (work) and the output
+        "k_sed_heat": 8 * (4 * S + 2 * W_ + S) + wb * (2 * S + 4 * S),
         # TRSK flux: mass flux (edges, w), PV (tangential, rhombus), output (work
         # arrays); ipw slots 6/7; TRSK tables once
         "k_vorticity_flux_e": wb * (E + (L + 1) * NSH + (L + 1) * NVH + L * NVH + E) + 8 * 2 * S
@@ -79,7 +81,7 @@ def kernel_bytes(name, dims, wb=8):
         # tendencies (read + write); moist: rates, temperature, heating, sediment
         # heat (work) and exner (state, background)
         "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (2 * S if moist else 0))
-                         + (wb * 8 * S if moist else 0) + 4 * 12 * NSH,
+                         + (wb * (8 + 4) * S if moist else 0) + 4 * 12 * NSH,
         "k_vector_tend": wb * (7 * V + W_) + 8 * 2 * E,
         "k_pgrad_e": 8 * (S * (3 + nc) + 2 * E) + wb * 5 * E + 4 * 2 * NVH,
         "k_pgrad_w": 8 * (S * (3 + nc) + 3 * W_) + wb * 6 * W_,
@@ -87,8 +89,8 @@ def kernel_bytes(name, dims, wb=8):
         "k_grad_w": wb * (S + W_) + 8 * W_,
         "k_inner_product": 8 * (V + 8 * S) + wb * S + 4 * 6 * NSH,
         "k_mass_flux": 8 * (S + E + W_) + wb * (E + W_) + 4 * 2 * NVH,
-        "k_phase_trans": 8 * nc * S + wb * 7 * S,
-        "k_temperature": 8 * 6 * S + wb * S,
+        "k_phase_trans": 8 * nc * S + wb * (7 + 4) * S,
+        "k_temperature": 8 * 6 * S + wb * (S + (4 * S if moist else 0)),
         "k_lincomb_scalars": 8 * (2 * (nc + 2) * S + (nc + 3) * S),
         "k_lincomb_vector": 8 * 3 * V,
     }
@@ -137,6 +139,26 @@ def copy_probe(mib):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(kernel, form, res, layers, moist):
+    """HBM bytes per launch of a kernel from the committed counter runs
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this bench at a few steps,
+    each scaled by the same counter on a plain device copy of known size);
+    None when no matching measurement is committed"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = doc.get("workload", {})
+    if (w.get("res"), w.get("layers"), w.get("moist")) != (res, layers, moist):
+        return None
+    k = doc.get("forms", {}).get(form, {}).get(kernel)
+    return None if k is None else k.get("traffic_bytes")
 
 
 def print_kernel_table(stats, d, wb):
@@ -297,7 +319,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": dom_name, "form": main_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_us": round(dom_avg_s * 1e6, 2), "algorithmic_bytes": dom_bytes},
+                     "traffic": measured_traffic(dom_name, main_name, args.res, args.layers, moist),
+                     "traffic_source": "profiles/hbm_traffic.json", "avg_launch_us": round(dom_avg_s * 1e6, 2),
+                     "algorithmic_bytes": dom_bytes},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -72,6 +72,9 @@ struct DevWorkT {
     double *vcc;
     /* wind tendency (the FP64 form keeps it in the tendency state) */
     TW *td_wind;
+    /* c_p of every condensate at the step's temperature, n_cond x S: the
+       temperature is fixed for the whole step, so each value is formed once */
+    TW *cpc;
 };
 typedef DevWorkT<double> DevWork;
 typedef DevWorkT<float> DevWorkF;
@@ -107,6 +110,18 @@ __device__ __forceinline__ double c_p_cond(int k, double t)
         r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
     }
     return 4186.8*r;
+}
+
+/* c_v_mass_weighted_air with the condensates' c_p taken from a table formed
+   at the step's temperature (same terms, same order) */
+template <typename TW>
+__device__ __forceinline__ double c_v_mass_weighted_cp(const DevGrid &G, const double *rho, const TW *cpc, i64 i)
+{
+    double r = 0.0;
+    for (int k = 0; k < G.n_cond; ++k) r += rho[k*G.s + i]*(double)cpc[k*G.s + i];
+    r += rho[(i64)G.n_cond*G.s + i]*GC_C_D_V;
+    if (G.moist == 1) r += rho[(i64)(G.n_cond + 1)*G.s + i]*GC_C_V_V;
+    return r;
 }
 
 /* derived_quantities.c:97-115 */

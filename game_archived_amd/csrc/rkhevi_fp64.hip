@@ -16,11 +16,29 @@
 #define GRID1(n) dim3((unsigned)(((n) + TPB - 1)/TPB))
 #define GRID2(n, chunks) dim3((unsigned)(((n) + TPB - 1)/TPB), (unsigned)(chunks))
 
-/* layers [l0, l1) of chunk blockIdx.y when n layers are split into gridDim.y chunks */
-__device__ __forceinline__ void layer_chunk(int n, int &l0, int &l1)
+/* Workgroups are dispatched to the eight XCDs round-robin (workgroup i on
+   XCD i % 8), each XCD with its own L2.  The gather kernels renumber their
+   workgroups so that each XCD takes one contiguous range of the grid: the
+   neighbour data a workgroup shares with the next one then sits in the same
+   L2 instead of being fetched by several XCDs. */
+#define N_XCD 8u
+struct Blk { unsigned x, y; };
+__device__ __forceinline__ Blk xcd_block()
 {
-    l0 = (int)(((i64)n*blockIdx.y)/gridDim.y);
-    l1 = (int)(((i64)n*(blockIdx.y + 1))/gridDim.y);
+    const unsigned T = gridDim.x*gridDim.y, lin = blockIdx.y*gridDim.x + blockIdx.x;
+    const unsigned q = T/N_XCD, r = T%N_XCD, x = lin%N_XCD, k = lin/N_XCD;
+    const unsigned m = (x < r) ? x*(q + 1) + k : r*(q + 1) + (x - r)*q + k;
+    Blk b;
+    b.x = m%gridDim.x;
+    b.y = m/gridDim.x;
+    return b;
+}
+
+/* layers [l0, l1) of chunk `chunk` when n layers are split into gridDim.y chunks */
+__device__ __forceinline__ void layer_chunk(int n, unsigned chunk, int &l0, int &l1)
+{
+    l0 = (int)(((i64)n*chunk)/gridDim.y);
+    l1 = (int)(((i64)n*(chunk + 1))/gridDim.y);
 }
 
 /* ------------------------------------------------------------------ */
@@ -55,7 +73,7 @@ __device__ __forceinline__ double remap_verpri2horpri(const DevGrid &G, const do
 
 /* derived_quantities.c:15-40 */
 template <typename TO>
-__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *T)
+__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *T, TO *cpc)
 {
     i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
     if (i >= G.s) return;
@@ -63,6 +81,10 @@ __global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *
     if (G.moist)
         t = t/(1.0 + s.rho[(i64)(G.n_cond + 1)*G.s + i]/s.rho[(i64)G.n_cond*G.s + i]*(GC_M_D/GC_M_V - 1.0));
     T[i] = (TO)t;
+    if (cpc) {
+        const double tt = (double)(TO)t;
+        for (int k = 0; k < G.n_cond; ++k) cpc[(i64)k*G.s + i] = (TO)c_p_cond(k, tt);   /* dictionary.c:111-190 */
+    }
 }
 
 /* manage_pressure_gradient.c:17-87 at the interior w points (layers 1..L-1),
@@ -71,7 +93,7 @@ template <typename TW>
 __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad)
 {
     i64 n = (i64)(G.L - 1)*G.nsh;
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
     if (t >= n) return;
     int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
     i64 w = h + (i64)lev*G.vpl, up = h + (i64)(lev - 1)*G.nsh, lo = h + (i64)lev*G.nsh;
@@ -119,12 +141,13 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int
 template <bool FLAT, typename TW>
 __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step)
 {
-    int e = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int e = bk.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int fr = G.from[e], to = G.to[e];
     const i64 gi = (i64)G.n_cond*G.s;
     int l0, l1;
-    layer_chunk(G.L, l0, l1);
+    layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 vi = G.nsh + (i64)l*G.vpl + e;
         if (!first_step) W.pgrad_old[vi] = (TW)(-(double)W.pgrad_nl[vi] - (double)W.pgrad_l[vi]);
@@ -147,7 +170,7 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_gas, const double *wind, TW *flux)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
     i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
     if (t < ne) {
         int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
@@ -167,7 +190,7 @@ __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, TW *res)
 {
-    int h = blockIdx.x*TPB + threadIdx.x;
+    int h = xcd_block().x*TPB + threadIdx.x;
     if (h >= G.nt) return;
     int vit[3], vst[3];
     for (int j = 0; j < 3; ++j) { vit[j] = G.vit[3*h + j]; vst[j] = G.vst[3*h + j]; }
@@ -235,7 +258,8 @@ template <bool FLAT, typename TW>
 __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, const double *rho, const TW *rvt,
                                                   TW *pv)
 {
-    int e = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int e = bk.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int nvh = G.nvh, nsh = G.nsh, L = G.L;
     const int fr = G.from[e], to = G.to[e], fd = G.dfrom[e], td = G.dto[e];
@@ -244,7 +268,7 @@ __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, co
     for (int j = 0; j < 4; ++j) { dri[j] = G.dri[4*e + j]; drw[j] = G.drw[4*e + j]; }
     const double f_t = G.fvec[e], f_r = G.fvec[nvh + e];
     int l0, l1;
-    layer_chunk(L + 1, l0, l1);
+    layer_chunk(L + 1, bk.y, l0, l1);
     double rv1 = (l0 <= 1) ? rel_vort_tangential<FLAT>(G, w, 1, e, fr, to) : 0.0;
     for (int l = l0; l < l1; ++l) {
         /* tangential element at level l; the top copies level 1 (vorticities.c:158-163) */
@@ -279,7 +303,8 @@ __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, co
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F, const TW *pv, TW *out)
 {
-    int e = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int e = bk.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int nsh = G.nsh, nvh = G.nvh;
     double tw[10];
@@ -289,7 +314,7 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F
     const bool pent_f = fr < GAME_N_PENT, pent_t = to < GAME_N_PENT;
     const double *ipw = G.ipw;
     int l0, l1;
-    layer_chunk(G.L, l0, l1);
+    layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
         double o = 0;
@@ -323,14 +348,15 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F, const TW *pv, TW *out)
 {
-    int h = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int h = bk.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
     const int nsh = G.nsh, nvh = G.nvh;
     const int ne = h < GAME_N_PENT ? 5 : 6;
     int adj[6];
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
     int l0, l1;
-    layer_chunk(G.L + 1, l0, l1);
+    layer_chunk(G.L + 1, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         double vw = 0.5;
         if (l == 0 || l == G.L) vw = 1;
@@ -351,13 +377,14 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *a, const double *b, TW *out)
 {
-    int h = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int h = bk.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
     const int ne = h < GAME_N_PENT ? 5 : 6;
     int adj[6];
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
     int l0, l1;
-    layer_chunk(G.L, l0, l1);
+    layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 i = (i64)l*G.nsh + h;
         const double *w = G.ipw + 8*i;
@@ -377,11 +404,12 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
 template <bool FLAT, typename TW>
 __global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const TW *f, TW *out)
 {
-    int e = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int e = bk.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
     const int fr = G.from[e], to = G.to[e];
     int l0, l1;
-    layer_chunk(G.L, l0, l1);
+    layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) out[G.nsh + (i64)l*G.vpl + e] = (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to);
 }
 
@@ -389,7 +417,7 @@ __global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const TW *f, TW *out)
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_grad_w(DevGrid G, const TW *f, TW *out)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
     if (t >= (i64)(G.L - 1)*G.nsh) return;
     int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
     out[h + (i64)lev*G.vpl] = (TW)grad_w_raw(G, f, lev, h);
@@ -433,7 +461,7 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWorkT<TW> W, 
    exactly as scalar_times_vector_h forms them (multiplications.c:45-67). */
 __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
 {
-    i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
     const i64 n = (i64)G.n_oro*G.nsh;
     if (t >= n) return;
     const int nsh = G.nsh, L = G.L;
@@ -470,7 +498,8 @@ template <bool FLAT, typename TW>
 __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevWorkT<TW> W, double *td_rho,
                                                      double *td_rt, int rk)
 {
-    int h = blockIdx.x*TPB + threadIdx.x;
+    const Blk bk = xcd_block();
+    int h = bk.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
     const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro, ncc = G.n_cond, nc = G.n_const;
     const i64 S = G.s, nvcc = (i64)no*nsh;
@@ -488,7 +517,7 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
         if (G.sgn[6*h + j] > 0) from_bits |= 1u << j;
     }
     int l0, l1;
-    layer_chunk(L, l0, l1);
+    layer_chunk(L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         const i64 i = (i64)l*nsh + h;
         double a[6], u[6];
@@ -551,7 +580,7 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                     double ptr_v = (double)W.ptr[shift + i];
                     const double th = th_h;
                     td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
-                    double cv = c_v_mass_weighted(G, s.rho, (double)W.temp[i], i);
+                    double cv = c_v_mass_weighted_cp(G, s.rho, W.cpc, i);
                     double heat = (double)W.phr[i] + (double)W.sed_heat[i];
                     td_rt[i] = old_w*td_rt[i] + new_w*(
                         -div2
@@ -763,7 +792,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
 {
     extern __shared__ double lds[];
     const int L = G.L, nsh = G.nsh, vpl = G.vpl;
-    const int i0 = blockIdx.x*VW_NCOL;
+    const int i0 = xcd_block().x*VW_NCOL;
     const i64 gi = (i64)G.n_cond*G.s;
     const double dt = K.dt, iw = K.impl_weight;
     const double dt2 = dt*dt;
@@ -873,7 +902,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevStat
 
    The tracers of a column are independent of each other except through the
    sediment heat, which k_sed_heat forms separately, so one block solves NCOL
-   columns of one tracer (blockIdx.y) in three phases:
+   columns of one tracer in three phases:
      1. every thread builds rows (column, layer) of the tridiagonal system
         into LDS -- all loads independent, fully coalesced;
      2. one lane per column runs the Thomas elimination on LDS;
@@ -908,8 +937,12 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
 {
     extern __shared__ double lds[];
     const int L = G.L, nsh = G.nsh, ncc = G.n_cond;
-    const int k = (int)blockIdx.y < ncc ? (int)blockIdx.y : (int)blockIdx.y + 1;   /* skip the gas */
-    const int i0 = blockIdx.x*GD_NCOL;
+    /* the tracer is the fast block index: the blocks of one group of columns
+       run side by side on one XCD and share the wind, area and volume columns in L2 */
+    const unsigned ntr = (unsigned)G.n_const - 1, blk = xcd_block().x;
+    const int tr = (int)(blk%ntr);
+    const int k = tr < ncc ? tr : tr + 1;   /* skip the gas */
+    const int i0 = (int)(blk/ntr)*GD_NCOL;
     const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
     const double *vol = G.vol, *area = G.area;
     const double *rho_o = so.rho + (i64)k*G.s;
@@ -992,10 +1025,10 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
    sh updates of :500-611): one thread per grid box adds the contributions of
    the condensates in the reference's constituent order, starting from 0 */
 template <typename TW>
-__global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, const TW *T, TW *sed_heat,
-                                                  StepConsts K)
+__global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, const TW *T, const TW *cpc,
+                                                  TW *sed_heat, StepConsts K)
 {
-    const i64 b = (i64)blockIdx.x*TPB + threadIdx.x;
+    const i64 b = (i64)xcd_block().x*TPB + threadIdx.x;
     if (b >= G.s) return;
     const int nsh = G.nsh, vpl = G.vpl, L = G.L, ncc = G.n_cond;
     const int j = (int)(b/nsh), i = (int)(b - (i64)j*nsh);
@@ -1015,11 +1048,11 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
             fr -= vsink;
             fr = area[i + (i64)(jj + 1)*vpl]*fr;
             const i64 bb = i + (i64)jj*nsh;
-            double doi, toi;
-            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = (double)T[bb + nsh]; }
-            else { doi = rho_o[bb]; toi = (double)T[bb]; }
+            double doi, toi, cpi;
+            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = (double)T[bb + nsh]; cpi = (double)cpc[(i64)k*G.s + bb + nsh]; }
+            else { doi = rho_o[bb]; toi = (double)T[bb]; cpi = (double)cpc[(i64)k*G.s + bb]; }
             fr = doi*fr;
-            const double f = c_p_cond(k, toi)*toi*fr;
+            const double f = cpi*toi*fr;
             if (pass == 0) fe = f; else fe_m = f;
         }
         if (j == 0) sh += fe/vol[b];
@@ -1027,7 +1060,7 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
             const i64 low = i + G.s - nsh, sfc = i + G.v - nsh;
             sh += -fe_m/vol[b];
             const double tl = (double)T[low];
-            sh += -vsink*tl*c_p_cond(k, tl)*rho_o[low]*area[sfc]/vol[b];
+            sh += -vsink*tl*(double)cpc[(i64)k*G.s + low]*rho_o[low]*area[sfc]/vol[b];
         } else sh += (-fe_m + fe)/vol[b];
     }
     sed_heat[b] = (TW)sh;
@@ -1106,8 +1139,8 @@ __device__ __forceinline__ double phase_trans_heat(int dir, double t)
 }
 
 template <typename TW>
-__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, const TW *temp, TW *ptr, TW *phr,
-                                                     double dt)
+__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, const TW *temp, const TW *cpc, TW *ptr,
+                                                     TW *phr, double dt)
 {
     i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
     if (i >= G.s) return;
@@ -1122,7 +1155,7 @@ __global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, cons
     double ap = dp + wvp;
     double ef = (T >= GC_T_0) ? 1.00071*exp(0.000000045*ap) : 0.99882*exp(0.00000008*ap);
     sp = ef*sp;
-    double cvm = c_v_mass_weighted(G, rho, T, i);
+    double cvm = c_v_mass_weighted_cp(G, rho, cpc, i);
     int warm = T >= GC_T_0;
     int dir = warm ? 0 : 1;
     double pth = phase_trans_heat(dir, T);
@@ -1346,17 +1379,17 @@ void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double 
    mixed form) */
 namespace col {
 
-void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st)
-{ LAUNCH("k_temperature", k_temperature<double>, GRID1(G.s), dim3(TPB), st, G, s, T); }
-void temperature(const DevGrid &G, const DevState &s, float *T, hipStream_t st)
-{ LAUNCH("k_temperature", k_temperature<float>, GRID1(G.s), dim3(TPB), st, G, s, T); }
+void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<double>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc); }
+void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc); }
 
-void phase_trans(const DevGrid &G, const DevState &s, const double *T, double *ptr, double *phr, double dt,
-                 hipStream_t st)
-{ LAUNCH("k_phase_trans", k_phase_trans<double>, GRID1(G.s), dim3(TPB), st, G, s, T, ptr, phr, dt); }
-void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr, float *phr, double dt,
-                 hipStream_t st)
-{ LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, ptr, phr, dt); }
+void phase_trans(const DevGrid &G, const DevState &s, const double *T, const double *cpc, double *ptr, double *phr,
+                 double dt, hipStream_t st)
+{ LAUNCH("k_phase_trans", k_phase_trans<double>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, ptr, phr, dt); }
+void phase_trans(const DevGrid &G, const DevState &s, const float *T, const float *cpc, float *ptr, float *phr,
+                 double dt, hipStream_t st)
+{ LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, ptr, phr, dt); }
 
 template <typename TWIND>
 static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,
@@ -1393,7 +1426,7 @@ void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const 
 static void gen_densities_only(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
                                const StepConsts &K, hipStream_t st)
 {
-    dim3 g((G.nsh + GD_NCOL - 1)/GD_NCOL, G.n_const - 1), b(GD_THREADS);
+    dim3 g(((G.nsh + GD_NCOL - 1)/GD_NCOL)*(G.n_const - 1)), b(GD_THREADS);
     size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
     static size_t lds_allowed = 65536;
     if (lds > lds_allowed) {
@@ -1404,18 +1437,18 @@ static void gen_densities_only(const DevGrid &G, const DevState &so, const DevSt
 }
 
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
-                   double *sed_heat, const StepConsts &K, int rk, hipStream_t st)
+                   const double *cpc, double *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
     if (rk == 0 && G.n_cond > 0)
-        LAUNCH("k_sed_heat", k_sed_heat<double>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, sed_heat, K);
+        LAUNCH("k_sed_heat", k_sed_heat<double>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
     gen_densities_only(G, so, sn, td, K, st);
 }
 
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,
-                   float *sed_heat, const StepConsts &K, int rk, hipStream_t st)
+                   const float *cpc, float *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
     if (rk == 0 && G.n_cond > 0)
-        LAUNCH("k_sed_heat", k_sed_heat<float>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, sed_heat, K);
+        LAUNCH("k_sed_heat", k_sed_heat<float>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
     gen_densities_only(G, so, sn, td, K, st);
 }
 

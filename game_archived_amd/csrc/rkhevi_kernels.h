@@ -31,12 +31,13 @@ void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double 
 /* column solvers, point physics and state arithmetic shared by both forms:
    FP64 arithmetic; the work arrays are double (FP64 form) or float (mixed) */
 namespace col {
-void temperature(const DevGrid &G, const DevState &s, double *T, hipStream_t st);
-void temperature(const DevGrid &G, const DevState &s, float *T, hipStream_t st);
-void phase_trans(const DevGrid &G, const DevState &s, const double *T, double *ptr, double *phr, double dt,
-                 hipStream_t st);
-void phase_trans(const DevGrid &G, const DevState &s, const float *T, float *ptr, float *phr, double dt,
-                 hipStream_t st);
+/* cpc (may be null): c_p of every condensate at the temperature, n_cond x S */
+void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, hipStream_t st);
+void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, hipStream_t st);
+void phase_trans(const DevGrid &G, const DevState &s, const double *T, const double *cpc, double *ptr, double *phr,
+                 double dt, hipStream_t st);
+void phase_trans(const DevGrid &G, const DevState &s, const float *T, const float *cpc, float *ptr, float *phr,
+                 double dt, hipStream_t st);
 /* snr: the new state as read before this substep writes it (at rk 0 the old
    state, which equals it by the step's entry condition) */
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
@@ -44,9 +45,9 @@ void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
                const float *td_wind, const StepConsts &K, int rk, hipStream_t st);
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
-                   double *sed_heat, const StepConsts &K, int rk, hipStream_t st);
+                   const double *cpc, double *sed_heat, const StepConsts &K, int rk, hipStream_t st);
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,
-                   float *sed_heat, const StepConsts &K, int rk, hipStream_t st);
+                   const float *cpc, float *sed_heat, const StepConsts &K, int rk, hipStream_t st);
 void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
                     hipStream_t st);
 int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st);

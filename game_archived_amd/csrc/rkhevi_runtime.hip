@@ -310,6 +310,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
         if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
         if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+        if ((rc = dzeros(&W.cpc, (size_t)(d.n_cond > 0 ? d.n_cond : 1)*S))) return rc;
         W.td_wind = M->s[GAME_STATE_TENDENCY].wind;
     } else {
         /* mixed: every work array FP32; the tendency state keeps FP64 densities
@@ -323,6 +324,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if ((rc = fzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
         if ((rc = fzeros(&W.pv, (size_t)d.curl))) return rc;
         if ((rc = fzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
+        if ((rc = fzeros(&W.cpc, (size_t)(d.n_cond > 0 ? d.n_cond : 1)*S))) return rc;
         if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
         if ((rc = dzeros(&W.flux2, 8))) return rc;
         if ((rc = dzeros(&W.flux_divv, S))) return rc;
@@ -449,8 +451,8 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
     const DevGrid &G = M->G;
     DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
     M->K.dt = dt;
-    col::temperature(G, so, W.temp, st);
-    if (G.moist) col::phase_trans(G, so, W.temp, W.ptr, W.phr, dt, st);
+    col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, st);
+    if (G.moist) col::phase_trans(G, so, W.temp, W.cpc, W.ptr, W.phr, dt, st);
     for (int rk = 0; rk < 2; ++rk) {
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
         DevState sns = snr;                                 /* ... with the horizontal wind just updated */
@@ -462,7 +464,7 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
         hor::scalar_tend(G, sns, td, W, rk, st);
         if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
         else col::ver_waves(G, so, snr, sn, td, (const float *)td_wind, M->K, rk, st);
-        if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.sed_heat, M->K, rk, st);
+        if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.cpc, W.sed_heat, M->K, rk, st);
     }
     HIP_OK(hipGetLastError());
     return 0;
@@ -528,8 +530,8 @@ int game_gpu_temperature_diagnostics(int k)
     if (!M) return GAME_ERR_NOT_INIT;
     DevState *s = slot(k);
     if (!s) return GAME_ERR_ARG;
-    if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, M->stream);
-    else col::temperature(M->G, *s, M->W.temp, M->stream);
+    if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, (float *)nullptr, M->stream);
+    else col::temperature(M->G, *s, M->W.temp, (double *)nullptr, M->stream);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -581,7 +583,7 @@ int game_gpu_integrals(int k, double *out)
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 1]))) return rc;
     out[G.n_const + 1] *= 0.5;
     if ((rc = integral(G.s, rg, G.phi, &out[G.n_const + 2]))) return rc;
-    col::temperature(G, *s, M->tmp_s, M->stream);
+    col::temperature(G, *s, M->tmp_s, (double *)nullptr, M->stream);
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 3]))) return rc;
     out[G.n_const + 3] *= GC_C_D_V;
     return 0;
