@@ -48,8 +48,6 @@ struct DevGrid {
     int split_cell;
     /* the same for the per-edge kernels */
     int split_edge;
-    /* 1: sound-wave solver streams each column through registers; 0: LDS-staged */
-    int vw_streaming;
 };
 
 struct DevState {

@@ -674,9 +674,9 @@ __device__ __forceinline__ double rho_int_old(const DevGrid &G, const DevState &
     return 0.5*(so.rho[gi + b] + so.rho[gi + b + G.nsh]);
 }
 
-template <int LT>
+template <int LT, typename TWIND>
 __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevState snr, DevState sn, DevState td,
-                                                   StepConsts K, int rk)
+                                                   const TWIND *td_wind, StepConsts K, int rk)
 {
     int i = blockIdx.x*128 + threadIdx.x;
     if (i >= G.nsh) return;
@@ -703,7 +703,7 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
                    + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
                    - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
                      *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
-        double r = -(w_old + dt*td.wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+        double r = -(w_old + dt*(double)td_wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
                    + tin*(ca.epe - cb.epe)/dt
                    + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
                    - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
@@ -773,128 +773,6 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
         tin_m = tin_j;
         rho_new_m = rho_n;
         rio_m = rio_j;
-    }
-}
-
-/* The same sound-wave solve in three phases over LDS: a block takes VW_NCOL
-   columns; (1) every thread builds rows (column, interface) of the system and
-   the new-state interface theta, (2) one lane per column eliminates, back
-   substitutes and damps, (3a) every thread writes the new scalars of one
-   (column, layer), (3b) then the new w of one (column, interface).  Every
-   value is formed with the same arithmetic as in the streaming kernel above;
-   no thread reads a state element another thread of phase 3 writes. */
-#define VW_NCOL 64
-#define VW_THREADS 256
-
-template <typename TWIND>
-__global__ void __launch_bounds__(VW_THREADS) k_ver_waves_lds(DevGrid G, DevState so, DevState snr, DevState sn,
-                                                              DevState td, const TWIND *td_wind, StepConsts K, int rk)
-{
-    extern __shared__ double lds[];
-    const int L = G.L, nsh = G.nsh, vpl = G.vpl;
-    const int i0 = xcd_block().x*VW_NCOL;
-    const i64 gi = (i64)G.n_cond*G.s;
-    const double dt = K.dt, iw = K.impl_weight;
-    const double dt2 = dt*dt;
-    const double *zs = G.zs, *exbg = G.exbg, *vol = G.vol, *area = G.area;
-    const int P = L*VW_NCOL;
-    double *cc = lds, *dd = lds + P, *ee = lds + 2*P, *rr = lds + 3*P, *tt = lds + 4*P;
-
-    /* phase 1: row j couples the mass fluxes through interfaces j-1, j, j+1
-       (column_solvers.c:141-189) */
-    for (int idx = threadIdx.x; idx < (L - 1)*VW_NCOL; idx += VW_THREADS) {
-        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
-        if (i >= nsh) continue;
-        const i64 b = i + (i64)j*nsh, lo = b + nsh, wi = i + (i64)(j + 1)*vpl;
-        const ColumnLayer ca = column_layer(G, so, snr, td, dt, rk, i, j);
-        const ColumnLayer cb = column_layer(G, so, snr, td, dt, rk, i, j + 1);
-        const double tin = theta_int_new(G, snr, i, j), rio = rho_int_old(G, so, i, j);
-        const double w_old = so.wind[wi];
-        const double rie = 0.5*(ca.re + cb.re);
-        const double d = -(tin*tin)*(ca.ga + cb.ga)
-                         + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
-                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
-                           *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
-        const double r = -(w_old + dt*(double)td_wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
-                         + tin*(ca.epe - cb.epe)/dt
-                         + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
-                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
-        double c = 0.0, e = 0.0;
-        if (j < L - 2) {
-            const i64 l2 = lo + nsh;
-            const double tin_n = theta_int_new(G, snr, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
-            c = tin_n*cb.ga*tin
-                + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
-                - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
-            e = tin*cb.ga*tin_n
-                - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
-                + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
-        }
-        cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; tt[idx] = tin;
-    }
-    __syncthreads();
-    /* phase 2: Thomas (column_solvers.c:641-667) and Klemp damping (:284-298) */
-    if (threadIdx.x < VW_NCOL && i0 + (int)threadIdx.x < nsh) {
-        const int col = threadIdx.x, i = i0 + col, n = L - 1;
-        double ep = 0.0, x = rr[col]/dd[col];
-        if (n > 1) ep = ee[col]/dd[col];
-        ee[col] = ep; rr[col] = x;
-        for (int j = 1; j < n; ++j) {
-            const int o = j*VW_NCOL + col;
-            const double cp = cc[o - VW_NCOL];
-            const double den = dd[o] - ep*cp;
-            if (j < n - 1) ep = ee[o]/den;
-            x = (rr[o] - x*cp)/den;
-            ee[o] = ep; rr[o] = x;
-        }
-        for (int j = n - 2; j >= 0; --j) {
-            const int o = j*VW_NCOL + col;
-            x = rr[o] - ee[o]*x;
-            rr[o] = x;
-        }
-        for (int j = 0; j < n; ++j) {
-            const int o = j*VW_NCOL + col;
-            rr[o] = rr[o]/(1.0 + dt*G.damp[(i64)j*nsh + i]);
-        }
-    }
-    __syncthreads();
-    /* phase 3a: rho, rhotheta_v, theta_v_pert, exner_pert of layer j
-       (column_solvers.c:300-345, 357-370); the new gas density goes to LDS */
-    for (int idx = threadIdx.x; idx < L*VW_NCOL; idx += VW_THREADS) {
-        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
-        if (i >= nsh) continue;
-        const i64 b = i + (i64)j*nsh;
-        const ColumnLayer cl = column_layer(G, so, snr, td, dt, rk, i, j);
-        const double v = vol[b];
-        double rho_n, rt_n;
-        if (j == 0) {
-            const double x = rr[idx];
-            rho_n = cl.re + dt*(x)/v;
-            rt_n = cl.rte + dt*(tt[idx]*x)/v;
-        } else if (j == L - 1) {
-            const double xm = rr[idx - VW_NCOL];
-            rho_n = cl.re + dt*(-xm)/v;
-            rt_n = cl.rte + dt*(-tt[idx - VW_NCOL]*xm)/v;
-        } else {
-            const double x = rr[idx], xm = rr[idx - VW_NCOL];
-            rho_n = cl.re + dt*(-xm + x)/v;
-            rt_n = cl.rte + dt*(-tt[idx - VW_NCOL]*xm + tt[idx]*x)/v;
-        }
-        const double rt_o = so.rt[b];
-        sn.rho[gi + b] = rho_n;
-        sn.rt[b] = rt_n;
-        sn.tp[b] = rt_n/rho_n - G.thbg[b];
-        sn.ep[b] = so.ep[b] + v*cl.ga*(rt_n - rt_o);
-        cc[idx] = rho_n;
-    }
-    __syncthreads();
-    /* phase 3b: w at the interior interfaces (column_solvers.c:346-355) */
-    for (int idx = threadIdx.x; idx < (L - 1)*VW_NCOL; idx += VW_THREADS) {
-        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
-        if (i >= nsh) continue;
-        const i64 wi = i + (i64)(j + 1)*vpl;
-        const double din = 0.5*(cc[idx] + cc[idx + VW_NCOL]);
-        sn.wind[wi] = (2.0*rr[idx]/area[wi] - din*so.wind[wi])/rho_int_old(G, so, i, j);
     }
 }
 
@@ -1392,36 +1270,22 @@ void phase_trans(const DevGrid &G, const DevState &s, const float *T, const floa
 { LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, ptr, phr, dt); }
 
 template <typename TWIND>
-static void ver_waves_lds(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,
-                          const DevState &td, const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
+static void ver_waves_t(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,
+                        const DevState &td, const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
 {
-    dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
-    size_t lds = (size_t)5*G.L*VW_NCOL*sizeof(double);   /* 65 KiB at 26 layers, 160 KiB at 64 */
-    static size_t lds_allowed = 65536;
-    if (lds > lds_allowed) {
-        (void)hipFuncSetAttribute((const void *)k_ver_waves_lds<TWIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        lds_allowed = lds;
-    }
-    LAUNCH_SHM("k_ver_waves", k_ver_waves_lds<TWIND>, g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
+    dim3 g((G.nsh + 127)/128), b(128);
+    if (G.L == 26) LAUNCH("k_ver_waves", (k_ver_waves<26, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+    else if (G.L == 55) LAUNCH("k_ver_waves", (k_ver_waves<55, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+    else LAUNCH("k_ver_waves", (k_ver_waves<0, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
 }
 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
                const StepConsts &K, int rk, hipStream_t st)
-{
-    if (!G.vw_streaming) {
-        ver_waves_lds<double>(G, so, snr, sn, td, td.wind, K, rk, st);
-        return;
-    }
-    dim3 g((G.nsh + 127)/128), b(128);
-    if (G.L == 26) LAUNCH("k_ver_waves", k_ver_waves<26>, g, b, st, G, so, snr, sn, td, K, rk);
-    else if (G.L == 55) LAUNCH("k_ver_waves", k_ver_waves<55>, g, b, st, G, so, snr, sn, td, K, rk);
-    else LAUNCH("k_ver_waves", k_ver_waves<0>, g, b, st, G, so, snr, sn, td, K, rk);
-}
+{ ver_waves_t<double>(G, so, snr, sn, td, td.wind, K, rk, st); }
 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
                const float *td_wind, const StepConsts &K, int rk, hipStream_t st)
-{ ver_waves_lds<float>(G, so, snr, sn, td, td_wind, K, rk, st); }
+{ ver_waves_t<float>(G, so, snr, sn, td, td_wind, K, rk, st); }
 
 static void gen_densities_only(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
                                const StepConsts &K, hipStream_t st)

@@ -230,9 +230,11 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     G.flat = 1;
     for (size_t k = 0; k < (size_t)d.v && G.flat; ++k) if (g->slope[k] != 0.0) G.flat = 0;
     if (getenv("GAME_GPU_NO_FLAT_SHORTCUT")) G.flat = 0;
-    /* per-cell kernels run 163842 columns at res 7: splitting each column into
-       layer chunks multiplies the waves in flight (results do not depend on it) */
-    G.split_cell = 4;
+    /* per-cell kernels: one thread per (cell, layer) by default -- 163842
+       columns alone are only 2.5 waves per SIMD at res 7; a split into fewer
+       layer chunks keeps the cell's tables in registers for several layers
+       (results do not depend on it) */
+    G.split_cell = d.n_layers;
     if (const char *e = getenv("GAME_GPU_SPLIT_CELL")) {
         int v = atoi(e);
         if (v >= 1 && v <= d.n_layers) G.split_cell = v;
@@ -242,7 +244,6 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         int v = atoi(e);
         if (v >= 1 && v <= d.n_layers) G.split_edge = v;
     }
-    G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
     size_t S = (size_t)d.s, V = (size_t)d.v, NVH = (size_t)d.nvh, NSH = (size_t)d.nsh, NT = (size_t)d.nt;
 #define UP(dst, src, n) do { if ((rc = upload(&(dst), (src), (n)))) return rc; } while (0)
     UP(G.nd, g->normal_distance, V); UP(G.vol, g->volume, S); UP(G.area, g->area, V); UP(G.zs, g->z_scalar, S);

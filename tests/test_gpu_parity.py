@@ -138,15 +138,3 @@ def test_general_terrain_path_on_flat_grid(monkeypatch):
         e = rel_err(getattr(dev, f), getattr(om.new, f))
         assert e <= 1e-12, (f, e)
 
-
-def test_streaming_sound_wave_solver_matches_oracle(monkeypatch):
-    """The register-streaming variant of the sound-wave solver (selected by
-    GAME_GPU_VW_STREAMING) gives the oracle's bits as the LDS-staged one does."""
-    monkeypatch.setenv("GAME_GPU_VW_STREAMING", "1")
-    print("case: res 4 dry gaussian, streaming solver, 3 steps", flush=True)
-    g = get_grid(4, "gaussian")
-    s0 = oracle_state(g)
-    om, dev, forc = run_pair(g, s0, 3, 300.0)
-    for f in FIELDS:
-        e = rel_err(getattr(dev, f), getattr(om.new, f))
-        assert e <= 1e-12, (f, e)
