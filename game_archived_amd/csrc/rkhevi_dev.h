@@ -48,6 +48,8 @@ struct DevGrid {
     int split_cell;
     /* the same for the per-edge kernels */
     int split_edge;
+    /* 1: tracer solver with one (column, tracer) per lane in registers; 0: LDS-staged */
+    int gd_streaming;
 };
 
 struct DevState {

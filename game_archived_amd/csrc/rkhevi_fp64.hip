@@ -899,6 +899,80 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     }
 }
 
+/* The same tracer solve with one (column, tracer) per lane, the rows
+   generated and eliminated in one downward sweep through registers (the
+   layout k_ver_waves uses); selected with GAME_GPU_GD_STREAMING for
+   measurement against the LDS-staged kernel above. */
+template <int LT>
+__global__ void __launch_bounds__(128) k_gen_densities_stream(DevGrid G, DevState so, DevState sn, DevState td,
+                                                              StepConsts K)
+{
+    const unsigned ntr = (unsigned)G.n_const - 1, blk = xcd_block().x;
+    const int tr = (int)(blk%ntr);
+    const int ncc = G.n_cond;
+    const int k = tr < ncc ? tr : tr + 1;
+    const int i = (int)(blk/ntr)*128 + threadIdx.x;
+    if (i >= G.nsh) return;
+    const int L = LT > 0 ? LT : G.L;
+    const int nsh = G.nsh;
+    const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
+    const double *vol = G.vol, *area = G.area;
+    const double *rho_o = so.rho + (i64)k*G.s, *trho = td.rho + (i64)k*G.s;
+    double vsink = 0.0;
+    if (k < ncc/4) vsink = K.snow_v;
+    else if (k < ncc/2) vsink = K.rain_v;
+    else if (k < ncc) vsink = K.cloud_v;
+    double ep[LT > 0 ? LT : GAME_MAX_LAYERS], x[LT > 0 ? LT : GAME_MAX_LAYERS];
+    double fi_m = 0.0, fr_m = 0.0, c_prev = 0.0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const i64 b = i + (i64)j*nsh;
+        double fi = 0.0, fr = 0.0;
+        if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
+        double d;
+        if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - iw*dt/vol[b]*fi;
+        else if (j == L - 1) {
+            d = (fi_m >= 0.0) ? 1.0 + iw*dt/vol[b]*fi_m : 1.0;
+            if (k < ncc) d += iw*vsink*dt*area[i + G.v - nsh]/vol[b];
+        } else {
+            d = 1.0;
+            if (fi_m >= 0.0) d += iw*dt/vol[b]*fi_m;
+            if (fi < 0.0) d -= iw*dt/vol[b]*fi;
+        }
+        double r = rho_o[b] + dt*trho[b];
+        if (j == 0) r += ew*dt*fr/vol[b];
+        else if (j == L - 1) {
+            r += -ew*dt*fr_m/vol[b];
+            if (k < ncc) r += -ew*vsink*dt*rho_o[i + G.s - nsh]*area[i + G.v - nsh]/vol[b];
+        } else r += ew*dt*(-fr_m + fr)/vol[b];
+        double c = 0.0, e = 0.0;
+        if (j < L - 1) {
+            if (fi >= 0.0) e = -iw*dt/vol[b]*fi;
+            else c = iw*dt/vol[b + nsh]*fi;
+        }
+        if (j == 0) {
+            ep[0] = e/d;
+            x[0] = r/d;
+        } else {
+            const double den = d - ep[j - 1]*c_prev;
+            if (j < L - 1) ep[j] = e/den;
+            x[j] = (r - x[j - 1]*c_prev)/den;
+        }
+        c_prev = c;
+        fi_m = fi;
+        fr_m = fr;
+    }
+#pragma unroll
+    for (int j = L - 2; j >= 0; --j) x[j] = x[j] - ep[j]*x[j + 1];
+    double *rho_n = sn.rho + (i64)k*G.s;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        double v = x[j];
+        if (v < 0.0) v = 0.0;
+        rho_n[i + (i64)j*nsh] = v;
+    }
+}
+
 /* sediment heat of the condensates at rk 0 (column_solvers.c:414-465 and the
    sh updates of :500-611): one thread per grid box adds the contributions of
    the condensates in the reference's constituent order, starting from 0 */
@@ -1290,6 +1364,13 @@ void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const 
 static void gen_densities_only(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td,
                                const StepConsts &K, hipStream_t st)
 {
+    if (G.gd_streaming) {
+        dim3 g(((G.nsh + 127)/128)*(G.n_const - 1)), b(128);
+        if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities_stream<26>, g, b, st, G, so, sn, td, K);
+        else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities_stream<55>, g, b, st, G, so, sn, td, K);
+        else LAUNCH("k_gen_densities", k_gen_densities_stream<0>, g, b, st, G, so, sn, td, K);
+        return;
+    }
     dim3 g(((G.nsh + GD_NCOL - 1)/GD_NCOL)*(G.n_const - 1)), b(GD_THREADS);
     size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
     static size_t lds_allowed = 65536;
