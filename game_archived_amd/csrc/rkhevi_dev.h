@@ -50,6 +50,8 @@ struct DevGrid {
     int split_edge;
     /* 1: tracer solver with one (column, tracer) per lane in registers; 0: LDS-staged */
     int gd_streaming;
+    /* 1: sound-wave solver with one column per lane; 0: three phases over LDS */
+    int vw_streaming;
 };
 
 struct DevState {
@@ -87,6 +89,21 @@ __device__ __forceinline__ double density_total(const DevGrid &G, const double *
 {   /* derived_quantities.c:83-95: left-to-right sum from 0 */
     double r = 0.0;
     for (int k = 0; k < G.n_const; ++k) r += rho[k*G.s + i];
+    return r;
+}
+
+/* the same sum with the constituent count known at compile time (NC > 0):
+   all loads issue before the first add */
+template <int NC>
+__device__ __forceinline__ double density_total_n(const DevGrid &G, const double *rho, i64 i)
+{
+    if (NC <= 0) return density_total(G, rho, i);
+    double v[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v[k] = rho[k*G.s + i];
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) r += v[k];
     return r;
 }
 

@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *
 
 /* manage_pressure_gradient.c:17-87 at the interior w points (layers 1..L-1),
    fused with calc_pressure_grad_condensates_v */
-template <typename TW>
+template <typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad)
 {
     i64 n = (i64)(G.L - 1)*G.nsh;
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT
     int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
     i64 w = h + (i64)lev*G.vpl, up = h + (i64)(lev - 1)*G.nsh, lo = h + (i64)lev*G.nsh;
     i64 gi = (i64)G.n_cond*G.s;
-    double rt_up = density_total(G, s.rho, up), rt_lo = density_total(G, s.rho, lo);
+    double rt_up = density_total_n<NC>(G, s.rho, up), rt_lo = density_total_n<NC>(G, s.rho, lo);
     double d_up = s.rho[gi + up]/rt_up, d_lo = s.rho[gi + lo]/rt_lo;
     if (with_pgrad) {
         if (!first_step) W.pgrad_old[w] = (TW)(-(double)W.pgrad_nl[w] - (double)W.pgrad_l[w]);
@@ -138,7 +138,7 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int
     return g;
 }
 
-template <bool FLAT, typename TW>
+template <bool FLAT, typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step)
 {
     const Blk bk = xcd_block();
@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
         double nl = (0.5*(p1_t + p1_f))*g;
         double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
         double ll = (0.5*(p2_t + p2_f))*G.exbg_grad[vi];
-        double d_t = s.rho[gi + st]/density_total(G, s.rho, st), d_f = s.rho[gi + sf]/density_total(G, s.rho, sf);
+        double d_t = s.rho[gi + st]/density_total_n<NC>(G, s.rho, st),
+               d_f = s.rho[gi + sf]/density_total_n<NC>(G, s.rho, sf);
         nl = (0.5*(d_t + d_f))*nl;
         ll = (0.5*(d_t + d_f))*ll;
         W.pgrad_nl[vi] = (TW)nl;
@@ -353,22 +354,35 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
     if (h >= G.nsh) return;
     const int nsh = G.nsh, nvh = G.nvh;
     const int ne = h < GAME_N_PENT ? 5 : 6;
+    /* a fixed six edges: a pentagon's sixth slot reads its first edge and
+       adds -0.0, the exact identity of IEEE addition */
     int adj[6];
-    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + (j < ne ? j : 0)];
     int l0, l1;
     layer_chunk(G.L + 1, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         double vw = 0.5;
         if (l == 0 || l == G.L) vw = 1;
         double o = 0;
-        if (l >= 1)
-            for (int j = 0; j < ne; ++j)
-                o += vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
-                     *(double)pv[(i64)l*2*nvh + adj[j]];
-        if (l <= G.L - 1)
-            for (int j = 0; j < ne; ++j)
-                o += vw*G.ipw[8*((i64)l*nsh + h) + j]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
-                     *(double)pv[(i64)l*2*nvh + adj[j]];
+        if (l >= 1) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                double t = vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
+                           *(double)pv[(i64)l*2*nvh + adj[j]];
+                if (j >= ne) t = -0.0;
+                o += t;
+            }
+        }
+        if (l <= G.L - 1) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                double t = vw*G.ipw[8*((i64)l*nsh + h) + j]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
+                           *(double)pv[(i64)l*2*nvh + adj[j]];
+                if (j >= ne) t = -0.0;
+                o += t;
+            }
+        }
         out[(i64)l*G.vpl + h] = (TW)o;
     }
 }
@@ -381,17 +395,23 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
     int h = bk.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
     const int ne = h < GAME_N_PENT ? 5 : 6;
+    /* a fixed six edges: a pentagon's sixth slot reads its first edge and
+       adds -0.0, the exact identity of IEEE addition */
     int adj[6];
-    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + (j < ne ? j : 0)];
     int l0, l1;
     layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 i = (i64)l*G.nsh + h;
         const double *w = G.ipw + 8*i;
         double o = 0;
-        for (int j = 0; j < ne; ++j) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
             i64 vi = G.nsh + (i64)l*G.vpl + adj[j];
-            o += w[j]*a[vi]*b[vi];
+            double t = w[j]*a[vi]*b[vi];
+            if (j >= ne) t = -0.0;
+            o += t;
         }
         o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
         o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
@@ -494,35 +514,44 @@ __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
    their divergences (divergences.c:14-148) and the RK-weighted tendencies.
    One thread per cell walks all layers; the edge fluxes are formed on the fly
    from the reference's own formulas, so no flux array is written. */
-template <bool FLAT, typename TW>
+template <bool FLAT, typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevWorkT<TW> W, double *td_rho,
                                                      double *td_rt, int rk)
 {
     const Blk bk = xcd_block();
     int h = bk.x*TPB + threadIdx.x;
     if (h >= G.nsh) return;
-    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro, ncc = G.n_cond, nc = G.n_const;
+    /* NC: the number of constituents as a compile-time constant (1 dry, 6 moist),
+       so the constituent loop unrolls and all neighbour loads can be in flight together */
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro, ncc = NC > 0 ? (NC > 1 ? NC - 2 : 0) : G.n_cond;
+    const int nc = NC > 0 ? NC : G.n_const;
     const i64 S = G.s, nvcc = (i64)no*nsh;
     const int ne = h < GAME_N_PENT ? 5 : 6;
     /* each edge as (edge, the cell across it, whether h is its from-cell);
        the centred fluxes are symmetric in the two cells and IEEE addition
        commutes, so they come out bit for bit as the reference forms them
-       from from/to */
+       from from/to.  The neighbour loops run over a fixed six: a pentagon's
+       sixth slot reads valid entries (its first edge, its own cell) and adds
+       -0.0, the exact identity of IEEE addition, so no branch guards a load
+       and the per-edge arrays stay in registers. */
     int ej[6], nb[6];
     unsigned from_bits = 0;
+#pragma unroll
     for (int j = 0; j < 6; ++j) {
-        const int e = G.adj[6*h + j], f = G.from[e], t = G.to[e];
+        const int jj = j < ne ? j : 0;
+        const int e = G.adj[6*h + jj], f = G.from[e], t = G.to[e];
         ej[j] = e;
-        nb[j] = (f == h) ? t : f;
-        if (G.sgn[6*h + j] > 0) from_bits |= 1u << j;
+        nb[j] = j < ne ? ((f == h) ? t : f) : h;
+        if (j < ne && G.sgn[6*h + jj] > 0) from_bits |= 1u << j;
     }
     int l0, l1;
     layer_chunk(L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         const i64 i = (i64)l*nsh + h;
         double a[6], u[6];
-        for (int j = 0; j < ne; ++j) {
-            i64 vi = nsh + (i64)l*vpl + ej[j];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const i64 vi = nsh + (i64)l*vpl + ej[j];
             a[j] = G.area[vi];
             u[j] = s.wind[vi];
         }
@@ -537,22 +566,32 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
         const double area_u = has_u ? G.area[h + (i64)l*vpl] : 0.0;
         const double area_l = has_l ? G.area[h + (i64)(l + 1)*vpl] : 0.0;
         const i64 vu = (i64)(l - (L - no))*nsh + h, vl = (i64)(l + 1 - (L - no))*nsh + h;
+        /* with NC known the new tendencies are kept in registers and stored
+           after every load of the layer, so no store orders the loads */
+        double out_rho[NC > 0 ? NC : 1];
+        double out_rt = 0.0;
+#pragma unroll
         for (int k = 0; k < nc; ++k) {
             const i64 shift = (i64)k*S;
             const double new_w = (rk == 1 && k != ncc) ? 0.5 : 1.0, old_w = 1.0 - new_w;
             const double *rk_ = s.rho + shift;
             const double rho_h = rk_[i];
+            double res;
             if (k == ncc) {
                 const double th_h = s.rt[i]/rho_h;
                 double c1 = 0.0, c2 = 0.0;
-                for (int j = 0; j < ne; ++j) {
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
                     const i64 sb = (i64)l*nsh + nb[j];
                     const double rho_nb = rk_[sb];
-                    const double sg = ((from_bits >> j) & 1u) ? 1.0 : -1.0;
+                    /* times the edge sign +-1: exact, so a sign flip */
+                    const bool pos = (from_bits >> j) & 1u;
                     double f1 = (0.5*(rho_h + rho_nb))*u[j];
                     double f2 = (0.5*(th_h + s.rt[sb]/rho_nb))*f1;
-                    c1 += f1*sg*a[j];
-                    c2 += f2*sg*a[j];
+                    double t1 = (pos ? f1 : -f1)*a[j], t2 = (pos ? f2 : -f2)*a[j];
+                    if (j >= ne) { t1 = -0.0; t2 = -0.0; }
+                    c1 += t1;
+                    c2 += t2;
                 }
                 double v1 = 0.0, v2 = 0.0;
                 if (!FLAT) {
@@ -572,29 +611,33 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 const double rg = rho_h;
                 if (!G.moist) {
                     /* dry: every source term of the reference sum is a zero array */
-                    td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1);
-                    td_rt[i] = old_w*td_rt[i] + new_w*(-div2);
+                    res = old_w*td_rho[shift + i] + new_w*(-div1);
+                    out_rt = old_w*td_rt[i] + new_w*(-div2);
                 } else {
                     /* the moist-air gas takes the vapour phase-transition rate
                        (scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
                     double ptr_v = (double)W.ptr[shift + i];
                     const double th = th_h;
-                    td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
+                    res = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
                     double cv = c_v_mass_weighted_cp(G, s.rho, W.cpc, i);
                     double heat = (double)W.phr[i] + (double)W.sed_heat[i];
-                    td_rt[i] = old_w*td_rt[i] + new_w*(
+                    out_rt = old_w*td_rt[i] + new_w*(
                         -div2
                         + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[i] + s.ep[i]))
                         + ptr_v*th);
                 }
             } else {
                 double c = 0.0;
-                for (int j = 0; j < ne; ++j) {
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
                     /* upstream density: the from-cell for u >= 0 */
                     const bool is_from = (from_bits >> j) & 1u;
                     const double rho_nb = rk_[(i64)l*nsh + nb[j]];
                     const double sv = ((u[j] >= 0.0) == is_from) ? rho_h : rho_nb;
-                    c += (sv*u[j])*(is_from ? 1.0 : -1.0)*a[j];
+                    const double f = sv*u[j];
+                    double t = (is_from ? f : -f)*a[j];
+                    if (j >= ne) t = -0.0;
+                    c += t;
                 }
                 double v = 0.0;
                 if (!FLAT) {
@@ -615,9 +658,16 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 }
                 double div = 1.0/vol*(c + v);
                 i64 shift_pt = (k == ncc + 1) ? shift - S : shift;
-                td_rho[shift + i] = old_w*td_rho[shift + i] + new_w*(-div + (double)W.ptr[shift_pt + i]);
+                res = old_w*td_rho[shift + i] + new_w*(-div + (double)W.ptr[shift_pt + i]);
             }
+            if (NC > 0) out_rho[NC > 0 ? k : 0] = res;
+            else td_rho[shift + i] = res;
         }
+        if (NC > 0) {
+#pragma unroll
+            for (int k = 0; k < (NC > 0 ? NC : 1); ++k) td_rho[(i64)k*S + i] = out_rho[k];
+        }
+        td_rt[i] = out_rt;
     }
 }
 
@@ -776,6 +826,169 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
     }
 }
 
+/* The same solve in three phases over a block of VW_NCOL columns, for
+   parallelism: the streaming kernel above gives each column one lane for
+   all of its work, 2.5 waves per SIMD at res 7, and its recurrence arrays
+   live in scratch.  Here
+     1. every thread builds rows (column, interface) of the tridiagonal
+        system into LDS, each from the two layers around the interface;
+     2. one lane per column runs the Thomas elimination and back
+        substitution on LDS, then all threads apply the damping;
+     3. every thread forms the new state of (column, layer) boxes into
+        registers; after a barrier (the reads of state snr may be of the
+        state being written) they are stored.
+   Every quantity is formed with the arithmetic of the streaming kernel, so
+   the two kernels agree bit for bit. */
+#define VW_NCOL 32
+#define VW_THREADS 256
+
+template <int LT, typename TWIND>
+__global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState so, DevState snr, DevState sn,
+                                                             DevState td, const TWIND *td_wind, StepConsts K, int rk)
+{
+    extern __shared__ double lds[];
+    const int L = LT > 0 ? LT : G.L, NI = L - 1;
+    const int nsh = G.nsh, vpl = G.vpl;
+    const i64 gi = (i64)G.n_cond*G.s;
+    const double dt = K.dt, iw = K.impl_weight;
+    const double dt2 = dt*dt;
+    const double *zs = G.zs, *exbg = G.exbg, *vol = G.vol, *area = G.area;
+    const int i0 = xcd_block().x*VW_NCOL;
+    double *cc = lds, *dd = lds + NI*VW_NCOL, *ee = lds + 2*NI*VW_NCOL, *rr = lds + 3*NI*VW_NCOL;
+
+    /* phase 1: row j couples interfaces j-1, j, j+1 (column_solvers.c:141-189) */
+    constexpr int MAXIT1 = ((LT > 0 ? LT : GAME_MAX_LAYERS) - 1)*VW_NCOL/VW_THREADS + 1;
+#pragma unroll
+    for (int it = 0; it < MAXIT1; ++it) {
+        const int idx = threadIdx.x + it*VW_THREADS;
+        if (idx >= NI*VW_NCOL) break;
+        const int j = idx/VW_NCOL, i = i0 + (idx - j*VW_NCOL);
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh, lo = b + nsh;
+        const i64 wi = i + (i64)(j + 1)*vpl;
+        const ColumnLayer ca = column_layer(G, so, snr, td, dt, rk, i, j);
+        const ColumnLayer cb = column_layer(G, so, snr, td, dt, rk, i, j + 1);
+        const double tin = theta_int_new(G, snr, i, j), rio = rho_int_old(G, so, i, j);
+        const double w_old = so.wind[wi];
+        const double rie = 0.5*(ca.re + cb.re);
+        const double d = -(tin*tin)*(ca.ga + cb.ga)
+                         + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
+                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
+                           *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
+        const double r = -(w_old + dt*(double)td_wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+                         + tin*(ca.epe - cb.epe)/dt
+                         + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
+                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
+        double e = 0.0, c = 0.0;
+        if (j < L - 2) {
+            const i64 l2 = lo + nsh;
+            const double tin_n = theta_int_new(G, snr, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
+            c = tin_n*cb.ga*tin
+                + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
+                - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
+            e = tin*cb.ga*tin_n
+                - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
+                + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
+        }
+        cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
+    }
+    __syncthreads();
+    /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
+       the multipliers overwrite e, the solution overwrites r */
+    if (threadIdx.x < VW_NCOL && i0 + (int)threadIdx.x < nsh) {
+        const int col = threadIdx.x;
+        double ep = 0.0, x;
+        if (NI > 1) ep = ee[col]/dd[col];
+        x = rr[col]/dd[col];
+        ee[col] = ep; rr[col] = x;
+        for (int j = 1; j < NI; ++j) {
+            const int o = j*VW_NCOL + col;
+            const double cp = cc[o - VW_NCOL];
+            const double den = dd[o] - ep*cp;
+            if (j < NI - 1) ep = ee[o]/den;
+            x = (rr[o] - x*cp)/den;
+            ee[o] = ep; rr[o] = x;
+        }
+        for (int j = NI - 2; j >= 0; --j) {
+            const int o = j*VW_NCOL + col;
+            x = rr[o] - ee[o]*x;
+            rr[o] = x;
+        }
+    }
+    __syncthreads();
+    /* Klemp (2008) damping of the solved fluxes (column_solvers.c:284-298) */
+#pragma unroll
+    for (int it = 0; it < MAXIT1; ++it) {
+        const int idx = threadIdx.x + it*VW_THREADS;
+        if (idx >= NI*VW_NCOL) break;
+        const int j = idx/VW_NCOL, i = i0 + (idx - j*VW_NCOL);
+        if (i >= nsh) continue;
+        rr[idx] = rr[idx]/(1.0 + dt*G.damp[(i64)j*nsh + i]);
+    }
+    __syncthreads();
+    /* phase 3: new state of box (column, layer j) (column_solvers.c:300-370) */
+    constexpr int MAXIT3 = (LT > 0 ? LT : GAME_MAX_LAYERS)*VW_NCOL/VW_THREADS + 1;
+    double o_rho[MAXIT3], o_rt[MAXIT3], o_tp[MAXIT3], o_ep[MAXIT3], o_w[MAXIT3];
+#pragma unroll
+    for (int it = 0; it < MAXIT3; ++it) {
+        o_rho[it] = 0.0; o_rt[it] = 0.0; o_tp[it] = 0.0; o_ep[it] = 0.0; o_w[it] = 0.0;
+        const int idx = threadIdx.x + it*VW_THREADS;
+        if (idx >= L*VW_NCOL) break;
+        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh;
+        const ColumnLayer cl = column_layer(G, so, snr, td, dt, rk, i, j);
+        const double tin_j = (j < L - 1) ? theta_int_new(G, snr, i, j) : 0.0;
+        const double tin_m = (j >= 1) ? theta_int_new(G, snr, i, j - 1) : 0.0;
+        const double x_j = (j < L - 1) ? rr[j*VW_NCOL + col] : 0.0;
+        const double x_m = (j >= 1) ? rr[(j - 1)*VW_NCOL + col] : 0.0;
+        const double v = vol[b];
+        double rho_n, rt_n;
+        if (j == 0) {
+            rho_n = cl.re + dt*(x_j)/v;
+            rt_n = cl.rte + dt*(tin_j*x_j)/v;
+        } else if (j == L - 1) {
+            rho_n = cl.re + dt*(-x_m)/v;
+            rt_n = cl.rte + dt*(-tin_m*x_m)/v;
+        } else {
+            rho_n = cl.re + dt*(-x_m + x_j)/v;
+            rt_n = cl.rte + dt*(-tin_m*x_m + tin_j*x_j)/v;
+        }
+        const double rt_o = so.rt[b];
+        o_rho[it] = rho_n;
+        o_rt[it] = rt_n;
+        o_tp[it] = rt_n/rho_n - G.thbg[b];
+        o_ep[it] = so.ep[b] + v*cl.ga*(rt_n - rt_o);
+        if (j >= 1) {
+            /* the new density of the box above, with the arithmetic of its own thread */
+            const i64 bm = b - nsh;
+            const double re_m = so.rho[gi + bm] + dt*td.rho[gi + bm];
+            const double vm = vol[bm];
+            double rho_m;
+            if (j - 1 == 0) rho_m = re_m + dt*(x_m)/vm;
+            else rho_m = re_m + dt*(-rr[(j - 2)*VW_NCOL + col] + x_m)/vm;
+            const double rio_m = rho_int_old(G, so, i, j - 1);
+            const i64 wi = i + (i64)j*vpl;
+            const double din = 0.5*(rho_m + rho_n);
+            o_w[it] = (2.0*x_m/area[wi] - din*so.wind[wi])/rio_m;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MAXIT3; ++it) {
+        const int idx = threadIdx.x + it*VW_THREADS;
+        if (idx >= L*VW_NCOL) break;
+        const int j = idx/VW_NCOL, i = i0 + (idx - j*VW_NCOL);
+        if (i >= nsh) continue;
+        const i64 b = i + (i64)j*nsh;
+        sn.rho[gi + b] = o_rho[it];
+        sn.rt[b] = o_rt[it];
+        sn.tp[b] = o_tp[it];
+        sn.ep[b] = o_ep[it];
+        if (j >= 1) sn.wind[i + (i64)j*vpl] = o_w[it];
+    }
+}
+
 /* three_band_solver_gen_densities (column_solvers.c:385-639).
 
    The tracers of a column are independent of each other except through the
@@ -806,15 +1019,18 @@ __device__ __forceinline__ void tracer_iface_flux(const DevGrid &G, const DevSta
     fi = a*fi;
     fr = a*fr;
     i64 b = i + (i64)jj*nsh;
-    double doi = (fr >= 0.0) ? rho_o[b + nsh] : rho_o[b];
+    /* both candidates load before the sign is known: no load waits on another */
+    const double d_lo = rho_o[b + nsh], d_up = rho_o[b];
+    double doi = (fr >= 0.0) ? d_lo : d_up;
     fr = doi*fr;
 }
 
+template <int LT>
 __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevState so, DevState sn, DevState td,
                                                               StepConsts K)
 {
     extern __shared__ double lds[];
-    const int L = G.L, nsh = G.nsh, ncc = G.n_cond;
+    const int L = LT > 0 ? LT : G.L, nsh = G.nsh, ncc = G.n_cond;
     /* the tracer is the fast block index: the blocks of one group of columns
        run side by side on one XCD and share the wind, area and volume columns in L2 */
     const unsigned ntr = (unsigned)G.n_const - 1, blk = xcd_block().x;
@@ -831,10 +1047,15 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     else if (k < ncc) vsink = K.cloud_v;
     double *cc = lds, *dd = lds + L*GD_NCOL, *ee = lds + 2*L*GD_NCOL, *rr = lds + 3*L*GD_NCOL;
 
-    /* phase 1: rows (column_solvers.c:466-615) */
-    for (int idx = threadIdx.x; idx < L*GD_NCOL; idx += GD_THREADS) {
+    /* phase 1: rows (column_solvers.c:466-615); with the layer count known at
+       compile time the loop unrolls and the loads of all of a thread's rows
+       are in flight together */
+    const int n_it = (L*GD_NCOL + GD_THREADS - 1)/GD_THREADS;
+#pragma unroll
+    for (int it = 0; it < n_it; ++it) {
+        const int idx = threadIdx.x + it*GD_THREADS;
         const int j = idx/GD_NCOL, col = idx - j*GD_NCOL, i = i0 + col;
-        if (i >= nsh) continue;
+        if (idx >= L*GD_NCOL || i >= nsh) continue;
         const i64 b = i + (i64)j*nsh;
         double fi = 0.0, fr = 0.0, fi_m = 0.0, fr_m = 0.0;
         if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
@@ -976,16 +1197,18 @@ __global__ void __launch_bounds__(128) k_gen_densities_stream(DevGrid G, DevStat
 /* sediment heat of the condensates at rk 0 (column_solvers.c:414-465 and the
    sh updates of :500-611): one thread per grid box adds the contributions of
    the condensates in the reference's constituent order, starting from 0 */
-template <typename TW>
+template <typename TW, int NCC>
 __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevState sn, const TW *T, const TW *cpc,
                                                   TW *sed_heat, StepConsts K)
 {
     const i64 b = (i64)xcd_block().x*TPB + threadIdx.x;
     if (b >= G.s) return;
-    const int nsh = G.nsh, vpl = G.vpl, L = G.L, ncc = G.n_cond;
+    /* NCC > 0: the condensate count at compile time, so both loops unroll */
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, ncc = NCC > 0 ? NCC : G.n_cond;
     const int j = (int)(b/nsh), i = (int)(b - (i64)j*nsh);
     const double *vol = G.vol, *area = G.area;
     double sh = 0.0;
+#pragma unroll
     for (int k = 0; k < ncc; ++k) {
         const double *rho_o = so.rho + (i64)k*G.s;
         double vsink;
@@ -993,6 +1216,7 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
         else if (k < ncc/2) vsink = K.rain_v;
         else vsink = K.cloud_v;
         double fe = 0.0, fe_m = 0.0;
+#pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
             const int jj = pass == 0 ? j : j - 1;
             if (jj < 0 || jj > L - 2) continue;
@@ -1000,9 +1224,12 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
             fr -= vsink;
             fr = area[i + (i64)(jj + 1)*vpl]*fr;
             const i64 bb = i + (i64)jj*nsh;
-            double doi, toi, cpi;
-            if (fr >= 0.0) { doi = rho_o[bb + nsh]; toi = (double)T[bb + nsh]; cpi = (double)cpc[(i64)k*G.s + bb + nsh]; }
-            else { doi = rho_o[bb]; toi = (double)T[bb]; cpi = (double)cpc[(i64)k*G.s + bb]; }
+            /* the upstream box is chosen after both are loaded */
+            const bool lo = fr >= 0.0;
+            const double d0 = rho_o[bb + nsh], d1 = rho_o[bb];
+            const double t0 = (double)T[bb + nsh], t1 = (double)T[bb];
+            const double c0 = (double)cpc[(i64)k*G.s + bb + nsh], c1 = (double)cpc[(i64)k*G.s + bb];
+            const double doi = lo ? d0 : d1, toi = lo ? t0 : t1, cpi = lo ? c0 : c1;
             fr = doi*fr;
             const double f = cpi*toi*fr;
             if (pass == 0) fe = f; else fe_m = f;
@@ -1236,17 +1463,30 @@ namespace hor {
         if (G.flat) LAUNCH(NAME, (KERNEL<true, TW>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);    \
         else LAUNCH(NAME, (KERNEL<false, TW>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);          \
     } while (0)
+#define FLAT_DISPATCH_NC(NAME, KERNEL, NC, GRIDDIM, ...)                                          \
+    do {                                                                                      \
+        if (G.flat) LAUNCH(NAME, (KERNEL<true, TW, NC>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);\
+        else LAUNCH(NAME, (KERNEL<false, TW, NC>), GRIDDIM, dim3(TPB), st, __VA_ARGS__);      \
+    } while (0)
 
 template <typename TW>
 static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, hipStream_t st)
 {
-    LAUNCH("k_pgrad_w", k_pgrad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-    FLAT_DISPATCH("k_pgrad_e", k_pgrad_e, GRID2(G.nvh, G.split_edge), G, s, W, first);
+    if (G.n_const == 6) {
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 6>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 6, GRID2(G.nvh, G.split_edge), G, s, W, first);
+    } else if (G.n_const == 1) {
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 1>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 1, GRID2(G.nvh, G.split_edge), G, s, W, first);
+    } else {
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 0, GRID2(G.nvh, G.split_edge), G, s, W, first);
+    }
 }
 
 template <typename TW>
 static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, hipStream_t st)
-{ LAUNCH("k_pgrad_w", k_pgrad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
+{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
 
 template <typename TW>
 static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
@@ -1280,7 +1520,22 @@ static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &t
 {
     if (!G.flat)
         LAUNCH("k_vcc", k_vcc, GRID1((i64)G.n_oro*G.nsh), dim3(TPB), st, G, s, W.vcc);
-    FLAT_DISPATCH("k_scalar_tend", k_scalar_tend, GRID2(G.nsh, G.split_cell), G, s, W, td.rho, td.rt, rk);
+    if (G.n_const == 6) {
+        if (G.flat) LAUNCH("k_scalar_tend", (k_scalar_tend<true, TW, 6>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                           W, td.rho, td.rt, rk);
+        else LAUNCH("k_scalar_tend", (k_scalar_tend<false, TW, 6>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                    W, td.rho, td.rt, rk);
+    } else if (G.n_const == 1) {
+        if (G.flat) LAUNCH("k_scalar_tend", (k_scalar_tend<true, TW, 1>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                           W, td.rho, td.rt, rk);
+        else LAUNCH("k_scalar_tend", (k_scalar_tend<false, TW, 1>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                    W, td.rho, td.rt, rk);
+    } else {
+        if (G.flat) LAUNCH("k_scalar_tend", (k_scalar_tend<true, TW, 0>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                           W, td.rho, td.rt, rk);
+        else LAUNCH("k_scalar_tend", (k_scalar_tend<false, TW, 0>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
+                    W, td.rho, td.rt, rk);
+    }
 }
 
 void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
@@ -1347,10 +1602,21 @@ template <typename TWIND>
 static void ver_waves_t(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,
                         const DevState &td, const TWIND *td_wind, const StepConsts &K, int rk, hipStream_t st)
 {
-    dim3 g((G.nsh + 127)/128), b(128);
-    if (G.L == 26) LAUNCH("k_ver_waves", (k_ver_waves<26, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
-    else if (G.L == 55) LAUNCH("k_ver_waves", (k_ver_waves<55, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
-    else LAUNCH("k_ver_waves", (k_ver_waves<0, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+    if (G.vw_streaming) {
+        dim3 g((G.nsh + 127)/128), b(128);
+        if (G.L == 26) LAUNCH("k_ver_waves", (k_ver_waves<26, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+        else if (G.L == 55)
+            LAUNCH("k_ver_waves", (k_ver_waves<55, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+        else LAUNCH("k_ver_waves", (k_ver_waves<0, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+        return;
+    }
+    dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
+    size_t lds = (size_t)4*(G.L - 1)*VW_NCOL*sizeof(double);   /* 25 KiB at 26 layers, 63 KiB at 64 */
+    if (G.L == 26)
+        LAUNCH_SHM("k_ver_waves", (k_ver_waves_3p<26, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
+    else if (G.L == 55)
+        LAUNCH_SHM("k_ver_waves", (k_ver_waves_3p<55, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
+    else LAUNCH_SHM("k_ver_waves", (k_ver_waves_3p<0, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
 }
 
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,
@@ -1373,27 +1639,40 @@ static void gen_densities_only(const DevGrid &G, const DevState &so, const DevSt
     }
     dim3 g(((G.nsh + GD_NCOL - 1)/GD_NCOL)*(G.n_const - 1)), b(GD_THREADS);
     size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
-    static size_t lds_allowed = 65536;
-    if (lds > lds_allowed) {
-        (void)hipFuncSetAttribute((const void *)k_gen_densities, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_allowed = lds;
+    if (lds > 65536) {
+        static bool raised = false;
+        if (!raised) {
+            (void)hipFuncSetAttribute((const void *)k_gen_densities<55>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(4*GAME_MAX_LAYERS*GD_NCOL*sizeof(double)));
+            (void)hipFuncSetAttribute((const void *)k_gen_densities<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(4*GAME_MAX_LAYERS*GD_NCOL*sizeof(double)));
+            raised = true;
+        }
     }
-    LAUNCH_SHM("k_gen_densities", k_gen_densities, g, b, lds, st, G, so, sn, td, K);
+    if (G.L == 26) LAUNCH_SHM("k_gen_densities", k_gen_densities<26>, g, b, lds, st, G, so, sn, td, K);
+    else if (G.L == 55) LAUNCH_SHM("k_gen_densities", k_gen_densities<55>, g, b, lds, st, G, so, sn, td, K);
+    else LAUNCH_SHM("k_gen_densities", k_gen_densities<0>, g, b, lds, st, G, so, sn, td, K);
 }
 
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const double *T,
                    const double *cpc, double *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
-    if (rk == 0 && G.n_cond > 0)
-        LAUNCH("k_sed_heat", k_sed_heat<double>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+    if (rk == 0 && G.n_cond > 0) {
+        if (G.n_cond == 4)
+            LAUNCH("k_sed_heat", (k_sed_heat<double, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+        else LAUNCH("k_sed_heat", (k_sed_heat<double, 0>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+    }
     gen_densities_only(G, so, sn, td, K, st);
 }
 
 void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, const DevState &td, const float *T,
                    const float *cpc, float *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
-    if (rk == 0 && G.n_cond > 0)
-        LAUNCH("k_sed_heat", k_sed_heat<float>, GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+    if (rk == 0 && G.n_cond > 0) {
+        if (G.n_cond == 4)
+            LAUNCH("k_sed_heat", (k_sed_heat<float, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+        else LAUNCH("k_sed_heat", (k_sed_heat<float, 0>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
+    }
     gen_densities_only(G, so, sn, td, K, st);
 }
 

@@ -240,6 +240,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if (v >= 1 && v <= d.n_layers) G.split_cell = v;
     }
     G.gd_streaming = getenv("GAME_GPU_GD_STREAMING") ? 1 : 0;
+    G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
     G.split_edge = 2;
     if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
         int v = atoi(e);

@@ -138,3 +138,31 @@ def test_general_terrain_path_on_flat_grid(monkeypatch):
         e = rel_err(getattr(dev, f), getattr(om.new, f))
         assert e <= 1e-12, (f, e)
 
+
+
+@pytest.mark.parametrize("switch", ["GAME_GPU_VW_STREAMING", "GAME_GPU_GD_STREAMING"])
+def test_column_solver_variants(monkeypatch, switch):
+    """The one-column-per-lane variants of the sound-wave and tracer solvers
+    (kept for measurement) give the oracle's result like the LDS-staged ones."""
+    monkeypatch.setenv(switch, "1")
+    print(f"case: res 4 moist gaussian, {switch}, 2 steps", flush=True)
+    g = get_grid(4, "gaussian", moist=1)
+    s0 = seeded_condensates(g, oracle_state(g, moist=True))
+    om, dev, forc = run_pair(g, s0, 2, 300.0)
+    for f in FIELDS:
+        e = rel_err(getattr(dev, f), getattr(om.new, f))
+        assert e <= 1e-10, (f, e)
+
+
+@pytest.mark.parametrize("n_layers,n_oro", [(10, 7), (55, 40)])
+def test_other_layer_counts(n_layers, n_oro):
+    """Column kernels are instantiated for 26 and 55 layers and fall back to a
+    run-time layer count otherwise: both paths against the oracle."""
+    print(f"case: res 3 moist gaussian, {n_layers} layers, 3 steps", flush=True)
+    g = get_grid(3, "gaussian", moist=1, n_layers=n_layers, n_oro=n_oro)
+    s0 = seeded_condensates(g, oracle_state(g, moist=True))
+    om, dev, forc = run_pair(g, s0, 3, 300.0)
+    for f in FIELDS:
+        e = rel_err(getattr(dev, f), getattr(om.new, f))
+        print(f, e)
+        assert e <= 1e-10, (f, e)
