@@ -64,8 +64,8 @@ def kernel_bytes(name, dims, wb=8):
         # area at w, damping (3 W), the w tendency (work), writes 4 S + W
         "k_ver_waves": 8 * (14 * S + 3 * W_ + 4 * S + W_) + wb * W_,
         # every tracer of a column: rho_k old, tendency and new; w old and new,
-        # area at w, volume (shared)
-        "k_gen_densities": 8 * ((nc - 1) * 3 * S + 3 * W_ + S),
+        # area at w, volume and the (iw dt)/volume table (shared)
+        "k_gen_densities": 8 * ((nc - 1) * 3 * S + 3 * W_ + 2 * S),
         # condensate densities, w and area at two interfaces, volume; temperature,
         # the condensates' c_p table (work) and the output
         "k_sed_heat": 8 * (4 * S + 2 * W_ + S) + wb * (2 * S + 4 * S),
@@ -83,14 +83,17 @@ def kernel_bytes(name, dims, wb=8):
         "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (2 * S if moist else 0))
                          + (wb * (8 + 4) * S if moist else 0) + 4 * 12 * NSH,
         "k_vector_tend": wb * (7 * V + W_) + 8 * 2 * E,
-        "k_pgrad_e": 8 * (S * (3 + nc) + 2 * E) + wb * 5 * E + 4 * 2 * NVH,
-        "k_pgrad_w": 8 * (S * (3 + nc) + 3 * W_) + wb * 6 * W_,
+        # exner, theta_bg, theta pert and the gas fraction of the cells
+        "k_pgrad_e": 8 * (4 * S + 2 * E) + wb * 5 * E + 4 * 2 * NVH,
+        "k_pgrad_w": 8 * (4 * S + 3 * W_) + wb * 6 * W_,
         "k_grad_e": wb * (S + E) + 8 * E + 4 * 2 * NVH,
         "k_grad_w": wb * (S + W_) + 8 * W_,
         "k_inner_product": 8 * (V + 8 * S) + wb * S + 4 * 6 * NSH,
         "k_mass_flux": 8 * (S + E + W_) + wb * (E + W_) + 4 * 2 * NVH,
         "k_phase_trans": 8 * nc * S + wb * (7 + 4) * S,
-        "k_temperature": 8 * 6 * S + wb * (S + (4 * S if moist else 0)),
+        # theta, exner (background and perturbation), all densities; writes the
+        # temperature, the condensates' c_p (work) and the gas fraction
+        "k_temperature": 8 * ((4 + nc) * S + S) + wb * (S + (4 * S if moist else 0)),
         "k_lincomb_scalars": 8 * (2 * (nc + 2) * S + (nc + 3) * S),
         "k_lincomb_vector": 8 * 3 * V,
     }

@@ -35,6 +35,10 @@ struct DevGrid {
     i64 s, v, curl;
     double radius, toa;
     const double *nd, *vol, *area, *zs, *zv, *phi, *gm, *slope, *thbg, *exbg, *exbg_grad, *ipw;
+    /* the inner-product weights in layer-major, slot-major order, so that
+       lanes of consecutive cells read consecutive addresses:
+       ipwT[(l*8 + j)*nsh + h] = ipw[8*(l*nsh + h) + j] */
+    const double *ipwT;
     const double *tw, *drw, *dnd, *dzv, *darea, *fvec;
     const int *from, *to, *adj, *sgn, *ti, *tmc, *dri, *dfrom, *dto, *vit, *vst;
     /* per-edge differences that calc_pot_vort forms for its density
@@ -43,6 +47,10 @@ struct DevGrid {
     const double *pv_top_dz1, *pv_top_dz2, *pv_sfc_dz1, *pv_sfc_dz2;
     /* Klemp damping coefficient of every interior interface, (L-1)*NSH */
     const double *damp;
+    /* (0.5 dt)/vol of every grid box for the dt of the last step (the tracer
+       solver's implicit weight times dt over the volume); the runtime refills
+       it when dt changes */
+    double *idv;
     /* number of layer chunks the per-cell kernels split a column into
        (gridDim.y); more chunks give more waves to hide gather latency */
     int split_cell;
@@ -52,6 +60,8 @@ struct DevGrid {
     int gd_streaming;
     /* 1: sound-wave solver with one column per lane; 0: three phases over LDS */
     int vw_streaming;
+    /* 1: sediment heat with one thread per grid box; 0: one thread per column (26 layers) */
+    int sh_per_box;
 };
 
 struct DevState {
@@ -77,6 +87,9 @@ struct DevWorkT {
     /* c_p of every condensate at the step's temperature, n_cond x S: the
        temperature is fixed for the whole step, so each value is formed once */
     TW *cpc;
+    /* gas density over total density of the step's old state, S (formed with
+       the temperature; read by the pressure gradient at the first substep) */
+    double *gas_frac;
 };
 typedef DevWorkT<double> DevWork;
 typedef DevWorkT<float> DevWorkF;

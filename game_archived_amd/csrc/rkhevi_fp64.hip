@@ -73,10 +73,13 @@ __device__ __forceinline__ double remap_verpri2horpri(const DevGrid &G, const do
 
 /* derived_quantities.c:15-40 */
 template <typename TO>
-__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *T, TO *cpc)
+__global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *T, TO *cpc, double *gf)
 {
     i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
     if (i >= G.s) return;
+    /* the gas fraction of the box, as the pressure gradient forms it
+       (manage_pressure_gradient.c:17-87) */
+    if (gf) gf[i] = s.rho[(i64)G.n_cond*G.s + i]/density_total(G, s.rho, i);
     double t = (G.thbg[i] + s.tp[i])*(G.exbg[i] + s.ep[i]);
     if (G.moist)
         t = t/(1.0 + s.rho[(i64)(G.n_cond + 1)*G.s + i]/s.rho[(i64)G.n_cond*G.s + i]*(GC_M_D/GC_M_V - 1.0));
@@ -90,7 +93,8 @@ __global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *
 /* manage_pressure_gradient.c:17-87 at the interior w points (layers 1..L-1),
    fused with calc_pressure_grad_condensates_v */
 template <typename TW, int NC>
-__global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad)
+__global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad,
+                                                 const double *gf)
 {
     i64 n = (i64)(G.L - 1)*G.nsh;
     i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
@@ -98,8 +102,15 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT
     int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
     i64 w = h + (i64)lev*G.vpl, up = h + (i64)(lev - 1)*G.nsh, lo = h + (i64)lev*G.nsh;
     i64 gi = (i64)G.n_cond*G.s;
-    double rt_up = density_total_n<NC>(G, s.rho, up), rt_lo = density_total_n<NC>(G, s.rho, lo);
-    double d_up = s.rho[gi + up]/rt_up, d_lo = s.rho[gi + lo]/rt_lo;
+    /* gf: the gas fractions of state s, formed with this arithmetic by k_temperature */
+    double rt_up = 0.0, rt_lo = 0.0, d_up, d_lo;
+    if (gf) { d_up = gf[up]; d_lo = gf[lo]; }
+    else {
+        rt_up = density_total_n<NC>(G, s.rho, up);
+        rt_lo = density_total_n<NC>(G, s.rho, lo);
+        d_up = s.rho[gi + up]/rt_up;
+        d_lo = s.rho[gi + lo]/rt_lo;
+    }
     if (with_pgrad) {
         if (!first_step) W.pgrad_old[w] = (TW)(-(double)W.pgrad_nl[w] - (double)W.pgrad_l[w]);
         double raw = (s.ep[up] - s.ep[lo])/G.nd[w];
@@ -113,7 +124,7 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT
         W.pgrad_l[w] = (TW)ll;
         if (first_step) W.pgrad_old[w] = (TW)(-nl - ll);
     }
-    double c_up = s.rho[gi + up]/rt_up - 1.0, c_lo = s.rho[gi + lo]/rt_lo - 1.0;
+    double c_up = d_up - 1.0, c_lo = d_lo - 1.0;   /* s.rho[gi + up]/rt_up - 1.0 */
     W.pgc_v[w] = (TW)((0.5*(c_up + c_lo))*G.gm[w]);
 }
 
@@ -139,7 +150,8 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int
 }
 
 template <bool FLAT, typename TW, int NC>
-__global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step)
+__global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step,
+                                                 const double *gf)
 {
     const Blk bk = xcd_block();
     int e = bk.x*TPB + threadIdx.x;
@@ -157,8 +169,12 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
         double nl = (0.5*(p1_t + p1_f))*g;
         double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
         double ll = (0.5*(p2_t + p2_f))*G.exbg_grad[vi];
-        double d_t = s.rho[gi + st]/density_total_n<NC>(G, s.rho, st),
-               d_f = s.rho[gi + sf]/density_total_n<NC>(G, s.rho, sf);
+        double d_t, d_f;
+        if (gf) { d_t = gf[st]; d_f = gf[sf]; }
+        else {
+            d_t = s.rho[gi + st]/density_total_n<NC>(G, s.rho, st);
+            d_f = s.rho[gi + sf]/density_total_n<NC>(G, s.rho, sf);
+        }
         nl = (0.5*(d_t + d_f))*nl;
         ll = (0.5*(d_t + d_f))*ll;
         W.pgrad_nl[vi] = (TW)nl;
@@ -368,7 +384,7 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
         if (l >= 1) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                double t = vw*G.ipw[8*((i64)(l - 1)*nsh + h) + j]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
+                double t = vw*G.ipwT[((i64)(l - 1)*8 + j)*nsh + h]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
                            *(double)pv[(i64)l*2*nvh + adj[j]];
                 if (j >= ne) t = -0.0;
                 o += t;
@@ -377,7 +393,7 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
         if (l <= G.L - 1) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                double t = vw*G.ipw[8*((i64)l*nsh + h) + j]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
+                double t = vw*G.ipwT[((i64)l*8 + j)*nsh + h]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
                            *(double)pv[(i64)l*2*nvh + adj[j]];
                 if (j >= ne) t = -0.0;
                 o += t;
@@ -404,17 +420,17 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
     layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 i = (i64)l*G.nsh + h;
-        const double *w = G.ipw + 8*i;
+        const double *w = G.ipwT + (i64)l*8*G.nsh + h;   /* weight j at w[j*nsh] */
         double o = 0;
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             i64 vi = G.nsh + (i64)l*G.vpl + adj[j];
-            double t = w[j]*a[vi]*b[vi];
+            double t = w[j*(i64)G.nsh]*a[vi]*b[vi];
             if (j >= ne) t = -0.0;
             o += t;
         }
-        o += w[6]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
-        o += w[7]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
+        o += w[6*(i64)G.nsh]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
+        o += w[7*(i64)G.nsh]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
         out[i] = (TW)o;
     }
 }
@@ -1047,45 +1063,62 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     else if (k < ncc) vsink = K.cloud_v;
     double *cc = lds, *dd = lds + L*GD_NCOL, *ee = lds + 2*L*GD_NCOL, *rr = lds + 3*L*GD_NCOL;
 
-    /* phase 1: rows (column_solvers.c:466-615); with the layer count known at
-       compile time the loop unrolls and the loads of all of a thread's rows
-       are in flight together */
-    const int n_it = (L*GD_NCOL + GD_THREADS - 1)/GD_THREADS;
+    /* phase 1: rows (column_solvers.c:466-615).  Each wave takes a run of
+       layers of the block's columns (lane = column, so every load is
+       coalesced) and walks down it: the flux through the interface below
+       row j is the flux above row j+1, formed once.  (iw dt)/vol comes from
+       the per-box table the runtime keeps for the current dt: the same
+       quotient, formed once instead of once per tracer and row. */
+    {
+        constexpr int NW = GD_THREADS/GD_NCOL;
+        constexpr int PER_MAX = ((LT > 0 ? LT : GAME_MAX_LAYERS) + NW - 1)/NW;
+        const int per = (L + NW - 1)/NW;
+        const int w = threadIdx.x/GD_NCOL, col = threadIdx.x - w*GD_NCOL, i = i0 + col;
+        const int j0 = w*per, j1 = j0 + per < L ? j0 + per : L;
+        const double *idv = G.idv;
+        if (i < nsh && j0 < j1) {
+            double fi_m = 0.0, fr_m = 0.0;
+            if (j0 > 0) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j0 - 1, fi_m, fr_m);
 #pragma unroll
-    for (int it = 0; it < n_it; ++it) {
-        const int idx = threadIdx.x + it*GD_THREADS;
-        const int j = idx/GD_NCOL, col = idx - j*GD_NCOL, i = i0 + col;
-        if (idx >= L*GD_NCOL || i >= nsh) continue;
-        const i64 b = i + (i64)j*nsh;
-        double fi = 0.0, fr = 0.0, fi_m = 0.0, fr_m = 0.0;
-        if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
-        if (j > 0) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j - 1, fi_m, fr_m);
-        double d;
-        if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - iw*dt/vol[b]*fi;
-        else if (j == L - 1) {
-            d = (fi_m >= 0.0) ? 1.0 + iw*dt/vol[b]*fi_m : 1.0;
-            if (k < ncc) d += iw*vsink*dt*area[i + G.v - nsh]/vol[b];
-        } else {
-            d = 1.0;
-            if (fi_m >= 0.0) d += iw*dt/vol[b]*fi_m;
-            if (fi < 0.0) d -= iw*dt/vol[b]*fi;
-        }
-        double r = rho_o[b] + dt*trho[b];
-        if (j == 0) r += ew*dt*fr/vol[b];
-        else if (j == L - 1) {
-            r += -ew*dt*fr_m/vol[b];
-            if (k < ncc) {
-                const i64 low = i + G.s - nsh;
-                r += -ew*vsink*dt*rho_o[low]*area[i + G.v - nsh]/vol[b];
+            for (int t = 0; t < PER_MAX; ++t) {
+                const int j = j0 + t;
+                if (j >= j1) break;
+                const int idx = j*GD_NCOL + col;
+                const i64 b = i + (i64)j*nsh;
+                double fi = 0.0, fr = 0.0;
+                if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
+                const double q = idv[b];   /* iw*dt/vol[b] */
+                double d;
+                if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - q*fi;
+                else if (j == L - 1) {
+                    d = (fi_m >= 0.0) ? 1.0 + q*fi_m : 1.0;
+                    if (k < ncc) d += iw*vsink*dt*area[i + G.v - nsh]/vol[b];
+                } else {
+                    d = 1.0;
+                    if (fi_m >= 0.0) d += q*fi_m;
+                    if (fi < 0.0) d -= q*fi;
+                }
+                double r = rho_o[b] + dt*trho[b];
+                if (j == 0) r += ew*dt*fr/vol[b];
+                else if (j == L - 1) {
+                    r += -ew*dt*fr_m/vol[b];
+                    if (k < ncc) {
+                        const i64 low = i + G.s - nsh;
+                        r += -ew*vsink*dt*rho_o[low]*area[i + G.v - nsh]/vol[b];
+                    }
+                } else r += ew*dt*(-fr_m + fr)/vol[b];
+                /* off-diagonals of interface j (column_solvers.c:475-488);
+                   -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
+                double c = 0.0, e = 0.0;
+                if (j < L - 1) {
+                    if (fi >= 0.0) e = (-q)*fi;
+                    else c = idv[b + nsh]*fi;
+                }
+                cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
+                fi_m = fi;
+                fr_m = fr;
             }
-        } else r += ew*dt*(-fr_m + fr)/vol[b];
-        /* off-diagonals of interface j (column_solvers.c:475-488) */
-        double c = 0.0, e = 0.0;
-        if (j < L - 1) {
-            if (fi >= 0.0) e = -iw*dt/vol[b]*fi;
-            else c = iw*dt/vol[b + nsh]*fi;
         }
-        cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
     }
     __syncthreads();
     /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
@@ -1243,6 +1276,83 @@ __global__ void __launch_bounds__(TPB) k_sed_heat(DevGrid G, DevState so, DevSta
         } else sh += (-fe_m + fe)/vol[b];
     }
     sed_heat[b] = (TW)sh;
+}
+
+/* The same sums with one thread per column walking down its layers: the flux
+   through each interface is formed once (by the reference's arithmetic) and
+   carried to the box below, and the upstream box's density, temperature and
+   c_p are loaded once per box and kept for the next interface.  The helper's
+   restrict-qualified pointers let the loads of later layers move ahead of the
+   stores of earlier ones. */
+template <typename TW, int NCC, int LT>
+__device__ __forceinline__ void sed_heat_column(const DevGrid &G, const StepConsts &K, int i,
+                                                const double *__restrict__ rho_o, const double *__restrict__ wind_n,
+                                                const double *__restrict__ area, const double *__restrict__ vol,
+                                                const TW *__restrict__ T, const TW *__restrict__ cpc,
+                                                TW *__restrict__ sed_heat)
+{
+    const int nsh = G.nsh, vpl = G.vpl, L = LT;
+    const i64 S = G.s;
+    double vsink[NCC];
+#pragma unroll
+    for (int k = 0; k < NCC; ++k) vsink[k] = k < NCC/4 ? K.snow_v : (k < NCC/2 ? K.rain_v : K.cloud_v);
+    /* values of the box above the current interface */
+    double d_up[NCC], c_up[NCC], t_up = (double)T[i];
+#pragma unroll
+    for (int k = 0; k < NCC; ++k) { d_up[k] = rho_o[k*S + i]; c_up[k] = (double)cpc[k*S + i]; }
+    double fe_m[NCC];
+#pragma unroll
+    for (int k = 0; k < NCC; ++k) fe_m[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const i64 b = i + (i64)j*nsh;
+        const double v = vol[b];
+        double fe[NCC], d_lo[NCC], c_lo[NCC], t_lo = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCC; ++k) { fe[k] = 0.0; d_lo[k] = 0.0; c_lo[k] = 0.0; }
+        if (j <= L - 2) {
+            /* interface j, below box j (column_solvers.c:414-465) */
+            const double wn = wind_n[i + (i64)(j + 1)*vpl], a = area[i + (i64)(j + 1)*vpl];
+            t_lo = (double)T[b + nsh];
+#pragma unroll
+            for (int k = 0; k < NCC; ++k) {
+                d_lo[k] = rho_o[k*S + b + nsh];
+                c_lo[k] = (double)cpc[k*S + b + nsh];
+                double fr = wn;
+                fr -= vsink[k];
+                fr = a*fr;
+                const bool lo = fr >= 0.0;
+                const double doi = lo ? d_lo[k] : d_up[k], toi = lo ? t_lo : t_up, cpi = lo ? c_lo[k] : c_up[k];
+                fr = doi*fr;
+                fe[k] = cpi*toi*fr;
+            }
+        }
+        double sh = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCC; ++k) {
+            if (j == 0) sh += fe[k]/v;
+            else if (j == L - 1) {
+                const i64 sfc = i + G.v - nsh;
+                sh += -fe_m[k]/v;
+                /* the lowest box: t_up, d_up, c_up hold its values */
+                sh += -vsink[k]*t_up*c_up[k]*d_up[k]*area[sfc]/v;
+            } else sh += (-fe_m[k] + fe[k])/v;
+            fe_m[k] = fe[k];
+            d_up[k] = d_lo[k];
+            c_up[k] = c_lo[k];
+        }
+        t_up = t_lo;
+        sed_heat[b] = (TW)sh;
+    }
+}
+
+template <typename TW, int NCC, int LT>
+__global__ void __launch_bounds__(TPB) k_sed_heat_col(DevGrid G, DevState so, DevState sn, const TW *T, const TW *cpc,
+                                                      TW *sed_heat, StepConsts K)
+{
+    const int i = xcd_block().x*TPB + threadIdx.x;
+    if (i >= G.nsh) return;
+    sed_heat_column<TW, NCC, LT>(G, K, i, so.rho, sn.wind, G.area, G.vol, T, cpc, sed_heat);
 }
 
 /* phase_trans.c:16-311 (surface phase transitions off), one thread per grid box */
@@ -1473,20 +1583,26 @@ template <typename TW>
 static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, hipStream_t st)
 {
     if (G.n_const == 6) {
-        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 6>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 6, GRID2(G.nvh, G.split_edge), G, s, W, first);
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 6>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
+               (const double *)W.gas_frac);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 6, GRID2(G.nvh, G.split_edge), G, s, W, first,
+                         (const double *)W.gas_frac);
     } else if (G.n_const == 1) {
-        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 1>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 1, GRID2(G.nvh, G.split_edge), G, s, W, first);
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 1>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
+               (const double *)W.gas_frac);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 1, GRID2(G.nvh, G.split_edge), G, s, W, first,
+                         (const double *)W.gas_frac);
     } else {
-        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1);
-        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 0, GRID2(G.nvh, G.split_edge), G, s, W, first);
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
+               (const double *)W.gas_frac);
+        FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 0, GRID2(G.nvh, G.split_edge), G, s, W, first,
+                         (const double *)W.gas_frac);
     }
 }
 
 template <typename TW>
 static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, hipStream_t st)
-{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0); }
+{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr); }
 
 template <typename TW>
 static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
@@ -1586,10 +1702,10 @@ void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double 
    mixed form) */
 namespace col {
 
-void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, hipStream_t st)
-{ LAUNCH("k_temperature", k_temperature<double>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc); }
-void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, hipStream_t st)
-{ LAUNCH("k_temperature", k_temperature<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc); }
+void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, double *gf, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<double>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, gf); }
+void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, double *gf, hipStream_t st)
+{ LAUNCH("k_temperature", k_temperature<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, gf); }
 
 void phase_trans(const DevGrid &G, const DevState &s, const double *T, const double *cpc, double *ptr, double *phr,
                  double dt, hipStream_t st)
@@ -1658,7 +1774,10 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
                    const double *cpc, double *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
     if (rk == 0 && G.n_cond > 0) {
-        if (G.n_cond == 4)
+        if (G.n_cond == 4 && G.L == 26 && !G.sh_per_box)
+            LAUNCH("k_sed_heat", (k_sed_heat_col<double, 4, 26>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
+                   K);
+        else if (G.n_cond == 4)
             LAUNCH("k_sed_heat", (k_sed_heat<double, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
         else LAUNCH("k_sed_heat", (k_sed_heat<double, 0>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
     }
@@ -1669,12 +1788,26 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
                    const float *cpc, float *sed_heat, const StepConsts &K, int rk, hipStream_t st)
 {
     if (rk == 0 && G.n_cond > 0) {
-        if (G.n_cond == 4)
+        if (G.n_cond == 4 && G.L == 26 && !G.sh_per_box)
+            LAUNCH("k_sed_heat", (k_sed_heat_col<float, 4, 26>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
+                   K);
+        else if (G.n_cond == 4)
             LAUNCH("k_sed_heat", (k_sed_heat<float, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
         else LAUNCH("k_sed_heat", (k_sed_heat<float, 0>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
     }
     gen_densities_only(G, so, sn, td, K, st);
 }
+
+__global__ void __launch_bounds__(TPB) k_tracer_quotients(DevGrid G, double dt)
+{
+    const i64 b = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (b >= G.s) return;
+    const double iw = 0.5;   /* the tracer solver's implicit weight */
+    G.idv[b] = iw*dt/G.vol[b];
+}
+
+void tracer_quotients(const DevGrid &G, double dt, hipStream_t st)
+{ LAUNCH("k_tracer_quotients", k_tracer_quotients, GRID1(G.s), dim3(TPB), st, G, dt); }
 
 void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
                     hipStream_t st)

@@ -10,6 +10,8 @@
 /* horizontal operators and tendencies: work arrays double (FP64 form) or
    float (mixed form), register arithmetic FP64 in both */
 namespace hor {
+/* the gas fractions come from W.gas_frac: s must be the state the last
+   col::temperature(..., W.gas_frac, ...) was formed from */
 void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st);
 void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, hipStream_t st);
 void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st);
@@ -31,9 +33,10 @@ void vorticity_flux(const DevGrid &G, const double *F, const double *pv, double 
 /* column solvers, point physics and state arithmetic shared by both forms:
    FP64 arithmetic; the work arrays are double (FP64 form) or float (mixed) */
 namespace col {
-/* cpc (may be null): c_p of every condensate at the temperature, n_cond x S */
-void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, hipStream_t st);
-void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, hipStream_t st);
+/* cpc (may be null): c_p of every condensate at the temperature, n_cond x S;
+   gf (may be null): gas density over total density, S */
+void temperature(const DevGrid &G, const DevState &s, double *T, double *cpc, double *gf, hipStream_t st);
+void temperature(const DevGrid &G, const DevState &s, float *T, float *cpc, double *gf, hipStream_t st);
 void phase_trans(const DevGrid &G, const DevState &s, const double *T, const double *cpc, double *ptr, double *phr,
                  double dt, hipStream_t st);
 void phase_trans(const DevGrid &G, const DevState &s, const float *T, const float *cpc, float *ptr, float *phr,
@@ -50,6 +53,8 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
                    const float *cpc, float *sed_heat, const StepConsts &K, int rk, hipStream_t st);
 void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, const DevState &o, double c0, double c1,
                     hipStream_t st);
+/* G.idv[b] = (0.5 dt)/vol[b] */
+void tracer_quotients(const DevGrid &G, double dt, hipStream_t st);
 int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st);
 }
 

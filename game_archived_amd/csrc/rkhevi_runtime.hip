@@ -98,6 +98,7 @@ struct Model {
     AdjTables A;    /* reverse tables of the operator adjoints, built on first use */
     int adj_ready = 0;
     StepConsts K;
+    double idv_dt = -1.0;   /* the dt G.idv holds quotients for */
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     long long bytes = 0;
@@ -241,6 +242,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     }
     G.gd_streaming = getenv("GAME_GPU_GD_STREAMING") ? 1 : 0;
     G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
+    G.sh_per_box = getenv("GAME_GPU_SH_PER_BOX") ? 1 : 0;
     G.split_edge = 2;
     if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
         int v = atoi(e);
@@ -292,6 +294,19 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         UP(G.damp, dmp.data(), dmp.size());
         HIP_OK(hipStreamSynchronize(M->stream));
     }
+    /* cell-ordered copy of the inner-product weights */
+    {
+        const int L = d.n_layers;
+        std::vector<double> t((size_t)8*S);
+        for (int l = 0; l < L; ++l)
+            for (int j = 0; j < 8; ++j)
+                for (size_t h = 0; h < NSH; ++h)
+                    t[((size_t)l*8 + j)*NSH + h] = g->inner_product_weights[8*((size_t)l*NSH + h) + j];
+        UP(G.ipwT, t.data(), t.size());
+        HIP_OK(hipStreamSynchronize(M->stream));
+    }
+    if ((rc = dalloc((void **)&G.idv, sizeof(double)*(size_t)d.s))) return rc;
+    M->idv_dt = -1.0;
 #undef UP
     for (int k = 0; k < 3; ++k) {
         DevState &s = M->s[k];
@@ -311,6 +326,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         for (double **f : sfields) if ((rc = dzeros(f, S))) return rc;
         if ((rc = dzeros(&W.rvt, (size_t)d.dual_v_vectors))) return rc;
         if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
+        if ((rc = dzeros(&W.gas_frac, S))) return rc;
         if ((rc = dzeros(&W.pv, (size_t)d.curl))) return rc;
         if ((rc = dzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
         if ((rc = dzeros(&W.cpc, (size_t)(d.n_cond > 0 ? d.n_cond : 1)*S))) return rc;
@@ -329,6 +345,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if ((rc = fzeros(&W.ptr, (size_t)(d.n_cond + 1)*S))) return rc;
         if ((rc = fzeros(&W.cpc, (size_t)(d.n_cond > 0 ? d.n_cond : 1)*S))) return rc;
         if ((rc = dzeros(&W.vcc, 3*(size_t)G.n_oro*NSH))) return rc;
+        if ((rc = dzeros(&W.gas_frac, S))) return rc;
         if ((rc = dzeros(&W.flux2, 8))) return rc;
         if ((rc = dzeros(&W.flux_divv, S))) return rc;
     }
@@ -454,7 +471,11 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
     const DevGrid &G = M->G;
     DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
     M->K.dt = dt;
-    col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, st);
+    if (G.n_const > 1 && dt != M->idv_dt) {
+        col::tracer_quotients(G, dt, st);
+        M->idv_dt = dt;
+    }
+    col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, W.gas_frac, st);
     if (G.moist) col::phase_trans(G, so, W.temp, W.cpc, W.ptr, W.phr, dt, st);
     for (int rk = 0; rk < 2; ++rk) {
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
@@ -533,8 +554,8 @@ int game_gpu_temperature_diagnostics(int k)
     if (!M) return GAME_ERR_NOT_INIT;
     DevState *s = slot(k);
     if (!s) return GAME_ERR_ARG;
-    if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, (float *)nullptr, M->stream);
-    else col::temperature(M->G, *s, M->W.temp, (double *)nullptr, M->stream);
+    if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, (float *)nullptr, nullptr, M->stream);
+    else col::temperature(M->G, *s, M->W.temp, (double *)nullptr, nullptr, M->stream);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -586,7 +607,7 @@ int game_gpu_integrals(int k, double *out)
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 1]))) return rc;
     out[G.n_const + 1] *= 0.5;
     if ((rc = integral(G.s, rg, G.phi, &out[G.n_const + 2]))) return rc;
-    col::temperature(G, *s, M->tmp_s, (double *)nullptr, M->stream);
+    col::temperature(G, *s, M->tmp_s, (double *)nullptr, nullptr, M->stream);
     if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 3]))) return rc;
     out[G.n_const + 3] *= GC_C_D_V;
     return 0;
