@@ -855,9 +855,110 @@ __global__ void __launch_bounds__(128) k_ver_waves(DevGrid G, DevState so, DevSt
         state being written) they are stored.
    Every quantity is formed with the arithmetic of the streaming kernel, so
    the two kernels agree bit for bit. */
+/* Thomas algorithm (column_solvers.c:641-667) on rows j*NCOL + col of LDS
+   arrays c, d, e, r (n rows): the multipliers overwrite e, the solution r.
+   The next row is read before the current one is written, so each step
+   waits on its division chain only, not on an LDS round trip. */
+template <int NCOL, int NMAX>
+__device__ __forceinline__ void thomas_lds(const double *cc, const double *dd, double *ee, double *rr, int n, int col)
+{
+    double ep = 0.0, x;
+    const double d0 = dd[col];
+    if (n > 1) ep = ee[col]/d0;
+    x = rr[col]/d0;
+    ee[col] = ep; rr[col] = x;
+    double cp = cc[col], dn = 0.0, en = 0.0, rn = 0.0;
+    if (n > 1) { dn = dd[NCOL + col]; en = ee[NCOL + col]; rn = rr[NCOL + col]; }
+#pragma unroll
+    for (int j = 1; j < NMAX; ++j) {
+        if (j >= n) break;
+        const int o = j*NCOL + col, on = (j + 1 < n ? j + 1 : j)*NCOL + col;
+        const double cpn = cc[o], dnn = dd[on], enn = ee[on], rnn = rr[on];
+        const double den = dn - ep*cp;
+        if (j < n - 1) ep = en/den;
+        x = (rn - x*cp)/den;
+        ee[o] = ep; rr[o] = x;
+        cp = cpn; dn = dnn; en = enn; rn = rnn;
+    }
+    /* back substitution, upwards */
+    double eb = 0.0, rb = 0.0;
+    if (n > 1) { eb = ee[(n - 2)*NCOL + col]; rb = rr[(n - 2)*NCOL + col]; }
+#pragma unroll
+    for (int t = 0; t < NMAX - 1; ++t) {
+        const int j = n - 2 - t;
+        if (j < 0) break;
+        const int o = j*NCOL + col, om = (j > 0 ? j - 1 : 0)*NCOL + col;
+        const double ebn = ee[om], rbn = rr[om];
+        x = rb - eb*x;
+        rr[o] = x;
+        eb = ebn; rb = rbn;
+    }
+}
+
 #define VW_NCOL 32
 #define VW_THREADS 256
 
+/* the state of box (i, j) the solver reads, loaded in one batch: old state,
+   tendencies, the state snr (the new state before this substep writes it),
+   volume and background exner */
+struct VwBox {
+    double vol, exbg, rho_o, rt_o, ep_o, tp_o, trho, trt, rho_n, rt_n, ep_n;
+};
+
+__device__ __forceinline__ VwBox vw_box(const DevGrid &G, const DevState &so, const DevState &snr,
+                                        const DevState &td, int i, int j)
+{
+    const i64 b = i + (i64)j*G.nsh, gi = (i64)G.n_cond*G.s;
+    VwBox x;
+    x.vol = G.vol[b]; x.exbg = G.exbg[b];
+    x.rho_o = so.rho[gi + b]; x.rt_o = so.rt[b]; x.ep_o = so.ep[b]; x.tp_o = so.tp[b];
+    x.trho = td.rho[gi + b]; x.trt = td.rt[b];
+    x.rho_n = snr.rho[gi + b]; x.rt_n = snr.rt[b]; x.ep_n = snr.ep[b];
+    return x;
+}
+
+/* column_layer's arithmetic on loaded values */
+__device__ __forceinline__ ColumnLayer vw_layer(const VwBox &x, double dt, int rk)
+{
+    ColumnLayer c;
+    c.re = x.rho_o + dt*x.trho;
+    c.rte = x.rt_o + dt*x.trt;
+    if (rk == 0) {
+        c.al = -x.rt_o/(x.rho_o*x.rho_o)/x.vol;
+        c.be = 1.0/x.rho_o/x.vol;
+        c.ga = GC_R_D/(GC_C_D_V*x.rt_o)*(x.exbg + x.ep_o)/x.vol;
+    } else {
+        const double pdw = 0.5;
+        double ao = -x.rt_o/(x.rho_o*x.rho_o);
+        double bo = 1.0/x.rho_o;
+        double go = GC_R_D/(GC_C_D_V*x.rt_o)*(x.exbg + x.ep_o);
+        double an = -x.rt_n/(x.rho_n*x.rho_n);
+        double bn = 1.0/x.rho_n;
+        double gn = GC_R_D/(GC_C_D_V*x.rt_n)*(x.exbg + x.ep_n);
+        c.al = ((1.0 - pdw)*ao + pdw*an)/x.vol;
+        c.be = ((1.0 - pdw)*bo + pdw*bn)/x.vol;
+        c.ga = ((1.0 - pdw)*go + pdw*gn)/x.vol;
+    }
+    c.tpe = x.tp_o + dt*x.vol*(c.al*x.trho + c.be*x.trt);
+    c.epe = x.ep_o + dt*x.vol*c.ga*x.trt;
+    return c;
+}
+
+/* The same solve in three phases over a block of VW_NCOL columns, for
+   parallelism: the streaming kernel above gives each column one lane for
+   all of its work, 2.5 waves per SIMD at res 7, and its recurrence arrays
+   live in scratch.  Here
+     1. every thread builds rows (column, interface) of the tridiagonal
+        system into LDS, each from the two layers around the interface;
+     2. one lane per column runs the Thomas elimination and back
+        substitution on LDS, then all threads apply the damping;
+     3. every thread forms the new state of (column, layer) boxes into
+        registers; after a barrier (the reads of state snr may be of the
+        state being written) they are stored.
+   Each row and box loads all of its inputs in one unconditional batch
+   (indices clamped in range, unused values discarded), so a thread waits on
+   memory once per row, not once per branch.  Every quantity is formed with
+   the arithmetic of the streaming kernel, so the two agree bit for bit. */
 template <int LT, typename TWIND>
 __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState so, DevState snr, DevState sn,
                                                              DevState td, const TWIND *td_wind, StepConsts K, int rk)
@@ -868,7 +969,7 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
     const i64 gi = (i64)G.n_cond*G.s;
     const double dt = K.dt, iw = K.impl_weight;
     const double dt2 = dt*dt;
-    const double *zs = G.zs, *exbg = G.exbg, *vol = G.vol, *area = G.area;
+    const double *zs = G.zs, *area = G.area;
     const int i0 = xcd_block().x*VW_NCOL;
     double *cc = lds, *dd = lds + NI*VW_NCOL, *ee = lds + 2*NI*VW_NCOL, *rr = lds + 3*NI*VW_NCOL;
 
@@ -878,67 +979,54 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
     for (int it = 0; it < MAXIT1; ++it) {
         const int idx = threadIdx.x + it*VW_THREADS;
         if (idx >= NI*VW_NCOL) break;
-        const int j = idx/VW_NCOL, i = i0 + (idx - j*VW_NCOL);
-        if (i >= nsh) continue;
-        const i64 b = i + (i64)j*nsh, lo = b + nsh;
+        const int j = idx/VW_NCOL, ic = i0 + (idx - j*VW_NCOL);
+        const int i = ic < nsh ? ic : nsh - 1;
+        const int j2 = j + 2 < L ? j + 2 : L - 1;   /* layer j+2, clamped */
+        const i64 b = i + (i64)j*nsh, lo = b + nsh, l2 = i + (i64)j2*nsh;
         const i64 wi = i + (i64)(j + 1)*vpl;
-        const ColumnLayer ca = column_layer(G, so, snr, td, dt, rk, i, j);
-        const ColumnLayer cb = column_layer(G, so, snr, td, dt, rk, i, j + 1);
-        const double tin = theta_int_new(G, snr, i, j), rio = rho_int_old(G, so, i, j);
-        const double w_old = so.wind[wi];
+        /* one batch of loads */
+        const VwBox A = vw_box(G, so, snr, td, i, j), B = vw_box(G, so, snr, td, i, j + 1);
+        const double rt_n2 = snr.rt[l2], rho_n2 = snr.rho[gi + l2], rho_o2 = so.rho[gi + l2], exbg2 = G.exbg[l2];
+        const double zb = zs[b], zl = zs[lo], z2 = zs[l2];
+        const double w_old = so.wind[wi], tdw = (double)td_wind[wi], a_w = area[wi];
+        const double w_old2 = so.wind[i + (i64)j2*vpl];   /* w at level j+2 (interface j+1) */
+        const ColumnLayer ca = vw_layer(A, dt, rk), cb = vw_layer(B, dt, rk);
+        const double tin = 0.5*(A.rt_n/A.rho_n + B.rt_n/B.rho_n), rio = 0.5*(A.rho_o + B.rho_o);
         const double rie = 0.5*(ca.re + cb.re);
         const double d = -(tin*tin)*(ca.ga + cb.ga)
-                         + 0.5*(exbg[b] - exbg[lo])*(cb.al - ca.al + tin*(cb.be - ca.be))
-                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P*rio)
-                           *(2.0/area[wi] + dt*w_old*0.5*(-1.0/vol[b] + 1.0/vol[lo]));
-        const double r = -(w_old + dt*(double)td_wind[wi])*(zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)
+                         + 0.5*(A.exbg - B.exbg)*(cb.al - ca.al + tin*(cb.be - ca.be))
+                         - (zb - zl)/(iw*dt2*GC_C_D_P*rio)
+                           *(2.0/a_w + dt*w_old*0.5*(-1.0/A.vol + 1.0/B.vol));
+        const double r = -(w_old + dt*tdw)*(zb - zl)/(iw*dt2*GC_C_D_P)
                          + tin*(ca.epe - cb.epe)/dt
-                         + 0.5/dt*(ca.tpe + cb.tpe)*(exbg[b] - exbg[lo])
-                         - (zs[b] - zs[lo])/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
+                         + 0.5/dt*(ca.tpe + cb.tpe)*(A.exbg - B.exbg)
+                         - (zb - zl)/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
         double e = 0.0, c = 0.0;
         if (j < L - 2) {
-            const i64 l2 = lo + nsh;
-            const double tin_n = theta_int_new(G, snr, i, j + 1), rio_n = rho_int_old(G, so, i, j + 1);
+            /* theta_int_new and rho_int_old of interface j+1 */
+            const double tin_n = 0.5*(B.rt_n/B.rho_n + rt_n2/rho_n2), rio_n = 0.5*(B.rho_o + rho_o2);
             c = tin_n*cb.ga*tin
-                + 0.5*(exbg[lo] - exbg[l2])*(cb.al + cb.be*tin)
-                - (zs[lo] - zs[l2])/(iw*dt*GC_C_D_P)*0.5*so.wind[i + (i64)(j + 2)*vpl]/(vol[lo]*rio_n);
+                + 0.5*(B.exbg - exbg2)*(cb.al + cb.be*tin)
+                - (zl - z2)/(iw*dt*GC_C_D_P)*0.5*w_old2/(B.vol*rio_n);
             e = tin*cb.ga*tin_n
-                - 0.5*(exbg[b] - exbg[lo])*(cb.al + cb.be*tin_n)
-                + (zs[b] - zs[lo])/(iw*dt*GC_C_D_P)*0.5*w_old/(vol[lo]*rio);
+                - 0.5*(A.exbg - B.exbg)*(cb.al + cb.be*tin_n)
+                + (zb - zl)/(iw*dt*GC_C_D_P)*0.5*w_old/(B.vol*rio);
         }
         cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
     }
     __syncthreads();
     /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
        the multipliers overwrite e, the solution overwrites r */
-    if (threadIdx.x < VW_NCOL && i0 + (int)threadIdx.x < nsh) {
-        const int col = threadIdx.x;
-        double ep = 0.0, x;
-        if (NI > 1) ep = ee[col]/dd[col];
-        x = rr[col]/dd[col];
-        ee[col] = ep; rr[col] = x;
-        for (int j = 1; j < NI; ++j) {
-            const int o = j*VW_NCOL + col;
-            const double cp = cc[o - VW_NCOL];
-            const double den = dd[o] - ep*cp;
-            if (j < NI - 1) ep = ee[o]/den;
-            x = (rr[o] - x*cp)/den;
-            ee[o] = ep; rr[o] = x;
-        }
-        for (int j = NI - 2; j >= 0; --j) {
-            const int o = j*VW_NCOL + col;
-            x = rr[o] - ee[o]*x;
-            rr[o] = x;
-        }
-    }
+    if (threadIdx.x < VW_NCOL && i0 + (int)threadIdx.x < nsh)
+        thomas_lds<VW_NCOL, (LT > 0 ? LT : GAME_MAX_LAYERS) - 1>(cc, dd, ee, rr, NI, threadIdx.x);
     __syncthreads();
     /* Klemp (2008) damping of the solved fluxes (column_solvers.c:284-298) */
 #pragma unroll
     for (int it = 0; it < MAXIT1; ++it) {
         const int idx = threadIdx.x + it*VW_THREADS;
         if (idx >= NI*VW_NCOL) break;
-        const int j = idx/VW_NCOL, i = i0 + (idx - j*VW_NCOL);
-        if (i >= nsh) continue;
+        const int j = idx/VW_NCOL, ic = i0 + (idx - j*VW_NCOL);
+        const int i = ic < nsh ? ic : nsh - 1;
         rr[idx] = rr[idx]/(1.0 + dt*G.damp[(i64)j*nsh + i]);
     }
     __syncthreads();
@@ -950,15 +1038,25 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
         o_rho[it] = 0.0; o_rt[it] = 0.0; o_tp[it] = 0.0; o_ep[it] = 0.0; o_w[it] = 0.0;
         const int idx = threadIdx.x + it*VW_THREADS;
         if (idx >= L*VW_NCOL) break;
-        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, i = i0 + col;
-        if (i >= nsh) continue;
-        const i64 b = i + (i64)j*nsh;
-        const ColumnLayer cl = column_layer(G, so, snr, td, dt, rk, i, j);
-        const double tin_j = (j < L - 1) ? theta_int_new(G, snr, i, j) : 0.0;
-        const double tin_m = (j >= 1) ? theta_int_new(G, snr, i, j - 1) : 0.0;
+        const int j = idx/VW_NCOL, col = idx - j*VW_NCOL, ic = i0 + col;
+        const int i = ic < nsh ? ic : nsh - 1;
+        const int jm = j >= 1 ? j - 1 : 0, jp = j < L - 1 ? j + 1 : L - 1;
+        const i64 b = i + (i64)j*nsh, bm = i + (i64)jm*nsh, bp = i + (i64)jp*nsh;
+        const i64 wi = i + (i64)(j >= 1 ? j : 1)*vpl;
+        /* one batch of loads */
+        const VwBox A = vw_box(G, so, snr, td, i, j);
+        const double rt_np = snr.rt[bp], rho_np = snr.rho[gi + bp];
+        const double rt_nm = snr.rt[bm], rho_nm = snr.rho[gi + bm], rho_om = so.rho[gi + bm];
+        const double trho_m = td.rho[gi + bm], vm = G.vol[bm];
+        const double thbg = G.thbg[b], a_w = area[wi], w_o = so.wind[wi];
         const double x_j = (j < L - 1) ? rr[j*VW_NCOL + col] : 0.0;
         const double x_m = (j >= 1) ? rr[(j - 1)*VW_NCOL + col] : 0.0;
-        const double v = vol[b];
+        const double x_mm = (j >= 2) ? rr[(j - 2)*VW_NCOL + col] : 0.0;
+        const ColumnLayer cl = vw_layer(A, dt, rk);
+        /* theta_int_new of the interfaces below (j) and above (j-1) the box */
+        const double tin_j = (j < L - 1) ? 0.5*(A.rt_n/A.rho_n + rt_np/rho_np) : 0.0;
+        const double tin_m = (j >= 1) ? 0.5*(rt_nm/rho_nm + A.rt_n/A.rho_n) : 0.0;
+        const double v = A.vol;
         double rho_n, rt_n;
         if (j == 0) {
             rho_n = cl.re + dt*(x_j)/v;
@@ -970,23 +1068,19 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
             rho_n = cl.re + dt*(-x_m + x_j)/v;
             rt_n = cl.rte + dt*(-tin_m*x_m + tin_j*x_j)/v;
         }
-        const double rt_o = so.rt[b];
         o_rho[it] = rho_n;
         o_rt[it] = rt_n;
-        o_tp[it] = rt_n/rho_n - G.thbg[b];
-        o_ep[it] = so.ep[b] + v*cl.ga*(rt_n - rt_o);
+        o_tp[it] = rt_n/rho_n - thbg;
+        o_ep[it] = A.ep_o + v*cl.ga*(rt_n - A.rt_o);
         if (j >= 1) {
             /* the new density of the box above, with the arithmetic of its own thread */
-            const i64 bm = b - nsh;
-            const double re_m = so.rho[gi + bm] + dt*td.rho[gi + bm];
-            const double vm = vol[bm];
+            const double re_m = rho_om + dt*trho_m;
             double rho_m;
             if (j - 1 == 0) rho_m = re_m + dt*(x_m)/vm;
-            else rho_m = re_m + dt*(-rr[(j - 2)*VW_NCOL + col] + x_m)/vm;
-            const double rio_m = rho_int_old(G, so, i, j - 1);
-            const i64 wi = i + (i64)j*vpl;
+            else rho_m = re_m + dt*(-x_mm + x_m)/vm;
+            const double rio_m = 0.5*(rho_om + A.rho_o);   /* rho_int_old(j - 1) */
             const double din = 0.5*(rho_m + rho_n);
-            o_w[it] = (2.0*x_m/area[wi] - din*so.wind[wi])/rio_m;
+            o_w[it] = (2.0*x_m/a_w - din*w_o)/rio_m;
         }
     }
     __syncthreads();
@@ -1068,79 +1162,92 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
        coalesced) and walks down it: the flux through the interface below
        row j is the flux above row j+1, formed once.  (iw dt)/vol comes from
        the per-box table the runtime keeps for the current dt: the same
-       quotient, formed once instead of once per tracer and row. */
+       quotient, formed once instead of once per tracer and row.  The inputs
+       of the next row are loaded before the current row is formed (indices
+       clamped in range; a step past the run repeats its last row and writes
+       nothing), so the walk waits on memory once, not once per row.  Lanes
+       past the last column compute the last column's rows; phases 2 and 3
+       skip them. */
     {
         constexpr int NW = GD_THREADS/GD_NCOL;
         constexpr int PER_MAX = ((LT > 0 ? LT : GAME_MAX_LAYERS) + NW - 1)/NW;
         const int per = (L + NW - 1)/NW;
-        const int w = threadIdx.x/GD_NCOL, col = threadIdx.x - w*GD_NCOL, i = i0 + col;
+        const int w = threadIdx.x/GD_NCOL, col = threadIdx.x - w*GD_NCOL;
+        const int i = i0 + col < nsh ? i0 + col : nsh - 1;
         const int j0 = w*per, j1 = j0 + per < L ? j0 + per : L;
         const double *idv = G.idv;
-        if (i < nsh && j0 < j1) {
-            double fi_m = 0.0, fr_m = 0.0;
-            if (j0 > 0) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j0 - 1, fi_m, fr_m);
+        const int vpl = G.vpl;
+        const double sv = k < ncc/4 ? K.snow_v : (k < ncc/2 ? K.rain_v : (k < ncc ? K.cloud_v : 0.0));
+        /* the bottom row's sedimentation terms: surface area and lowest density */
+        const double a_sfc = area[i + G.v - nsh], r_low = rho_o[i + G.s - nsh];
+        struct In { double wo, wn, a, dlo, dup, q, qlo, v, rb, tb; };
+        /* the loads of row j and of the interface below it */
+        auto load = [&](int j) {
+            In x;
+            const int jc = j < L - 1 ? j : L - 2;
+            const i64 wi = i + (i64)(jc + 1)*vpl, bc = i + (i64)jc*nsh, b = i + (i64)j*nsh;
+            x.wo = so.wind[wi]; x.wn = sn.wind[wi]; x.a = area[wi];
+            x.dlo = rho_o[bc + nsh]; x.dup = rho_o[bc];
+            x.q = idv[b]; x.qlo = idv[j < L - 1 ? b + nsh : b];
+            x.v = vol[b]; x.rb = rho_o[b]; x.tb = trho[b];
+            return x;
+        };
+        /* interface fluxes (column_solvers.c:414-465) */
+        auto iface = [&](const In &x, int jj, double &fi, double &fr) {
+            double wo = x.wo, wn = x.wn;
+            if (k < ncc) { wo -= sv; wn -= sv; }
+            wo = x.a*wo;
+            wn = x.a*wn;
+            const double doi = (wn >= 0.0) ? x.dlo : x.dup;
+            fi = jj < L - 1 ? wo : 0.0;
+            fr = jj < L - 1 ? doi*wn : 0.0;
+        };
+        double fi_m = 0.0, fr_m = 0.0;
+        if (j0 > 0) { const In xm = load(j0 - 1); iface(xm, j0 - 1, fi_m, fr_m); }
+        In cur = load(j0);
 #pragma unroll
-            for (int t = 0; t < PER_MAX; ++t) {
-                const int j = j0 + t;
-                if (j >= j1) break;
-                const int idx = j*GD_NCOL + col;
-                const i64 b = i + (i64)j*nsh;
-                double fi = 0.0, fr = 0.0;
-                if (j < L - 1) tracer_iface_flux(G, so, sn, K, rho_o, k, i, j, fi, fr);
-                const double q = idv[b];   /* iw*dt/vol[b] */
-                double d;
-                if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - q*fi;
-                else if (j == L - 1) {
-                    d = (fi_m >= 0.0) ? 1.0 + q*fi_m : 1.0;
-                    if (k < ncc) d += iw*vsink*dt*area[i + G.v - nsh]/vol[b];
-                } else {
-                    d = 1.0;
-                    if (fi_m >= 0.0) d += q*fi_m;
-                    if (fi < 0.0) d -= q*fi;
-                }
-                double r = rho_o[b] + dt*trho[b];
-                if (j == 0) r += ew*dt*fr/vol[b];
-                else if (j == L - 1) {
-                    r += -ew*dt*fr_m/vol[b];
-                    if (k < ncc) {
-                        const i64 low = i + G.s - nsh;
-                        r += -ew*vsink*dt*rho_o[low]*area[i + G.v - nsh]/vol[b];
-                    }
-                } else r += ew*dt*(-fr_m + fr)/vol[b];
-                /* off-diagonals of interface j (column_solvers.c:475-488);
-                   -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
-                double c = 0.0, e = 0.0;
-                if (j < L - 1) {
-                    if (fi >= 0.0) e = (-q)*fi;
-                    else c = idv[b + nsh]*fi;
-                }
-                cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
-                fi_m = fi;
-                fr_m = fr;
+        for (int t = 0; t < PER_MAX; ++t) {
+            const bool valid = j0 + t < j1;
+            const int j = valid ? j0 + t : j1 - 1;
+            const In nxt = load(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1);
+            const int idx = j*GD_NCOL + col;
+            double fi, fr;
+            iface(cur, j, fi, fr);
+            const double q = cur.q, v = cur.v;   /* q = iw*dt/vol[b] */
+            double d;
+            if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - q*fi;
+            else if (j == L - 1) {
+                d = (fi_m >= 0.0) ? 1.0 + q*fi_m : 1.0;
+                if (k < ncc) d += iw*vsink*dt*a_sfc/v;
+            } else {
+                d = 1.0;
+                if (fi_m >= 0.0) d += q*fi_m;
+                if (fi < 0.0) d -= q*fi;
             }
+            double r = cur.rb + dt*cur.tb;
+            if (j == 0) r += ew*dt*fr/v;
+            else if (j == L - 1) {
+                r += -ew*dt*fr_m/v;
+                if (k < ncc) r += -ew*vsink*dt*r_low*a_sfc/v;
+            } else r += ew*dt*(-fr_m + fr)/v;
+            /* off-diagonals of interface j (column_solvers.c:475-488);
+               -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
+            double c = 0.0, e = 0.0;
+            if (j < L - 1) {
+                if (fi >= 0.0) e = (-q)*fi;
+                else c = cur.qlo*fi;
+            }
+            if (valid) { cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; }
+            fi_m = fi;
+            fr_m = fr;
+            cur = nxt;
         }
     }
     __syncthreads();
     /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
        the multipliers overwrite e, the solution overwrites r */
-    if (threadIdx.x < GD_NCOL && i0 + (int)threadIdx.x < nsh) {
-        const int col = threadIdx.x;
-        double ep = ee[col]/dd[col], x = rr[col]/dd[col];
-        ee[col] = ep; rr[col] = x;
-        for (int j = 1; j < L; ++j) {
-            const int o = j*GD_NCOL + col;
-            const double cp = cc[o - GD_NCOL];
-            const double den = dd[o] - ep*cp;
-            if (j < L - 1) ep = ee[o]/den;
-            x = (rr[o] - x*cp)/den;
-            ee[o] = ep; rr[o] = x;
-        }
-        for (int j = L - 2; j >= 0; --j) {
-            const int o = j*GD_NCOL + col;
-            x = rr[o] - ee[o]*x;
-            rr[o] = x;
-        }
-    }
+    if (threadIdx.x < GD_NCOL && i0 + (int)threadIdx.x < nsh)
+        thomas_lds<GD_NCOL, (LT > 0 ? LT : GAME_MAX_LAYERS)>(cc, dd, ee, rr, L, threadIdx.x);
     __syncthreads();
     /* phase 3: clipped solution (column_solvers.c:616-637) */
     double *rho_n = sn.rho + (i64)k*G.s;
