@@ -82,10 +82,12 @@ def kernel_bytes(name, dims, wb=8):
         # heat (work) and exner (state, background)
         "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (2 * S if moist else 0))
                          + (wb * (8 + 4) * S if moist else 0) + 4 * 12 * NSH,
-        "k_vector_tend": wb * (7 * V + W_) + 8 * 2 * E,
+        # tendency (read, write), pressure-gradient combination, vorticity
+        # flux, kinetic-energy gradient (work arrays), pgc at w; wind old/new
+        "k_vector_tend": wb * (5 * V + W_) + 8 * 2 * E,
         # exner, theta_bg, theta pert and the gas fraction of the cells
-        "k_pgrad_e": 8 * (4 * S + 2 * E) + wb * 5 * E + 4 * 2 * NVH,
-        "k_pgrad_w": 8 * (4 * S + 3 * W_) + wb * 6 * W_,
+        "k_pgrad_e": 8 * (4 * S + 2 * E) + wb * 6 * E + 4 * 2 * NVH,
+        "k_pgrad_w": 8 * (4 * S + 3 * W_) + wb * 7 * W_,
         "k_grad_e": wb * (S + E) + 8 * E + 4 * 2 * NVH,
         "k_grad_w": wb * (S + W_) + 8 * W_,
         "k_inner_product": 8 * (V + 8 * S) + wb * S + 4 * 6 * NSH,

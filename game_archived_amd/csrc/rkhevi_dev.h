@@ -77,6 +77,10 @@ struct DevWorkT {
     double *flux2, *flux_divv;   /* reference diagnostics the step leaves untouched */
     /* forcings */
     TW *pgrad_old, *pgrad_nl, *pgrad_l, *pgc_v, *vsq_grad, *pvt;
+    /* the pressure-gradient part of the momentum tendency, formed with the
+       gradient: ohw*old - chw*(nl + l) at edges, -cvw*(nl + l) at interior w
+       points (vector_tendencies_expl.c's weights for the step's implicit weight) */
+    TW *pgrad_comb;
     /* irreversible quantities */
     TW *ptr, *phr, *sed_heat;
     /* vertical contravariant corrections of wind, gas mass flux and rhotheta_v

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(TPB) k_temperature(DevGrid G, DevState s, TO *
    fused with calc_pressure_grad_condensates_v */
 template <typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT<TW> W, int first_step, int with_pgrad,
-                                                 const double *gf)
+                                                 const double *gf, double cvw)
 {
     i64 n = (i64)(G.L - 1)*G.nsh;
     i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
@@ -123,6 +123,9 @@ __global__ void __launch_bounds__(TPB) k_pgrad_w(DevGrid G, DevState s, DevWorkT
         W.pgrad_nl[w] = (TW)nl;
         W.pgrad_l[w] = (TW)ll;
         if (first_step) W.pgrad_old[w] = (TW)(-nl - ll);
+        /* the momentum tendency's combination of the stored values
+           (vector_tendencies_expl.c) */
+        W.pgrad_comb[w] = (TW)(-cvw*((double)(TW)nl + (double)(TW)ll));
     }
     double c_up = d_up - 1.0, c_lo = d_lo - 1.0;   /* s.rho[gi + up]/rt_up - 1.0 */
     W.pgc_v[w] = (TW)((0.5*(c_up + c_lo))*G.gm[w]);
@@ -151,7 +154,7 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int
 
 template <bool FLAT, typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step,
-                                                 const double *gf)
+                                                 const double *gf, double chw, double ohw)
 {
     const Blk bk = xcd_block();
     int e = bk.x*TPB + threadIdx.x;
@@ -162,7 +165,11 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
     layer_chunk(G.L, bk.y, l0, l1);
     for (int l = l0; l < l1; ++l) {
         i64 vi = G.nsh + (i64)l*G.vpl + e;
-        if (!first_step) W.pgrad_old[vi] = (TW)(-(double)W.pgrad_nl[vi] - (double)W.pgrad_l[vi]);
+        TW old_t = 0;
+        if (!first_step) {
+            old_t = (TW)(-(double)W.pgrad_nl[vi] - (double)W.pgrad_l[vi]);
+            W.pgrad_old[vi] = old_t;
+        }
         i64 sf = (i64)l*G.nsh + fr, st = (i64)l*G.nsh + to;
         double g = grad_edge_t<FLAT, double>(G, s.ep, l, e, fr, to);
         double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
@@ -179,7 +186,13 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
         ll = (0.5*(d_t + d_f))*ll;
         W.pgrad_nl[vi] = (TW)nl;
         W.pgrad_l[vi] = (TW)ll;
-        if (first_step) W.pgrad_old[vi] = (TW)(-nl - ll);
+        if (first_step) {
+            old_t = (TW)(-nl - ll);
+            W.pgrad_old[vi] = old_t;
+        }
+        /* the momentum tendency's combination of the stored values
+           (vector_tendencies_expl.c) */
+        W.pgrad_comb[vi] = (TW)(ohw*(double)old_t - chw*((double)(TW)nl + (double)(TW)ll));
     }
 }
 
@@ -475,15 +488,14 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWorkT<TW> W, 
     /* friction_acc is zero with momentum diffusion and PBL off: x + 0.0 == x */
     if (h >= G.nsh) {
         double tv = old_w*(double)tend_w[i] + new_w*(
-            ohw*(double)W.pgrad_old[i]
-            - chw*((double)W.pgrad_nl[i] + (double)W.pgrad_l[i])
+            (double)W.pgrad_comb[i]   /* ohw*old - chw*(nl + l) */
             + (double)W.pvt[i]
             - 0.5*(double)W.vsq_grad[i]);
         tend_w[i] = (TW)tv;
         wind_new[i] = wind_old[i] + dt*tv;
     } else {
         tend_w[i] = (TW)(old_w*(double)tend_w[i] + new_w*(
-            -cvw*((double)W.pgrad_nl[i] + (double)W.pgrad_l[i])
+            (double)W.pgrad_comb[i]   /* -cvw*(nl + l) */
             + (double)W.pvt[i]
             - 0.5*(double)W.vsq_grad[i]
             + (double)W.pgc_v[i]));
@@ -1543,6 +1555,8 @@ __global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, cons
     const i64 S = G.s;
     const double *rho = s.rho;
     double T = (double)temp[i];
+    /* all densities of the box, loaded before any branch */
+    const double rk0 = rho[i], rk1 = rho[S + i], rk2 = rho[2*S + i], rk3 = rho[3*S + i], rk4 = rho[4*S + i];
     const double mcwc = 0.2e-3;
     int ncc = G.n_cond;
     double sp = (T >= GC_T_0) ? sat_water(T) : sat_ice(T);
@@ -1564,47 +1578,47 @@ __global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, cons
     double r0, r1, r2, r3, r4, heat;
     if (sp >= wvp) {
         if (warm) {
-            r2 = -rho[2*S + i]/dt;
-            double ptd = fmin(rho[3*S + i], dd);
+            r2 = -rk2/dt;
+            double ptd = fmin(rk3, dd);
             r4 = ptd/dt;
-            r3 = rho[2*S + i]/dt - ptd/dt;
+            r3 = rk2/dt - ptd/dt;
             heat = r2*phase_trans_heat(2, T) - ptd*phase_trans_heat(0, T)/dt;
         } else {
-            r3 = -rho[3*S + i]/dt;
-            double ptd = fmin(rho[2*S + i], dd);
+            r3 = -rk3/dt;
+            double ptd = fmin(rk2, dd);
             r4 = ptd/dt;
-            r2 = rho[3*S + i]/dt - ptd/dt;
+            r2 = rk3/dt - ptd/dt;
             heat = -r3*phase_trans_heat(2, T) - ptd*phase_trans_heat(1, T)/dt;
         }
     } else {
         if (warm) {
-            r2 = -rho[2*S + i]/dt;
+            r2 = -rk2/dt;
             r4 = dd/dt;
-            r3 = rho[2*S + i]/dt - dd/dt;
+            r3 = rk2/dt - dd/dt;
             heat = r2*phase_trans_heat(2, T) - dd*phase_trans_heat(0, T)/dt;
         } else {
-            r3 = -rho[3*S + i]/dt;
+            r3 = -rk3/dt;
             r4 = dd/dt;
-            r2 = rho[3*S + i]/dt - dd/dt;
+            r2 = rk3/dt - dd/dt;
             heat = -r3*phase_trans_heat(2, T) - dd*phase_trans_heat(1, T)/dt;
         }
     }
     r0 = 0.0;
     r1 = 0.0;
     if (T < GC_T_0) {
-        r0 = fmax(rho[2*S + i] - mcwc*rho[4*S + i], 0.0)/1000.0;
+        r0 = fmax(rk2 - mcwc*rk4, 0.0)/1000.0;
         r2 -= r0;
     } else if (T >= GC_T_0) {
-        r1 = fmax(rho[3*S + i] - mcwc*rho[4*S + i], 0.0)/1000.0;
+        r1 = fmax(rk3 - mcwc*rk4, 0.0)/1000.0;
         r3 -= r1;
     }
-    if (T >= GC_T_0 && rho[i] > 0.0) {
-        r0 = -rho[i]/dt;
+    if (T >= GC_T_0 && rk0 > 0.0) {
+        r0 = -rk0/dt;
         r1 -= r0;
         heat += r0*phase_trans_heat(2, T);
     }
-    if (T < GC_T_0 && rho[S + i] > 0.0) {
-        r1 = -rho[S + i]/dt;
+    if (T < GC_T_0 && rk1 > 0.0) {
+        r1 = -rk1/dt;
         r0 -= r1;
         heat += -r1*phase_trans_heat(2, T);
     }
@@ -1687,29 +1701,31 @@ namespace hor {
     } while (0)
 
 template <typename TW>
-static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, hipStream_t st)
+static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, int first, double itw, hipStream_t st)
 {
+    /* the weights of the momentum tendency, as vector_tend_t forms them */
+    const double chw = 0.5 + itw, ohw = 1.0 - chw, cvw = 1.0 - itw;
     if (G.n_const == 6) {
         LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 6>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
-               (const double *)W.gas_frac);
+               (const double *)W.gas_frac, cvw);
         FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 6, GRID2(G.nvh, G.split_edge), G, s, W, first,
-                         (const double *)W.gas_frac);
+                         (const double *)W.gas_frac, chw, ohw);
     } else if (G.n_const == 1) {
         LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 1>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
-               (const double *)W.gas_frac);
+               (const double *)W.gas_frac, cvw);
         FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 1, GRID2(G.nvh, G.split_edge), G, s, W, first,
-                         (const double *)W.gas_frac);
+                         (const double *)W.gas_frac, chw, ohw);
     } else {
         LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, first, 1,
-               (const double *)W.gas_frac);
+               (const double *)W.gas_frac, cvw);
         FLAT_DISPATCH_NC("k_pgrad_e", k_pgrad_e, 0, GRID2(G.nvh, G.split_edge), G, s, W, first,
-                         (const double *)W.gas_frac);
+                         (const double *)W.gas_frac, chw, ohw);
     }
 }
 
 template <typename TW>
 static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, hipStream_t st)
-{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr); }
+{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr, 0.0); }
 
 template <typename TW>
 static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
@@ -1761,10 +1777,10 @@ static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &t
     }
 }
 
-void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st)
-{ pgrad_t<double>(G, s, W, first, st); }
-void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, hipStream_t st)
-{ pgrad_t<float>(G, s, W, first, st); }
+void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, double itw, hipStream_t st)
+{ pgrad_t<double>(G, s, W, first, itw, st); }
+void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, double itw, hipStream_t st)
+{ pgrad_t<float>(G, s, W, first, itw, st); }
 void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st) { pgc_v_t<double>(G, s, W, st); }
 void pgc_v(const DevGrid &G, const DevState &s, const DevWorkF &W, hipStream_t st) { pgc_v_t<float>(G, s, W, st); }
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)

@@ -12,8 +12,9 @@
 namespace hor {
 /* the gas fractions come from W.gas_frac: s must be the state the last
    col::temperature(..., W.gas_frac, ...) was formed from */
-void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, hipStream_t st);
-void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, hipStream_t st);
+/* itw: the implicit weight of the momentum tendency (W.pgrad_comb) */
+void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, double itw, hipStream_t st);
+void pgrad(const DevGrid &G, const DevState &s, const DevWorkF &W, int first, double itw, hipStream_t st);
 void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st);
 void pgc_v(const DevGrid &G, const DevState &s, const DevWorkF &W, hipStream_t st);
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st);

@@ -319,7 +319,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     if ((rc = dzeros(&M->tmp_s, S))) return rc;
     if (precision == GAME_FP64) {
         DevWork &W = M->W;
-        double **vfields[] = {&W.flux, &W.flux2, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad,
+        double **vfields[] = {&W.flux, &W.flux2, &W.pgrad_comb, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad,
                               &W.pvt};
         for (double **f : vfields) if ((rc = dzeros(f, V))) return rc;
         double **sfields[] = {&W.flux_divv, &W.temp, &W.vsq, &W.phr, &W.sed_heat};
@@ -335,7 +335,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         /* mixed: every work array FP32; the tendency state keeps FP64 densities
            and rhotheta_v, the wind tendency is FP32 */
         DevWorkF &W = M->WF;
-        float **vfields[] = {&W.flux, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad, &W.pvt,
+        float **vfields[] = {&W.flux, &W.pgrad_comb, &W.pgrad_old, &W.pgrad_nl, &W.pgrad_l, &W.pgc_v, &W.vsq_grad, &W.pvt,
                              &W.td_wind};
         for (float **f : vfields) if ((rc = fzeros(f, V))) return rc;
         float **sfields[] = {&W.temp, &W.vsq, &W.phr, &W.sed_heat};
@@ -481,7 +481,7 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
         DevState sns = snr;                                 /* ... with the horizontal wind just updated */
         sns.wind = sn.wind;
-        if (rk == 0) hor::pgrad(G, snr, W, first, st);
+        if (rk == 0) hor::pgrad(G, snr, W, first, M->cfg.impl_thermo_weight, st);
         else hor::pgc_v(G, snr, W, st);
         if (rk == 1 || first) hor::advection(G, snr.wind, snr.rho + (i64)G.n_cond*G.s, W, st);
         hor::vector_tend(G, W, td_wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
