@@ -291,47 +291,81 @@ __global__ void __launch_bounds__(TPB) k_pot_vort(DevGrid G, const double *w, co
     const Blk bk = xcd_block();
     int e = bk.x*TPB + threadIdx.x;
     if (e >= G.nvh) return;
-    const int nvh = G.nvh, nsh = G.nsh, L = G.L;
+    const int nvh = G.nvh, nsh = G.nsh, L = G.L, vpl = G.vpl;
     const int fr = G.from[e], to = G.to[e], fd = G.dfrom[e], td = G.dto[e];
+    const double *nd = G.nd;
     int dri[4];
     double drw[4];
     for (int j = 0; j < 4; ++j) { dri[j] = G.dri[4*e + j]; drw[j] = G.drw[4*e + j]; }
+    /* the edge's two cells are two of its rhombus cells: their densities come
+       with the rhombus loads, and the layer above's are carried from the
+       previous level, as are the covariant wind and normal distance of the
+       edge in the layer above (each value is the one the reference's
+       formula reads, so the arithmetic is unchanged) */
+    int pf = 0, pt = 0;
+    for (int j = 0; j < 4; ++j) { if (dri[j] == fr) pf = j; if (dri[j] == to) pt = j; }
     const double f_t = G.fvec[e], f_r = G.fvec[nvh + e];
     int l0, l1;
     layer_chunk(L + 1, bk.y, l0, l1);
     double rv1 = (l0 <= 1) ? rel_vort_tangential<FLAT>(G, w, 1, e, fr, to) : 0.0;
+    /* layer l-1 at the edge: density of both cells, covariant wind, normal distance */
+    double rf_m = 0.0, rt_m = 0.0, cv_m = 0.0, nd_m = 0.0;
+    if (l0 >= 1) {
+        rf_m = rho[(i64)(l0 - 1)*nsh + fr];
+        rt_m = rho[(i64)(l0 - 1)*nsh + to];
+        cv_m = horizontal_covariant_t<FLAT>(G, w, l0 - 1, e);
+        nd_m = nd[nsh + (i64)(l0 - 1)*vpl + e];
+    }
     for (int l = l0; l < l1; ++l) {
+        if (l == L) {
+            /* the lowest level (vorticities.c:158-163 and the surface density extrapolation) */
+            const double rv = 1/G.darea[e + (i64)l*G.dvpl]*nd_m*cv_m;
+            const i64 a = (i64)(l - 1)*nsh, b = (i64)(l - 2)*nsh;
+            const double dv = 0.5*(rho[a + fr] + rho[a + to])
+                              + (0.5*(rho[b + fr] + rho[b + to]) - 0.5*(rho[a + fr] + rho[a + to]))
+                              /G.pv_sfc_dz1[e]*G.pv_sfc_dz2[e];
+            pv[(i64)l*2*nvh + e] = (TW)((rv + f_t)/dv);
+            break;
+        }
+        /* loads of level l */
+        const i64 base = (i64)l*vpl, i0 = base + nsh + e, i1 = base + fr, i3 = base + to;
+        const i64 db = nvh + (i64)l*G.dvpl;
+        double r4[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r4[j] = rho[(i64)l*nsh + dri[j]];
+        const double nd0 = nd[i0], nd1 = nd[i1], nd3 = nd[i3], w1 = w[i1], w3 = w[i3];
+        const double da_e = G.darea[e + (i64)l*G.dvpl], af = G.darea[db + fd], at = G.darea[db + td];
+        const double rvf = (double)rvt[(i64)l*G.nt + fd], rvd = (double)rvt[(i64)l*G.nt + td];
+        const double cv0 = horizontal_covariant_t<FLAT>(G, w, l, e);
+        const double rf = pf == 0 ? r4[0] : (pf == 1 ? r4[1] : (pf == 2 ? r4[2] : r4[3]));
+        const double rt = pt == 0 ? r4[0] : (pt == 1 ? r4[1] : (pt == 2 ? r4[2] : r4[3]));
         /* tangential element at level l; the top copies level 1 (vorticities.c:158-163) */
-        double rv = (l <= 1) ? rv1 : rel_vort_tangential<FLAT>(G, w, l, e, fr, to);
+        double rv;
+        if (l <= 1) rv = rv1;
+        else rv = 1/da_e*(- nd0*cv0 + nd1*w1 + nd_m*cv_m - nd3*w3);
         double dv;
         if (l == 0) {
-            dv = 0.5*(rho[fr] + rho[to])
-                 + (0.5*(rho[fr] + rho[to]) - 0.5*(rho[fr + nsh] + rho[to + nsh]))
-                 /G.pv_top_dz1[e]*G.pv_top_dz2[e];
-        } else if (l == L) {
-            i64 a = (i64)(l - 1)*nsh, b = (i64)(l - 2)*nsh;
-            dv = 0.5*(rho[a + fr] + rho[a + to])
-                 + (0.5*(rho[b + fr] + rho[b + to]) - 0.5*(rho[a + fr] + rho[a + to]))
-                 /G.pv_sfc_dz1[e]*G.pv_sfc_dz2[e];
-        } else {
-            i64 uf = (i64)(l - 1)*nsh + fr, ut = (i64)(l - 1)*nsh + to;
-            dv = 0.25*(rho[uf] + rho[ut] + rho[uf + nsh] + rho[ut + nsh]);
-        }
+            const double rf1 = rho[nsh + fr], rt1 = rho[nsh + to];
+            dv = 0.5*(rf + rt) + (0.5*(rf + rt) - 0.5*(rf1 + rt1))/G.pv_top_dz1[e]*G.pv_top_dz2[e];
+        } else dv = 0.25*(rf_m + rt_m + rf + rt);
         pv[(i64)l*2*nvh + e] = (TW)((rv + f_t)/dv);
-        if (l == L) break;
         /* rhombus element of layer l */
-        i64 base = nvh + (i64)l*G.dvpl;
-        double af = G.darea[base + fd], at = G.darea[base + td];
-        double rr = (af*(double)rvt[(i64)l*G.nt + fd] + at*(double)rvt[(i64)l*G.nt + td])/(af + at);
+        const double rr = (af*rvf + at*rvd)/(af + at);
         double dr = 0;
-        for (int j = 0; j < 4; ++j) dr += drw[j]*rho[(i64)l*nsh + dri[j]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dr += drw[j]*r4[j];
         pv[(i64)l*2*nvh + nvh + e] = (TW)((rr + f_r)/dr);
+        rf_m = rf; rt_m = rt; cv_m = cv0; nd_m = nd0;
     }
 }
 
 /* vorticity_flux.c:16-187, horizontal part: one thread per edge, all layers */
 template <typename TW>
-__global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F, const TW *pv, TW *out)
+/* at most two waves per SIMD: the register budget lets every gather of a
+   layer issue in one batch (measured faster than four waves with the gathers
+   issued in pairs) */
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_vorticity_flux_e(DevGrid G, const TW *F, const TW *pv, TW *out)
 {
     const Blk bk = xcd_block();
     int e = bk.x*TPB + threadIdx.x;
@@ -342,35 +376,44 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_e(DevGrid G, const TW *F
     for (int j = 0; j < 10; ++j) { tw[j] = G.tw[10*(i64)e + j]; ti[j] = G.ti[10*(i64)e + j]; tm[j] = G.tmc[10*(i64)e + j]; }
     const int fr = G.from[e], to = G.to[e];
     const bool pent_f = fr < GAME_N_PENT, pent_t = to < GAME_N_PENT;
-    const double *ipw = G.ipw;
+    /* next to a pentagon a side has four terms: its fifth slot reads the
+       first slot's (valid) entries and adds -0.0, the exact identity of IEEE
+       addition, so every load of a layer issues without a branch */
+    if (pent_f) { ti[4] = ti[0]; tm[4] = tm[0]; }
+    if (pent_t) { ti[9] = ti[5]; tm[9] = tm[5]; }
+    const double *ipwT = G.ipwT;
     int l0, l1;
     layer_chunk(G.L, bk.y, l0, l1);
+    /* the w-point terms of layer l+1 are those of layer l+1's upper level:
+       carried to the next layer instead of loaded again */
+    double Ff = (double)F[(i64)l0*G.vpl + fr], Ft = (double)F[(i64)l0*G.vpl + to];
+    double q0 = (double)pv[e + (i64)l0*2*nvh];
     for (int l = l0; l < l1; ++l) {
         i64 fb = nsh + (i64)l*G.vpl, pb = nvh + (i64)l*2*nvh;
+        double Fv[10], pvv[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) { Fv[j] = (double)F[fb + ti[j]]; pvv[j] = (double)pv[pb + tm[j]]; }
+        const double pve = (double)pv[pb + e];
         double o = 0;
-        if (pent_f) {
-            for (int j = 0; j < 4; ++j) o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
-        } else {
-            for (int j = 0; j < 5; ++j) {
-                if (j == 2) o += tw[j]*(double)F[fb + ti[j]]*0.5*((double)pv[pb + tm[j]] + (double)pv[pb + e]);
-                else o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
-            }
+#pragma unroll
+        for (int j = 0; j < 10; ++j) {
+            const bool pent = j < 5 ? pent_f : pent_t;
+            /* slots 2 and 7 average with the edge's own PV, except next to a pentagon */
+            double t = ((j == 2 || j == 7) && !pent) ? tw[j]*Fv[j]*0.5*(pvv[j] + pve) : tw[j]*Fv[j]*pvv[j];
+            if ((j == 4 || j == 9) && pent) t = -0.0;
+            o += t;
         }
-        if (pent_t) {
-            for (int j = 5; j < 9; ++j) o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
-        } else {
-            for (int j = 5; j < 10; ++j) {
-                if (j == 7) o += tw[j]*(double)F[fb + ti[j]]*0.5*((double)pv[pb + tm[j]] + (double)pv[pb + e]);
-                else o += tw[j]*(double)F[fb + ti[j]]*(double)pv[pb + tm[j]];
-            }
-        }
-        i64 sf = (i64)l*nsh + fr, st = (i64)l*nsh + to;
-        double q0 = (double)pv[e + (i64)l*2*nvh], q1 = (double)pv[e + (i64)(l + 1)*2*nvh];
-        o -= 0.5*ipw[8*sf + 6]*(double)F[(i64)l*G.vpl + fr]*q0;
-        o -= 0.5*ipw[8*st + 6]*(double)F[(i64)l*G.vpl + to]*q0;
-        o -= 0.5*ipw[8*sf + 7]*(double)F[(i64)(l + 1)*G.vpl + fr]*q1;
-        o -= 0.5*ipw[8*st + 7]*(double)F[(i64)(l + 1)*G.vpl + to]*q1;
+        /* inner-product weights 6 and 7 of the two cells in layer l */
+        const double w6f = ipwT[((i64)l*8 + 6)*nsh + fr], w6t = ipwT[((i64)l*8 + 6)*nsh + to];
+        const double w7f = ipwT[((i64)l*8 + 7)*nsh + fr], w7t = ipwT[((i64)l*8 + 7)*nsh + to];
+        const double q1 = (double)pv[e + (i64)(l + 1)*2*nvh];
+        const double Ff1 = (double)F[(i64)(l + 1)*G.vpl + fr], Ft1 = (double)F[(i64)(l + 1)*G.vpl + to];
+        o -= 0.5*w6f*Ff*q0;
+        o -= 0.5*w6t*Ft*q0;
+        o -= 0.5*w7f*Ff1*q1;
+        o -= 0.5*w7t*Ft1*q1;
         out[nsh + (i64)l*G.vpl + e] = (TW)o;
+        Ff = Ff1; Ft = Ft1; q0 = q1;
     }
 }
 
