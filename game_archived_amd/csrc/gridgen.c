@@ -1031,7 +1031,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
 /* rhombus_averaging.c:16-269 */
 static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_view_t *dg,
                              const double *lat_v, const double *lon_v, const double *lat_d, const double *lon_d,
-                             int *vir)
+                             int *vir, double rh_tol)
 {
     int nvh = dm->nvh, nsh = dm->nsh;
     const int *vit = dg->vorticity_indices_triangles, *from_d = dg->from_index, *to_d = dg->to_index;
@@ -1101,7 +1101,7 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
             check += drw[4*i + j];
             if (drw[4*i + j] <= 0 || drw[4*i + j] >= 1) err = 54;
         }
-        if (fabs(check - 1) > EPS_SECURITY) { err = 55; fprintf(stderr, "rhombus %d sum %.17g w %g %g %g %g\n", i, check, drw[4*i], drw[4*i+1], drw[4*i+2], drw[4*i+3]); }
+        if (fabs(check - 1) > rh_tol) { err = 55; fprintf(stderr, "rhombus %d sum %.17g w %g %g %g %g\n", i, check, drw[4*i], drw[4*i+1], drw[4*i+2], drw[4*i+3]); }
     }
     if (err) FAIL(err, "rhombus weights failed (code %d)", err);
     return 0;
@@ -1401,7 +1401,8 @@ int game_grid_generate(const game_grid_params_t *P, const game_dims_t *dm,
         dz = (l == L - 1) ? 2*(g->z_scalar[i] - g->z_vector[(long long)L*vpl + h]) : g->z_scalar[i] - g->z_scalar[i + nsh];
         w[7] = g->area[h + (long long)(l + 1)*vpl]*dz/(2*g->volume[i]);
     }
-    rc = rhombus_averaging(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d, X->vorticity_indices_rhombi);
+    rc = rhombus_averaging(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d, X->vorticity_indices_rhombi,
+                           P->rhombus_tolerance > 0 ? P->rhombus_tolerance : EPS_SECURITY);
     if (rc) goto done;
     rc = coriolis(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d);
     if (rc) goto done;

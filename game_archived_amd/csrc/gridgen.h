@@ -33,6 +33,9 @@ typedef struct game_grid_params {
     /* optional: 2*NSH generator coordinates (lat, lon interleaved) that replace
        the generation + Lloyd iterations (the reference's use_scalar_h_coords_file) */
     const double *generator_latlon;
+    /* bound on |sum of a rhombus's density weights - 1| (grid_generator/src/
+       rhombus_averaging.c:262-265); 0 selects the reference's 1e-10 */
+    double rhombus_tolerance;
 } game_grid_params_t;
 
 /* arrays of the grid file that are not members of struct grid / dualgrid */

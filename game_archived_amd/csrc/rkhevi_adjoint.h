@@ -14,7 +14,17 @@ struct AdjTables {
     const int *tri_slot;              /* 2 NVH: the slot of e in those triangles' edge lists */
     const int *rh_off, *rh_ent;       /* CSR over cells: (edge, slot) pairs of the rhombus weights reading the cell */
     const int *tr_off, *tr_ent;       /* CSR over edges: (edge, slot) pairs of the TRSK stencils reading the edge */
+    /* the same two lists padded to a fixed length, slot-major so that lanes of
+       consecutive cells/edges read consecutive words, with the forward weight
+       (and PV index) of each entry stored beside it; padding has weight 0 */
+    const int *rhp_e;                 /* ADJ_RH_MAX x NSH: rhombus edge */
+    const double *rhp_w;              /* ADJ_RH_MAX x NSH: its weight on the cell */
+    const int *trp_e, *trp_m;         /* ADJ_TR_MAX x NVH: stencil edge, its PV index m (-(m + 1) when
+                                         the term half-averages PV with the edge's own) */
+    const double *trp_w;              /* ADJ_TR_MAX x NVH: TRSK weight */
 };
+#define ADJ_RH_MAX 12
+#define ADJ_TR_MAX 10
 
 namespace adj {
 void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st);

@@ -24,6 +24,15 @@
 
 #define ATPB 256
 #define AGRID(n) dim3((unsigned)(((n) + ATPB - 1)/ATPB))
+/* kernels that walk layers: one thread per (cell or edge, chunk of layers) */
+#define ACHUNKS 4
+#define AGRID2(n) dim3((unsigned)(((n) + ATPB - 1)/ATPB), ACHUNKS)
+
+__device__ __forceinline__ void adj_chunk(int n, int &l0, int &l1)
+{
+    l0 = (int)(((i64)n*blockIdx.y)/gridDim.y);
+    l1 = (int)(((i64)n*(blockIdx.y + 1))/gridDim.y);
+}
 
 /* ------------------------------------------------------------------ */
 /* grad^T: fbar(c, l) from gbar at the cell's edges and w points        */
@@ -77,7 +86,21 @@ __global__ void __launch_bounds__(ATPB) k_divv(DevGrid G, const double *in, doub
     i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
     if (t >= G.s) return;
     const int l = (int)(t/G.nsh), h = (int)(t - (i64)l*G.nsh);
-    out[t] = divv_h_cell(G, in, l, h);
+    if (!G.flat) { out[t] = divv_h_cell(G, in, l, h); return; }
+    /* flat surface: the terrain terms are zero; the six edges load in one batch
+       (a pentagon's sixth slot reads its first edge and adds -0.0) */
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    double v[6], a[6], sg[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int jj = j < ne ? j : 0;
+        const i64 vi = G.nsh + (i64)l*G.vpl + G.adj[6*h + jj];
+        v[j] = in[vi]; a[j] = G.area[vi]; sg[j] = (double)G.sgn[6*h + jj];
+    }
+    double comp_h = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) comp_h += j < ne ? v[j]*sg[j]*a[j] : -0.0;
+    out[t] = 1.0/G.vol[t]*comp_h;
 }
 
 /* cotangent of the vertical contravariant correction vcc(c, lev), lev in
@@ -125,27 +148,49 @@ __global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const
     const int lp = (int)(t/vpl), h = (int)(t - (i64)lp*vpl);
     if (h < nsh) { wb[t] = 0.0; return; }
     const int e = h - nsh;
-    double r = 0.0;
+    /* the edge's two triangles at layers lp-1, lp, lp+1: every load first
+       (layer indices clamped in range, out-of-range layers masked) */
+    int tri[2];
+    double sg[2];
+#pragma unroll
     for (int p = 0; p < 2; ++p) {
-        const int tri = A.tri_of_edge[2*e + p], k = A.tri_slot[2*e + p];
-        const double sg = (double)G.vst[3*tri + k];
-        for (int l = lp - 1; l <= lp + 1; ++l) {
+        tri[p] = A.tri_of_edge[2*e + p];
+        sg[p] = (double)G.vst[3*tri[p] + A.tri_slot[2*e + p]];
+    }
+    double z[2][3], nd[3], da[2][3], dzv[2][3], zv[3], zu[3], zd[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int l = lp - 1 + q, lc = l < 0 ? 0 : (l > L - 1 ? L - 1 : l);
+        const i64 vi = nsh + (i64)lc*vpl + e;
+        const i64 vu = lc > 0 ? vi - vpl : vi, vd = lc < L - 1 ? vi + vpl : vu;
+        nd[q] = G.nd[vi]; zv[q] = G.zv[vi]; zu[q] = G.zv[vu]; zd[q] = G.zv[vd];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const i64 di = G.nvh + (i64)lc*G.dvpl + tri[p];
+            z[p][q] = zb[(i64)lc*G.nt + tri[p]];
+            da[p][q] = G.darea[di];
+            dzv[p][q] = G.dzv[di];
+        }
+    }
+    double r = 0.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int l = lp - 1 + q;
             if (l < 0 || l > L - 1) continue;
-            const i64 vi = nsh + (i64)l*vpl + e;
             double lrf = 1.0, self = 1.0, other = 0.0;
             int other_layer = -1;
             if (l >= L - no) {
-                const i64 di = G.nvh + (i64)l*G.dvpl + tri;
-                lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
-                const double dz = G.dzv[di] - G.zv[vi];
+                lrf = (G.radius + dzv[p][q])/(G.radius + zv[q]);
+                const double dz = dzv[p][q] - zv[q];
                 const bool up = dz > 0 || l == L - 1;
-                const i64 ivg = up ? vi - vpl : vi + vpl;
-                const double q = dz/(G.zv[vi] - G.zv[ivg]);
-                self = 1.0 + q;
-                other = -q;
+                const double qq = dz/(zv[q] - (up ? zu[q] : zd[q]));
+                self = 1.0 + qq;
+                other = -qq;
                 other_layer = up ? l - 1 : l + 1;
             }
-            const double base = zb[(i64)l*G.nt + tri]*lrf*G.nd[vi]*sg/G.darea[G.nvh + (i64)l*G.dvpl + tri];
+            const double base = z[p][q]*lrf*nd[q]*sg[p]/da[p][q];
             if (l == lp) r += base*self;
             if (other_layer == lp) r += base*other;
         }
@@ -168,15 +213,23 @@ __global__ void __launch_bounds__(ATPB) k_rhombus(DevGrid G, const double *rho, 
 
 __global__ void __launch_bounds__(ATPB) k_rhombus_adj(DevGrid G, AdjTables A, const double *db, double *rb)
 {
-    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
-    if (t >= G.s) return;
-    const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
-    double r = 0.0;
-    for (int k = A.rh_off[c]; k < A.rh_off[c + 1]; ++k) {
-        const int e = A.rh_ent[2*k], j = A.rh_ent[2*k + 1];
-        r += G.drw[4*e + j]*db[(i64)l*G.nvh + e];
+    /* one thread per (cell, chunk of layers): the cell's rhombus entries are
+       loaded once, in ascending (edge, slot) order, padding weighted 0 */
+    const int c = blockIdx.x*ATPB + threadIdx.x;
+    if (c >= G.nsh) return;
+    const int nsh = G.nsh, nvh = G.nvh;
+    int e[ADJ_RH_MAX];
+    double w[ADJ_RH_MAX];
+#pragma unroll
+    for (int k = 0; k < ADJ_RH_MAX; ++k) { e[k] = A.rhp_e[(i64)k*nsh + c]; w[k] = A.rhp_w[(i64)k*nsh + c]; }
+    int l0, l1;
+    adj_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < ADJ_RH_MAX; ++k) r += w[k]*db[(i64)l*nvh + e[k]];
+        rb[(i64)l*nsh + c] = r;
     }
-    rb[t] = r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -187,46 +240,78 @@ __device__ __forceinline__ double vflux_w_weight(const DevGrid &G, int lev)
     return (lev == 0 || lev == G.L) ? 1.0 : 0.5;
 }
 
-__global__ void __launch_bounds__(ATPB) k_vflux_adj(DevGrid G, AdjTables A, const double *pv, const double *ob,
-                                                    double *Fb)
+__global__ void __launch_bounds__(ATPB) k_vflux_adj_e(DevGrid G, AdjTables A, const double *pv, const double *ob,
+                                                      double *Fb)
 {
-    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
-    if (t >= G.v) return;
-    const int nsh = G.nsh, nvh = G.nvh, vpl = G.vpl, L = G.L;
-    const int l = (int)(t/vpl), h = (int)(t - (i64)l*vpl);
-    double r = 0.0;
-    if (h >= nsh && l < L) {
-        /* horizontal F of edge e' at layer l */
-        const int ep = h - nsh;
+    /* horizontal F of edge e' at layer l, one thread per (edge, chunk of
+       layers): its TRSK reverse entries are loaded once */
+    const int ep = blockIdx.x*ATPB + threadIdx.x;
+    if (ep >= G.nvh) return;
+    const int nsh = G.nsh, nvh = G.nvh, vpl = G.vpl;
+    int e[ADJ_TR_MAX], m[ADJ_TR_MAX];
+    bool hf[ADJ_TR_MAX];
+    double w[ADJ_TR_MAX];
+#pragma unroll
+    for (int k = 0; k < ADJ_TR_MAX; ++k) {
+        e[k] = A.trp_e[(i64)k*nvh + ep];
+        const int mm = A.trp_m[(i64)k*nvh + ep];
+        hf[k] = mm < 0;
+        m[k] = mm < 0 ? -mm - 1 : mm;
+        w[k] = A.trp_w[(i64)k*nvh + ep];
+    }
+    const int cell[2] = {G.from[ep], G.to[ep]}, slot[2] = {A.slot_from[ep], A.slot_to[ep]};
+    int l0, l1;
+    adj_chunk(G.L, l0, l1);
+    for (int l = l0; l < l1; ++l) {
         const i64 pb = nvh + (i64)l*2*nvh;
-        for (int k = A.tr_off[ep]; k < A.tr_off[ep + 1]; ++k) {
-            const int e = A.tr_ent[2*k], j = A.tr_ent[2*k + 1];
-            const bool half = (j == 2 && G.from[e] >= GAME_N_PENT) || (j == 7 && G.to[e] >= GAME_N_PENT);
-            const double q = half ? 0.5*(pv[pb + G.tmc[10*e + j]] + pv[pb + e]) : pv[pb + G.tmc[10*e + j]];
-            r += G.tw[10*e + j]*q*ob[nsh + (i64)l*vpl + e];
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < ADJ_TR_MAX; ++k) {
+            const double pm = pv[pb + m[k]], pe = pv[pb + e[k]];
+            const double q = hf[k] ? 0.5*(pm + pe) : pm;
+            r += w[k]*q*ob[nsh + (i64)l*vpl + e[k]];
         }
         /* the vertical outputs of the edge's two cells at levels l and l + 1 */
-        const int cell[2] = {G.from[ep], G.to[ep]}, slot[2] = {A.slot_from[ep], A.slot_to[ep]};
+        const double q1 = pv[(i64)(l + 1)*2*nvh + ep], q0 = pv[(i64)l*2*nvh + ep];
+#pragma unroll
         for (int p = 0; p < 2; ++p) {
             const int c = cell[p];
-            const double w = G.ipw[8*((i64)l*nsh + c) + slot[p]];
-            r += vflux_w_weight(G, l + 1)*w*pv[(i64)(l + 1)*2*nvh + ep]*ob[(i64)(l + 1)*vpl + c];
-            r += vflux_w_weight(G, l)*w*pv[(i64)l*2*nvh + ep]*ob[(i64)l*vpl + c];
+            const double wt = G.ipwT[((i64)l*8 + slot[p])*nsh + c];
+            r += vflux_w_weight(G, l + 1)*wt*q1*ob[(i64)(l + 1)*vpl + c];
+            r += vflux_w_weight(G, l)*wt*q0*ob[(i64)l*vpl + c];
         }
-    } else if (h < nsh) {
-        /* vertical F of cell h at level l: the w terms of the cell's edges at
-           layers l (ipw slot 6) and l - 1 (slot 7) (vorticity_flux.c:110-187) */
-        const int ne = h < GAME_N_PENT ? 5 : 6;
-        for (int j = 0; j < ne; ++j) {
-            const int e = G.adj[6*h + j];
-            const double q = pv[e + (i64)l*2*nvh];
-            if (l <= L - 1)
-                r += -0.5*G.ipw[8*((i64)l*nsh + h) + 6]*q*ob[nsh + (i64)l*vpl + e];
-            if (l >= 1)
-                r += -0.5*G.ipw[8*((i64)(l - 1)*nsh + h) + 7]*q*ob[nsh + (i64)(l - 1)*vpl + e];
-        }
+        Fb[nsh + (i64)l*vpl + ep] = r;
     }
-    Fb[t] = r;
+}
+
+__global__ void __launch_bounds__(ATPB) k_vflux_adj_w(DevGrid G, const double *pv, const double *ob, double *Fb)
+{
+    /* vertical F of cell h at level l: the w terms of the cell's edges at
+       layers l (ipw slot 6) and l - 1 (slot 7) (vorticity_flux.c:110-187);
+       all loads first, layer indices clamped, out-of-range terms masked */
+    const i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
+    const int nsh = G.nsh, nvh = G.nvh, vpl = G.vpl, L = G.L;
+    if (t >= (i64)(L + 1)*nsh) return;
+    const int l = (int)(t/nsh), h = (int)(t - (i64)l*nsh);
+    const int ne = h < GAME_N_PENT ? 5 : 6;
+    const int la = l <= L - 1 ? l : L - 1, lb = l >= 1 ? l - 1 : 0;
+    const double w6 = G.ipwT[((i64)la*8 + 6)*nsh + h], w7 = G.ipwT[((i64)lb*8 + 7)*nsh + h];
+    double q[6], oa[6], obb[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int e = G.adj[6*h + (j < ne ? j : 0)];
+        q[j] = pv[e + (i64)l*2*nvh];
+        oa[j] = ob[nsh + (i64)la*vpl + e];
+        obb[j] = ob[nsh + (i64)lb*vpl + e];
+    }
+    double r = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        if (j >= ne) continue;
+        if (l <= L - 1) r += -0.5*w6*q[j]*oa[j];
+        if (l >= 1) r += -0.5*w7*q[j]*obb[j];
+    }
+    Fb[(i64)l*vpl + h] = r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -335,11 +420,14 @@ void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream
 { LAUNCH("k_rhombus", k_rhombus, AGRID((i64)G.L*G.nvh), dim3(ATPB), st, G, rho, out); }
 
 void rhombus_density_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *rb, hipStream_t st)
-{ LAUNCH("k_rhombus_adj", k_rhombus_adj, AGRID(G.s), dim3(ATPB), st, G, A, db, rb); }
+{ LAUNCH("k_rhombus_adj", k_rhombus_adj, AGRID2(G.nsh), dim3(ATPB), st, G, A, db, rb); }
 
 void vorticity_flux_adjoint(const DevGrid &G, const AdjTables &A, const double *pv, const double *ob, double *Fb,
                             hipStream_t st)
-{ LAUNCH("k_vflux_adj", k_vflux_adj, AGRID(G.v), dim3(ATPB), st, G, A, pv, ob, Fb); }
+{
+    LAUNCH("k_vflux_adj_e", k_vflux_adj_e, AGRID2(G.nvh), dim3(ATPB), st, G, A, pv, ob, Fb);
+    LAUNCH("k_vflux_adj_w", k_vflux_adj_w, AGRID((i64)(G.L + 1)*G.nsh), dim3(ATPB), st, G, pv, ob, Fb);
+}
 
 void inner_product_adjoint(const DevGrid &G, const AdjTables &A, const double *b, const double *ib, double *ab,
                            hipStream_t st)

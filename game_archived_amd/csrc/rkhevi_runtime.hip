@@ -708,11 +708,14 @@ static int build_adjoint_tables(void)
     if (M->adj_ready) return 0;
     const DevGrid &G = M->G;
     const size_t NSH = (size_t)G.nsh, NVH = (size_t)G.nvh, NT = (size_t)G.nt;
-    std::vector<int> adj(6*NSH), from(NVH), to(NVH), vit(3*NT), dri(4*NVH), ti(10*NVH);
+    std::vector<int> adj(6*NSH), from(NVH), to(NVH), vit(3*NT), dri(4*NVH), ti(10*NVH), tmc(10*NVH);
+    std::vector<double> drw(4*NVH), tw(10*NVH);
     struct { int *h; const int *d; size_t n; } cp[] = {
         {adj.data(), G.adj, 6*NSH}, {from.data(), G.from, NVH}, {to.data(), G.to, NVH},
-        {vit.data(), G.vit, 3*NT}, {dri.data(), G.dri, 4*NVH}, {ti.data(), G.ti, 10*NVH}};
+        {vit.data(), G.vit, 3*NT}, {dri.data(), G.dri, 4*NVH}, {ti.data(), G.ti, 10*NVH}, {tmc.data(), G.tmc, 10*NVH}};
     for (auto &c : cp) HIP_OK(hipMemcpyAsync(c.h, c.d, c.n*sizeof(int), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipMemcpyAsync(drw.data(), G.drw, 4*NVH*sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipMemcpyAsync(tw.data(), G.tw, 10*NVH*sizeof(double), hipMemcpyDeviceToHost, M->stream));
     HIP_OK(hipStreamSynchronize(M->stream));
     std::vector<int> sf(NVH, -1), stt(NVH, -1);
     for (size_t c = 0; c < NSH; ++c) {
@@ -773,8 +776,38 @@ static int build_adjoint_tables(void)
                 tent[2*(size_t)k + 1] = j;
             }
     }
+    /* padded slot-major copies of both lists, with the weights (and PV
+       indices) of each entry; a TRSK term that half-averages PV with the
+       edge's own (slots 2 and 7 away from pentagons) stores its PV index m
+       as -(m + 1) */
+    std::vector<int> rhp_e((size_t)ADJ_RH_MAX*NSH, 0), trp_e((size_t)ADJ_TR_MAX*NVH, 0), trp_m((size_t)ADJ_TR_MAX*NVH, 0);
+    std::vector<double> rhp_w((size_t)ADJ_RH_MAX*NSH, 0.0), trp_w((size_t)ADJ_TR_MAX*NVH, 0.0);
+    for (size_t c = 0; c < NSH; ++c) {
+        const int n = roff[c + 1] - roff[c];
+        if (n > ADJ_RH_MAX) { snprintf(g_err, sizeof g_err, "cell %zu in %d rhombi", c, n); return GAME_ERR_ARG; }
+        for (int k = 0; k < n; ++k) {
+            const int e = rent[2*(size_t)(roff[c] + k)], j = rent[2*(size_t)(roff[c] + k) + 1];
+            rhp_e[(size_t)k*NSH + c] = e;
+            rhp_w[(size_t)k*NSH + c] = drw[4*(size_t)e + j];
+        }
+    }
+    for (size_t ep = 0; ep < NVH; ++ep) {
+        const int n = toff[ep + 1] - toff[ep];
+        if (n > ADJ_TR_MAX) { snprintf(g_err, sizeof g_err, "edge %zu in %d TRSK stencils", ep, n); return GAME_ERR_ARG; }
+        for (int k = 0; k < ADJ_TR_MAX; ++k) trp_e[(size_t)k*NVH + ep] = (int)ep;
+        for (int k = 0; k < n; ++k) {
+            const int e = tent[2*(size_t)(toff[ep] + k)], j = tent[2*(size_t)(toff[ep] + k) + 1];
+            const bool half = (j == 2 && from[e] >= GAME_N_PENT) || (j == 7 && to[e] >= GAME_N_PENT);
+            trp_e[(size_t)k*NVH + ep] = e;
+            trp_m[(size_t)k*NVH + ep] = half ? -(tmc[10*(size_t)e + j] + 1) : tmc[10*(size_t)e + j];
+            trp_w[(size_t)k*NVH + ep] = tw[10*(size_t)e + j];
+        }
+    }
     AdjTables &A = M->A;
     int rc;
+    if ((rc = upload_ints(&A.rhp_e, rhp_e)) || (rc = upload_ints(&A.trp_e, trp_e)) || (rc = upload_ints(&A.trp_m, trp_m))
+        || (rc = upload(&A.rhp_w, rhp_w.data(), rhp_w.size())) || (rc = upload(&A.trp_w, trp_w.data(), trp_w.size())))
+        return rc;
     if ((rc = upload_ints(&A.slot_from, sf)) || (rc = upload_ints(&A.slot_to, stt))
         || (rc = upload_ints(&A.tri_of_edge, toe)) || (rc = upload_ints(&A.tri_slot, tsl))
         || (rc = upload_ints(&A.rh_off, roff)) || (rc = upload_ints(&A.rh_ent, rent))

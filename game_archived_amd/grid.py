@@ -45,7 +45,7 @@ class Grid:
 
     def __init__(self, res_id, n_layers=26, n_oro_layers=23, moisture_on=0, n_lloyd=2000,
                  toa=41152.0, stretching=1.3, orography="flat", oro_h0=2000.0, oro_half_width=1500e3,
-                 oro_lat_deg=30.0, oro_lon_deg=90.0, radius_rescale=1.0, cache=True):
+                 oro_lat_deg=30.0, oro_lon_deg=90.0, radius_rescale=1.0, cache=True, rhombus_tolerance=0.0):
         t0 = time.time()
         self.dims = N.make_dims(res_id, n_layers, moisture_on, n_oro_layers)
         d = self.dims
@@ -80,6 +80,8 @@ class Grid:
         p.oro_h0, p.oro_half_width = oro_h0, oro_half_width
         p.oro_lat, p.oro_lon = math.radians(oro_lat_deg), math.radians(oro_lon_deg)
         p.generator_latlon = N.ptr(gen)
+        # 0: the reference's bound on the rhombus weight sums (1e-10)
+        p.rhombus_tolerance = rhombus_tolerance
         self._gen = gen
         self.orography = orography
         self.n_lloyd = n_lloyd
