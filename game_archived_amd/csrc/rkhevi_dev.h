@@ -60,6 +60,8 @@ struct DevGrid {
     int gd_streaming;
     /* 1: sound-wave solver with one column per lane; 0: three phases over LDS */
     int vw_streaming;
+    /* 1 (26 layers): the row-parallel three-phase sound-wave kernel instead of the layer-run one */
+    int vw_rows;
     /* 1: sediment heat with one thread per grid box; 0: one thread per column (26 layers) */
     int sh_per_box;
 };

@@ -1154,6 +1154,162 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
     }
 }
 
+/* The 26-layer sound-wave solve with each wave walking a run of layers of a
+   block of VR_NCOL columns (lane = column): a row of the three-band system
+   needs the layers above and below its interface, so walking down a run
+   forms each layer's linearisation (column_layer, nine divisions at rk 1)
+   once instead of twice and loads each box once; the inputs of the next
+   layer are loaded before the current row is formed.  Phase 2 is the same
+   one-lane-per-column elimination (a full wave per block), phase 3 walks
+   runs of boxes the same way, carrying the box above's values (its new
+   density is the one its own step formed).  Every quantity keeps the
+   arithmetic of k_ver_waves. */
+#define VR_NCOL 64
+#define VR_THREADS 256
+
+template <int LT, typename TWIND>
+__global__ void __launch_bounds__(VR_THREADS) k_ver_waves_run(DevGrid G, DevState so, DevState snr, DevState sn,
+                                                              DevState td, const TWIND *td_wind, StepConsts K, int rk)
+{
+    extern __shared__ double lds[];
+    constexpr int L = LT, NI = LT - 1;
+    constexpr int NW = VR_THREADS/VR_NCOL;
+    constexpr int PER1 = (NI + NW - 1)/NW, PER3 = (L + NW - 1)/NW;
+    const int nsh = G.nsh, vpl = G.vpl;
+    const i64 gi = (i64)G.n_cond*G.s;
+    const double dt = K.dt, iw = K.impl_weight;
+    const double dt2 = dt*dt;
+    const double *zs = G.zs, *area = G.area;
+    const int i0 = xcd_block().x*VR_NCOL;
+    const int w = threadIdx.x/VR_NCOL, col = threadIdx.x - w*VR_NCOL;
+    const bool live = i0 + col < nsh;
+    const int i = live ? i0 + col : nsh - 1;
+    double *cc = lds, *dd = lds + NI*VR_NCOL, *ee = lds + 2*NI*VR_NCOL, *rr = lds + 3*NI*VR_NCOL;
+
+    /* phase 1: rows j0..j1-1 of this wave (column_solvers.c:141-189) */
+    {
+        const int j0 = w*PER1, j1 = j0 + PER1 < NI ? j0 + PER1 : NI;
+        VwBox A = vw_box(G, so, snr, td, i, j0), B = vw_box(G, so, snr, td, i, j0 + 1);
+        double za = zs[i + (i64)j0*nsh], zb = zs[i + (i64)(j0 + 1)*nsh];
+        ColumnLayer ca = vw_layer(A, dt, rk), cb = vw_layer(B, dt, rk);
+#pragma unroll
+        for (int t = 0; t < PER1; ++t) {
+            const int j = j0 + t;
+            if (j >= j1) break;
+            const int j2 = j + 2 < L ? j + 2 : L - 1;   /* layer j+2, clamped */
+            const VwBox C = vw_box(G, so, snr, td, i, j2);
+            const double zc = zs[i + (i64)j2*nsh];
+            const i64 wi = i + (i64)(j + 1)*vpl;
+            const double w_old = so.wind[wi], tdw = (double)td_wind[wi], a_w = area[wi];
+            const double w_old2 = so.wind[i + (i64)j2*vpl];   /* w at level j+2 (interface j+1) */
+            const double tin = 0.5*(A.rt_n/A.rho_n + B.rt_n/B.rho_n), rio = 0.5*(A.rho_o + B.rho_o);
+            const double rie = 0.5*(ca.re + cb.re);
+            const double d = -(tin*tin)*(ca.ga + cb.ga)
+                             + 0.5*(A.exbg - B.exbg)*(cb.al - ca.al + tin*(cb.be - ca.be))
+                             - (za - zb)/(iw*dt2*GC_C_D_P*rio)
+                               *(2.0/a_w + dt*w_old*0.5*(-1.0/A.vol + 1.0/B.vol));
+            const double r = -(w_old + dt*tdw)*(za - zb)/(iw*dt2*GC_C_D_P)
+                             + tin*(ca.epe - cb.epe)/dt
+                             + 0.5/dt*(ca.tpe + cb.tpe)*(A.exbg - B.exbg)
+                             - (za - zb)/(iw*dt2*GC_C_D_P)*w_old*rie/rio;
+            double e = 0.0, c = 0.0;
+            if (j < L - 2) {
+                const double tin_n = 0.5*(B.rt_n/B.rho_n + C.rt_n/C.rho_n), rio_n = 0.5*(B.rho_o + C.rho_o);
+                c = tin_n*cb.ga*tin
+                    + 0.5*(B.exbg - C.exbg)*(cb.al + cb.be*tin)
+                    - (zb - zc)/(iw*dt*GC_C_D_P)*0.5*w_old2/(B.vol*rio_n);
+                e = tin*cb.ga*tin_n
+                    - 0.5*(A.exbg - B.exbg)*(cb.al + cb.be*tin_n)
+                    + (za - zb)/(iw*dt*GC_C_D_P)*0.5*w_old/(B.vol*rio);
+            }
+            const int idx = j*VR_NCOL + col;
+            cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r;
+            A = B; ca = cb; za = zb;
+            B = C; cb = vw_layer(C, dt, rk); zb = zc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < VR_NCOL && live) thomas_lds<VR_NCOL, NI>(cc, dd, ee, rr, NI, col);
+    __syncthreads();
+    /* Klemp (2008) damping of the solved fluxes (column_solvers.c:284-298) */
+#pragma unroll
+    for (int it = 0; it < (NI*VR_NCOL + VR_THREADS - 1)/VR_THREADS; ++it) {
+        const int idx = threadIdx.x + it*VR_THREADS;
+        if (idx >= NI*VR_NCOL) break;
+        const int j = idx/VR_NCOL, ic = i0 + (idx - j*VR_NCOL);
+        const int ii = ic < nsh ? ic : nsh - 1;
+        rr[idx] = rr[idx]/(1.0 + dt*G.damp[(i64)j*nsh + ii]);
+    }
+    __syncthreads();
+    /* phase 3: new state of boxes k0..k1-1 of this wave (column_solvers.c:300-370) */
+    const int k0 = w*PER3, k1 = k0 + PER3 < L ? k0 + PER3 : L;
+    double o_rho[PER3], o_rt[PER3], o_tp[PER3], o_ep[PER3], o_w[PER3];
+    {
+        VwBox A = vw_box(G, so, snr, td, i, k0);
+        /* the box above the run: theta, old density and new density */
+        double rt_nm = 0.0, rho_nm = 1.0, rho_om = 0.0, rho_prev = 0.0;
+        if (k0 >= 1) {
+            const i64 bm = i + (i64)(k0 - 1)*nsh;
+            rt_nm = snr.rt[bm]; rho_nm = snr.rho[gi + bm]; rho_om = so.rho[gi + bm];
+            const double re_m = rho_om + dt*td.rho[gi + bm], vm = G.vol[bm];
+            const double x_m = rr[(k0 - 1)*VR_NCOL + col];
+            if (k0 - 1 == 0) rho_prev = re_m + dt*(x_m)/vm;
+            else rho_prev = re_m + dt*(-rr[(k0 - 2)*VR_NCOL + col] + x_m)/vm;
+        }
+#pragma unroll
+        for (int t = 0; t < PER3; ++t) {
+            o_rho[t] = 0.0; o_rt[t] = 0.0; o_tp[t] = 0.0; o_ep[t] = 0.0; o_w[t] = 0.0;
+            const int j = k0 + t;
+            if (j >= k1) break;
+            const int jp = j + 1 < L ? j + 1 : L - 1;
+            const VwBox P = vw_box(G, so, snr, td, i, jp);
+            const i64 b = i + (i64)j*nsh, wi = i + (i64)(j >= 1 ? j : 1)*vpl;
+            const double thbg = G.thbg[b], a_w = area[wi], w_o = so.wind[wi];
+            const double x_j = (j < L - 1) ? rr[j*VR_NCOL + col] : 0.0;
+            const double x_m = (j >= 1) ? rr[(j - 1)*VR_NCOL + col] : 0.0;
+            const ColumnLayer cl = vw_layer(A, dt, rk);
+            const double tin_j = (j < L - 1) ? 0.5*(A.rt_n/A.rho_n + P.rt_n/P.rho_n) : 0.0;
+            const double tin_m = (j >= 1) ? 0.5*(rt_nm/rho_nm + A.rt_n/A.rho_n) : 0.0;
+            const double v = A.vol;
+            double rho_n, rt_n;
+            if (j == 0) {
+                rho_n = cl.re + dt*(x_j)/v;
+                rt_n = cl.rte + dt*(tin_j*x_j)/v;
+            } else if (j == L - 1) {
+                rho_n = cl.re + dt*(-x_m)/v;
+                rt_n = cl.rte + dt*(-tin_m*x_m)/v;
+            } else {
+                rho_n = cl.re + dt*(-x_m + x_j)/v;
+                rt_n = cl.rte + dt*(-tin_m*x_m + tin_j*x_j)/v;
+            }
+            o_rho[t] = rho_n;
+            o_rt[t] = rt_n;
+            o_tp[t] = rt_n/rho_n - thbg;
+            o_ep[t] = A.ep_o + v*cl.ga*(rt_n - A.rt_o);
+            if (j >= 1) {
+                const double rio_m = 0.5*(rho_om + A.rho_o);   /* rho_int_old(j - 1) */
+                const double din = 0.5*(rho_prev + rho_n);
+                o_w[t] = (2.0*x_m/a_w - din*w_o)/rio_m;
+            }
+            rt_nm = A.rt_n; rho_nm = A.rho_n; rho_om = A.rho_o; rho_prev = rho_n;
+            A = P;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int t = 0; t < PER3; ++t) {
+        const int j = k0 + t;
+        if (j >= k1) break;
+        const i64 b = i + (i64)j*nsh;
+        sn.rho[gi + b] = o_rho[t];
+        sn.rt[b] = o_rt[t];
+        sn.tp[b] = o_tp[t];
+        sn.ep[b] = o_ep[t];
+        if (j >= 1) sn.wind[i + (i64)j*vpl] = o_w[t];
+    }
+}
+
 /* three_band_solver_gen_densities (column_solvers.c:385-639).
 
    The tracers of a column are independent of each other except through the
@@ -1890,6 +2046,12 @@ static void ver_waves_t(const DevGrid &G, const DevState &so, const DevState &sn
         else if (G.L == 55)
             LAUNCH("k_ver_waves", (k_ver_waves<55, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
         else LAUNCH("k_ver_waves", (k_ver_waves<0, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
+        return;
+    }
+    if (G.L == 26 && !G.vw_rows) {
+        dim3 g((G.nsh + VR_NCOL - 1)/VR_NCOL), b(VR_THREADS);
+        size_t lds = (size_t)4*(G.L - 1)*VR_NCOL*sizeof(double);   /* 50 KiB */
+        LAUNCH_SHM("k_ver_waves", (k_ver_waves_run<26, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
         return;
     }
     dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);

@@ -242,6 +242,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     }
     G.gd_streaming = getenv("GAME_GPU_GD_STREAMING") ? 1 : 0;
     G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
+    G.vw_rows = getenv("GAME_GPU_VW_ROWS") ? 1 : 0;
     G.sh_per_box = getenv("GAME_GPU_SH_PER_BOX") ? 1 : 0;
     G.split_edge = 2;
     if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
