@@ -93,6 +93,8 @@ def kernel_bytes(name, dims, wb=8):
         "k_inner_product": 8 * (V + 8 * S) + wb * S + 4 * 6 * NSH,
         "k_mass_flux": 8 * (S + E + W_) + wb * (E + W_) + 4 * 2 * NVH,
         "k_phase_trans": 8 * nc * S + wb * (7 + 4) * S,
+        # temperature, c_p, gas fraction and phase transitions in one pass
+        "k_temp_phase": 8 * ((4 + nc) * S + S) + wb * (S + 4 * S + 6 * S),
         # theta, exner (background and perturbation), all densities; writes the
         # temperature, the condensates' c_p (work) and the gas fraction
         "k_temperature": 8 * ((4 + nc) * S + S) + wb * (S + (4 * S if moist else 0)),

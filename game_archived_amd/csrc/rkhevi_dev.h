@@ -64,6 +64,8 @@ struct DevGrid {
     int vw_rows;
     /* 1: sediment heat with one thread per grid box; 0: one thread per column (26 layers) */
     int sh_per_box;
+    /* 1: temperature and phase transitions as two kernels instead of one */
+    int unfused_phase;
 };
 
 struct DevState {

@@ -1745,32 +1745,32 @@ __device__ __forceinline__ double phase_trans_heat(int dir, double t)
     return enthalpy_sublimation(t) - enthalpy_evaporation(t);
 }
 
-template <typename TW>
-__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, const TW *temp, const TW *cpc, TW *ptr,
-                                                     TW *phr, double dt)
+/* the saturation adjustment of one grid box (phase_trans.c:16-311, surface
+   phase transitions off) on loaded values: densities r[0..5] (snow, rain,
+   cloud ice, cloud water, gas, vapour), temperature T and the condensates'
+   c_p cp[0..3] as stored; rates out[0..4] and the heating rate */
+__device__ __forceinline__ void phase_trans_point(const double *r, double T, const double *cp, double dt, double *out,
+                                                  double &heat_out)
 {
-    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
-    if (i >= G.s) return;
-    const i64 S = G.s;
-    const double *rho = s.rho;
-    double T = (double)temp[i];
-    /* all densities of the box, loaded before any branch */
-    const double rk0 = rho[i], rk1 = rho[S + i], rk2 = rho[2*S + i], rk3 = rho[3*S + i], rk4 = rho[4*S + i];
+    const double rk0 = r[0], rk1 = r[1], rk2 = r[2], rk3 = r[3], rk4 = r[4], rk5 = r[5];
     const double mcwc = 0.2e-3;
-    int ncc = G.n_cond;
     double sp = (T >= GC_T_0) ? sat_water(T) : sat_ice(T);
-    double wvp = rho[(i64)(ncc + 1)*S + i]*GC_R_V*T;
-    double dp = (rho[(i64)ncc*S + i] - rho[(i64)(ncc + 1)*S + i])*GC_R_D*T;
+    double wvp = rk5*GC_R_V*T;
+    double dp = (rk4 - rk5)*GC_R_D*T;
     double ap = dp + wvp;
     double ef = (T >= GC_T_0) ? 1.00071*exp(0.000000045*ap) : 0.99882*exp(0.00000008*ap);
     sp = ef*sp;
-    double cvm = c_v_mass_weighted_cp(G, rho, cpc, i);
+    /* c_v_mass_weighted_air with the condensates' c_p (derived_quantities.c:97-115) */
+    double cvm = 0.0;
+    for (int k = 0; k < 4; ++k) cvm += r[k]*cp[k];
+    cvm += rk4*GC_C_D_V;
+    cvm += rk5*GC_C_V_V;
     int warm = T >= GC_T_0;
     int dir = warm ? 0 : 1;
     double pth = phase_trans_heat(dir, T);
     double ds = warm ? dsat_water(T) : dsat_ice(T);
     double a = -GC_R_V*pth/cvm;
-    double b = GC_R_V*T - GC_R_V*rho[(i64)(ncc + 1)*S + i]*pth/cvm + ef*ds*pth/cvm;
+    double b = GC_R_V*T - GC_R_V*rk5*pth/cvm + ef*ds*pth/cvm;
     double cc = wvp - sp;
     double p = b/a, q = cc/a;
     double dd = -0.5*p - pow(0.25*(p*p) - q, 0.5);
@@ -1821,11 +1821,53 @@ __global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, cons
         r0 -= r1;
         heat += -r1*phase_trans_heat(2, T);
     }
-    ptr[i] = (TW)r0;
-    ptr[S + i] = (TW)r1;
-    ptr[2*S + i] = (TW)r2;
-    ptr[3*S + i] = (TW)r3;
-    ptr[4*S + i] = (TW)r4;
+    out[0] = r0; out[1] = r1; out[2] = r2; out[3] = r3; out[4] = r4;
+    heat_out = heat;
+}
+
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_phase_trans(DevGrid G, DevState s, const TW *temp, const TW *cpc, TW *ptr,
+                                                     TW *phr, double dt)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    const i64 S = G.s;
+    double r[6], cp[4], out[5], heat;
+    for (int k = 0; k < 6; ++k) r[k] = s.rho[k*S + i];
+    for (int k = 0; k < 4; ++k) cp[k] = (double)cpc[k*S + i];
+    phase_trans_point(r, (double)temp[i], cp, dt, out, heat);
+    for (int k = 0; k < 5; ++k) ptr[k*S + i] = (TW)out[k];
+    phr[i] = (TW)heat;
+}
+
+/* temperature_diagnostics, the condensates' c_p, the gas fraction and the
+   saturation adjustment of a moist grid box in one pass: the adjustment
+   takes the temperature and c_p as stored (rounded to TW), so its results
+   are those of k_temperature followed by k_phase_trans */
+template <typename TW>
+__global__ void __launch_bounds__(TPB) k_temp_phase(DevGrid G, DevState s, TW *T, TW *cpc, double *gf, TW *ptr,
+                                                    TW *phr, double dt)
+{
+    i64 i = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (i >= G.s) return;
+    const i64 S = G.s;
+    double r[6];
+    for (int k = 0; k < 6; ++k) r[k] = s.rho[k*S + i];
+    const double thbg = G.thbg[i], tp = s.tp[i], exbg = G.exbg[i], ep = s.ep[i];
+    gf[i] = r[4]/density_total(G, s.rho, i);
+    double t = (thbg + tp)*(exbg + ep);
+    t = t/(1.0 + r[5]/r[4]*(GC_M_D/GC_M_V - 1.0));
+    T[i] = (TW)t;
+    const double tt = (double)(TW)t;
+    double cp[4];
+    for (int k = 0; k < 4; ++k) {
+        const TW c = (TW)c_p_cond(k, tt);   /* dictionary.c:111-190 */
+        cpc[k*S + i] = c;
+        cp[k] = (double)c;
+    }
+    double out[5], heat;
+    phase_trans_point(r, tt, cp, dt, out, heat);
+    for (int k = 0; k < 5; ++k) ptr[k*S + i] = (TW)out[k];
     phr[i] = (TW)heat;
 }
 
@@ -2035,6 +2077,12 @@ void phase_trans(const DevGrid &G, const DevState &s, const double *T, const dou
 void phase_trans(const DevGrid &G, const DevState &s, const float *T, const float *cpc, float *ptr, float *phr,
                  double dt, hipStream_t st)
 { LAUNCH("k_phase_trans", k_phase_trans<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, ptr, phr, dt); }
+void temperature_phase_trans(const DevGrid &G, const DevState &s, double *T, double *cpc, double *gf, double *ptr,
+                             double *phr, double dt, hipStream_t st)
+{ LAUNCH("k_temp_phase", k_temp_phase<double>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, gf, ptr, phr, dt); }
+void temperature_phase_trans(const DevGrid &G, const DevState &s, float *T, float *cpc, double *gf, float *ptr,
+                             float *phr, double dt, hipStream_t st)
+{ LAUNCH("k_temp_phase", k_temp_phase<float>, GRID1(G.s), dim3(TPB), st, G, s, T, cpc, gf, ptr, phr, dt); }
 
 template <typename TWIND>
 static void ver_waves_t(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn,

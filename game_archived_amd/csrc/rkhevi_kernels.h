@@ -42,6 +42,12 @@ void phase_trans(const DevGrid &G, const DevState &s, const double *T, const dou
                  double dt, hipStream_t st);
 void phase_trans(const DevGrid &G, const DevState &s, const float *T, const float *cpc, float *ptr, float *phr,
                  double dt, hipStream_t st);
+/* moist (6 constituents, 4 condensates): temperature, c_p, gas fraction and
+   phase transitions in one pass, the results of temperature + phase_trans */
+void temperature_phase_trans(const DevGrid &G, const DevState &s, double *T, double *cpc, double *gf, double *ptr,
+                             double *phr, double dt, hipStream_t st);
+void temperature_phase_trans(const DevGrid &G, const DevState &s, float *T, float *cpc, double *gf, float *ptr,
+                             float *phr, double dt, hipStream_t st);
 /* snr: the new state as read before this substep writes it (at rk 0 the old
    state, which equals it by the step's entry condition) */
 void ver_waves(const DevGrid &G, const DevState &so, const DevState &snr, const DevState &sn, const DevState &td,

@@ -244,6 +244,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
     G.vw_rows = getenv("GAME_GPU_VW_ROWS") ? 1 : 0;
     G.sh_per_box = getenv("GAME_GPU_SH_PER_BOX") ? 1 : 0;
+    G.unfused_phase = getenv("GAME_GPU_UNFUSED_PHASE") ? 1 : 0;
     G.split_edge = 2;
     if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
         int v = atoi(e);
@@ -476,8 +477,12 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
         col::tracer_quotients(G, dt, st);
         M->idv_dt = dt;
     }
-    col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, W.gas_frac, st);
-    if (G.moist) col::phase_trans(G, so, W.temp, W.cpc, W.ptr, W.phr, dt, st);
+    if (G.moist && G.n_const == 6 && !G.unfused_phase)
+        col::temperature_phase_trans(G, so, W.temp, W.cpc, W.gas_frac, W.ptr, W.phr, dt, st);
+    else {
+        col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, W.gas_frac, st);
+        if (G.moist) col::phase_trans(G, so, W.temp, W.cpc, W.ptr, W.phr, dt, st);
+    }
     for (int rk = 0; rk < 2; ++rk) {
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
         DevState sns = snr;                                 /* ... with the horizontal wind just updated */

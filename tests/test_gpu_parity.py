@@ -140,11 +140,14 @@ def test_general_terrain_path_on_flat_grid(monkeypatch):
 
 
 
-@pytest.mark.parametrize("switch", ["GAME_GPU_VW_STREAMING", "GAME_GPU_VW_ROWS", "GAME_GPU_GD_STREAMING"])
-def test_column_solver_variants(monkeypatch, switch):
-    """The other variants of the sound-wave solver (one column per lane; rows
-    in parallel over 32 columns) and of the tracer solver (one column per
-    lane), kept for measurement, give the oracle's result like the defaults."""
+@pytest.mark.parametrize("switch", ["GAME_GPU_VW_STREAMING", "GAME_GPU_VW_ROWS", "GAME_GPU_GD_STREAMING",
+                                    "GAME_GPU_SH_PER_BOX", "GAME_GPU_UNFUSED_PHASE"])
+def test_kernel_variants(monkeypatch, switch):
+    """The other variants kept for measurement -- the sound-wave solver with
+    one column per lane or rows in parallel over 32 columns, the tracer
+    solver with one column per lane, the sediment heat per grid box, the
+    temperature and phase transitions as two kernels -- give the oracle's
+    result like the defaults."""
     monkeypatch.setenv(switch, "1")
     print(f"case: res 4 moist gaussian, {switch}, 2 steps", flush=True)
     g = get_grid(4, "gaussian", moist=1)
