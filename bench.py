@@ -78,13 +78,16 @@ def kernel_bytes(name, dims, wb=8):
                       + 4 * 8 * NVH + 8 * 4 * NVH,
         "k_rel_vort_tri": 8 * (2 * E + L * NT) + wb * L * NT + 4 * 6 * NT,
         # all constituents: rho_k, rhotheta_v, wind and area at edges, volume,
-        # tendencies (read + write); moist: rates, temperature, heating, sediment
-        # heat (work) and exner (state, background)
-        "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + 2 * S * (nc + 1) + (2 * S if moist else 0))
+        # tendencies written; the old tendencies of the nc - 1 tracers are read
+        # in the second substep only (weight 0 otherwise), so half of them per
+        # launch on average; moist: rates, temperature, heating, sediment heat
+        # (work) and exner (state, background)
+        "k_scalar_tend": 8 * (S * (nc + 1) + 2 * E + S + S * (nc + 1) + S * (nc - 1) // 2 + (2 * S if moist else 0))
                          + (wb * (8 + 4) * S if moist else 0) + 4 * 12 * NSH,
-        # tendency (read, write), pressure-gradient combination, vorticity
-        # flux, kinetic-energy gradient (work arrays), pgc at w; wind old/new
-        "k_vector_tend": wb * (5 * V + W_) + 8 * 2 * E,
+        # tendency (written; read in the second substep only, half a read per
+        # launch on average), pressure-gradient combination, vorticity flux,
+        # kinetic-energy gradient (work arrays), pgc at w; wind old/new
+        "k_vector_tend": wb * (4 * V + V // 2 + W_) + 8 * 2 * E,
         # exner, theta_bg, theta pert and the gas fraction of the cells
         "k_pgrad_e": 8 * (4 * S + 2 * E) + wb * 6 * E + 4 * 2 * NVH,
         "k_pgrad_w": 8 * (4 * S + 3 * W_) + wb * 7 * W_,

@@ -528,20 +528,26 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWorkT<TW> W, 
     int l = (int)(i/G.vpl);
     int h = (int)(i - (i64)l*G.vpl);
     if (i < G.nsh || i >= G.v - G.nsh) { tend_w[i] = (TW)0.0; return; }
-    /* friction_acc is zero with momentum diffusion and PBL off: x + 0.0 == x */
+    /* friction_acc is zero with momentum diffusion and PBL off: x + 0.0 == x.
+       The old tendency has weight 0 in the first substep, where it is neither
+       read nor added (its product with 0 is a zero, the identity of the sum);
+       IEEE addition commutes, so adding it last changes no bit */
     if (h >= G.nsh) {
-        double tv = old_w*(double)tend_w[i] + new_w*(
+        double tv = new_w*(
             (double)W.pgrad_comb[i]   /* ohw*old - chw*(nl + l) */
             + (double)W.pvt[i]
             - 0.5*(double)W.vsq_grad[i]);
+        if (old_w != 0.0) tv = old_w*(double)tend_w[i] + tv;
         tend_w[i] = (TW)tv;
         wind_new[i] = wind_old[i] + dt*tv;
     } else {
-        tend_w[i] = (TW)(old_w*(double)tend_w[i] + new_w*(
+        double tv = new_w*(
             (double)W.pgrad_comb[i]   /* -cvw*(nl + l) */
             + (double)W.pvt[i]
             - 0.5*(double)W.vsq_grad[i]
-            + (double)W.pgc_v[i]));
+            + (double)W.pgc_v[i]);
+        if (old_w != 0.0) tv = old_w*(double)tend_w[i] + tv;
+        tend_w[i] = (TW)tv;
     }
 }
 
@@ -680,19 +686,22 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 double div1 = 1.0/vol*(c1 + v1);
                 double div2 = 1.0/vol*(c2 + v2);
                 const double rg = rho_h;
+                /* the gas weights are 1 and 0 in both substeps: the old
+                   tendencies are neither read nor added (their product with 0
+                   is a zero, the identity of the sum) */
                 if (!G.moist) {
                     /* dry: every source term of the reference sum is a zero array */
-                    res = old_w*td_rho[shift + i] + new_w*(-div1);
-                    out_rt = old_w*td_rt[i] + new_w*(-div2);
+                    res = new_w*(-div1);
+                    out_rt = new_w*(-div2);
                 } else {
                     /* the moist-air gas takes the vapour phase-transition rate
                        (scalar_tendencies_expl.c:94-99, 125-131, 174-176) */
                     double ptr_v = (double)W.ptr[shift + i];
                     const double th = th_h;
-                    res = old_w*td_rho[shift + i] + new_w*(-div1 + ptr_v);
+                    res = new_w*(-div1 + ptr_v);
                     double cv = c_v_mass_weighted_cp(G, s.rho, W.cpc, i);
                     double heat = (double)W.phr[i] + (double)W.sed_heat[i];
-                    out_rt = old_w*td_rt[i] + new_w*(
+                    out_rt = new_w*(
                         -div2
                         + GC_C_D_V*rg/cv*heat/(GC_C_D_P*(G.exbg[i] + s.ep[i]))
                         + ptr_v*th);
@@ -729,7 +738,10 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
                 }
                 double div = 1.0/vol*(c + v);
                 i64 shift_pt = (k == ncc + 1) ? shift - S : shift;
-                res = old_w*td_rho[shift + i] + new_w*(-div + (double)W.ptr[shift_pt + i]);
+                /* the old tendency enters with weight 0.5 in the second
+                   substep only (a uniform branch); IEEE addition commutes */
+                res = new_w*(-div + (double)W.ptr[shift_pt + i]);
+                if (old_w != 0.0) res = old_w*td_rho[shift + i] + res;
             }
             if (NC > 0) out_rho[NC > 0 ? k : 0] = res;
             else td_rho[shift + i] = res;
