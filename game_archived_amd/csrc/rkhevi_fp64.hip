@@ -216,33 +216,60 @@ __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_
     }
 }
 
-/* vorticities.c:167-222, one thread per triangle walking all layers */
+/* vorticities.c:167-222, one thread per triangle walking all layers.
+   The loads of the layer below are issued before the current layer is
+   formed, and the wind and edge heights of the layers above and below (the
+   two candidates of the terrain interpolation) are carried in registers, so
+   no load waits on the sign test that selects between them. */
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_rel_vort_tri(DevGrid G, const double *w, TW *res)
 {
     int h = xcd_block().x*TPB + threadIdx.x;
     if (h >= G.nt) return;
-    int vit[3], vst[3];
-    for (int j = 0; j < 3; ++j) { vit[j] = G.vit[3*h + j]; vst[j] = G.vst[3*h + j]; }
-    for (int l = 0; l < G.L; ++l) {
-        double r = 0;
+    const int L = G.L, lt = L - G.n_oro, nsh = G.nsh, vpl = G.vpl, nvh = G.nvh;
+    const i64 dvpl = G.dvpl;
+    int vit[3];
+    double sg[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { vit[j] = G.vit[3*h + j]; sg[j] = (double)G.vst[3*h + j]; }
+    struct Lay { double w[3], z[3], nd[3], dz, da; };
+    auto load = [&](int l) {
+        Lay x;
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            i64 vi = G.nsh + (i64)l*G.vpl + vit[j];
-            double vel = w[vi], lrf = 1;
-            if (l >= G.L - G.n_oro) {
-                i64 di = G.nvh + (i64)l*G.dvpl + h;
-                lrf = (G.radius + G.dzv[di])/(G.radius + G.zv[vi]);
-                double delta_z = G.dzv[di] - G.zv[vi];
-                i64 ivg;
-                if (delta_z > 0) ivg = vi - G.vpl;
-                else if (l == G.L - 1) ivg = vi - G.vpl;
-                else ivg = vi + G.vpl;
-                double vg = (w[vi] - w[ivg])/(G.zv[vi] - G.zv[ivg]);
+            const i64 vi = nsh + (i64)l*vpl + vit[j];
+            x.w[j] = w[vi];
+            x.z[j] = G.zv[vi];
+            x.nd[j] = G.nd[vi];
+        }
+        const i64 di = nvh + (i64)l*dvpl + h;
+        x.dz = G.dzv[di];
+        x.da = G.darea[di];
+        return x;
+    };
+    double wp[3] = {0.0, 0.0, 0.0}, zp[3] = {0.0, 0.0, 0.0};   /* layer l - 1 */
+    Lay cur = load(0);
+    for (int l = 0; l < L; ++l) {
+        const Lay nxt = load(l + 1 < L ? l + 1 : l);
+        double r = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double vel = cur.w[j], lrf = 1;
+            if (l >= lt) {
+                lrf = (G.radius + cur.dz)/(G.radius + cur.z[j]);
+                const double delta_z = cur.dz - cur.z[j];
+                /* the layer above where delta_z > 0 or at the lowest layer, else the layer below */
+                const bool up = delta_z > 0 || l == L - 1;
+                const double wg = up ? wp[j] : nxt.w[j], zg = up ? zp[j] : nxt.z[j];
+                const double vg = (cur.w[j] - wg)/(cur.z[j] - zg);
                 vel += delta_z*vg;
             }
-            r += lrf*G.nd[vi]*vst[j]*vel;
+            r += lrf*cur.nd[j]*sg[j]*vel;
         }
-        res[(i64)l*G.nt + h] = (TW)(r/G.darea[G.nvh + (i64)l*G.dvpl + h]);
+        res[(i64)l*G.nt + h] = (TW)(r/cur.da);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) { wp[j] = cur.w[j]; zp[j] = cur.z[j]; }
+        cur = nxt;
     }
 }
 
@@ -431,31 +458,48 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
     int adj[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + (j < ne ? j : 0)];
+    const int L = G.L, vpl = G.vpl;
+    const double *ipwT = G.ipwT;
+    /* level l sums the terms of layers l - 1 and l; the flux and weights of
+       layer l are carried to level l + 1 instead of loaded again */
+    auto load_layer = [&](int l, double *Fx, double *Wx) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            Fx[j] = (double)F[nsh + (i64)l*vpl + adj[j]];
+            Wx[j] = ipwT[((i64)l*8 + j)*nsh + h];
+        }
+    };
     int l0, l1;
-    layer_chunk(G.L + 1, bk.y, l0, l1);
+    layer_chunk(L + 1, bk.y, l0, l1);
+    double Fm[6], Wm[6];
+    load_layer(l0 >= 1 ? l0 - 1 : 0, Fm, Wm);
     for (int l = l0; l < l1; ++l) {
+        double Fc[6], Wc[6], q[6];
+        load_layer(l <= L - 1 ? l : L - 1, Fc, Wc);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) q[j] = (double)pv[(i64)l*2*nvh + adj[j]];
         double vw = 0.5;
-        if (l == 0 || l == G.L) vw = 1;
+        if (l == 0 || l == L) vw = 1;
         double o = 0;
         if (l >= 1) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                double t = vw*G.ipwT[((i64)(l - 1)*8 + j)*nsh + h]*(double)F[nsh + (i64)(l - 1)*G.vpl + adj[j]]
-                           *(double)pv[(i64)l*2*nvh + adj[j]];
+                double t = vw*Wm[j]*Fm[j]*q[j];
                 if (j >= ne) t = -0.0;
                 o += t;
             }
         }
-        if (l <= G.L - 1) {
+        if (l <= L - 1) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                double t = vw*G.ipwT[((i64)l*8 + j)*nsh + h]*(double)F[nsh + (i64)l*G.vpl + adj[j]]
-                           *(double)pv[(i64)l*2*nvh + adj[j]];
+                double t = vw*Wc[j]*Fc[j]*q[j];
                 if (j >= ne) t = -0.0;
                 o += t;
             }
         }
-        out[(i64)l*G.vpl + h] = (TW)o;
+        out[(i64)l*vpl + h] = (TW)o;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { Fm[j] = Fc[j]; Wm[j] = Wc[j]; }
     }
 }
 
@@ -474,9 +518,12 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
     for (int j = 0; j < 6; ++j) adj[j] = G.adj[6*h + (j < ne ? j : 0)];
     int l0, l1;
     layer_chunk(G.L, bk.y, l0, l1);
+    /* the w point below layer l is the one above layer l + 1: carried */
+    double au = a[h + (i64)l0*G.vpl], bu = b[h + (i64)l0*G.vpl];
     for (int l = l0; l < l1; ++l) {
         i64 i = (i64)l*G.nsh + h;
         const double *w = G.ipwT + (i64)l*8*G.nsh + h;   /* weight j at w[j*nsh] */
+        const double al = a[h + (i64)(l + 1)*G.vpl], bl = b[h + (i64)(l + 1)*G.vpl];
         double o = 0;
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
@@ -485,9 +532,11 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
             if (j >= ne) t = -0.0;
             o += t;
         }
-        o += w[6*(i64)G.nsh]*a[h + (i64)l*G.vpl]*b[h + (i64)l*G.vpl];
-        o += w[7*(i64)G.nsh]*a[h + (i64)(l + 1)*G.vpl]*b[h + (i64)(l + 1)*G.vpl];
+        o += w[6*(i64)G.nsh]*au*bu;
+        o += w[7*(i64)G.nsh]*al*bl;
         out[i] = (TW)o;
+        au = al;
+        bu = bl;
     }
 }
 
@@ -1978,7 +2027,15 @@ static void pgrad_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, 
 
 template <typename TW>
 static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, hipStream_t st)
-{ LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr, 0.0); }
+{
+    /* the constituent count as a compile-time constant where it is 6 or 1, so
+       the density sums unroll and their loads issue together */
+    if (G.n_const == 6)
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 6>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr, 0.0);
+    else if (G.n_const == 1)
+        LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 1>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr, 0.0);
+    else LAUNCH("k_pgrad_w", (k_pgrad_w<TW, 0>), GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, s, W, 0, 0, nullptr, 0.0);
+}
 
 template <typename TW>
 static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
