@@ -115,7 +115,8 @@ class GridParams(C.Structure):
                 ("n_lloyd_iterations", C.c_int), ("stretching_parameter", C.c_double), ("toa", C.c_double),
                 ("radius_rescale", C.c_double), ("oro_id", C.c_int), ("oro_h0", C.c_double),
                 ("oro_half_width", C.c_double), ("oro_lat", C.c_double), ("oro_lon", C.c_double),
-                ("generator_latlon", DP), ("rhombus_tolerance", C.c_double)]
+                ("generator_latlon", DP), ("rhombus_tolerance", C.c_double),
+                ("trsk_tolerance", C.c_double)]
 
 
 class GridExtras(C.Structure):

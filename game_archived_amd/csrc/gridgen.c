@@ -885,7 +885,7 @@ static void host_grad(const game_dims_t *d, const game_grid_view_t *g, const dou
 /* TRSK weights and Gassmann (2018) resorting, coriolis.c:16-438          */
 /* ------------------------------------------------------------------ */
 static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_view_t *dg,
-                    const double *lat_v, const double *lon_v, const double *lat_d, const double *lon_d)
+                    const double *lat_v, const double *lon_v, const double *lat_d, const double *lon_d, double tol)
 {
     int nsh = dm->nsh, nvh = dm->nvh;
     const int *from = g->from_index, *to = g->to_index, *adj = g->adjacent_vector_indices_h;
@@ -898,6 +898,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
     int *ti = g->trsk_indices, *tmc = g->trsk_modified_curl_indices;
     double *tw = g->trsk_weights;
     int err = 0;
+    double bad_ratio = 0.0;   /* the failing sum ratio of check 30, for the message */
     #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < nvh; ++i) {
         if (err) continue;
@@ -944,7 +945,11 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
                 areas[l] = pow(radius + zv[nsh + i], 2)*(t0 + t1);
                 check_sum += areas[l];
             }
-            if (fabs(check_sum/(rescale_2d*area[cell]) - 1) > EPS_SECURITY) { err = 30; break; }
+            if (fabs(check_sum/(rescale_2d*area[cell]) - 1) > tol) {
+                bad_ratio = check_sum/(rescale_2d*area[cell]);
+                err = 30;
+                break;
+            }
             int first_index = -1, last_index = -1;
             for (int l = 0; l < ne; ++l) {
                 if (ei[l] == i) last_index = l;
@@ -1008,6 +1013,9 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
             for (int k = j + 1; k < 10; ++k)
                 if (t[j] == t[k] && tw[10*i + j] != 0 && tw[10*i + k] != 0) err = 29;
     }
+    if (err == 30)
+        FAIL(err, "TRSK construction failed (reference problem code 30: kite areas / cell area = %.17g, "
+             "bound %g)", bad_ratio, tol);
     if (err) FAIL(err, "TRSK construction failed (reference problem code %d)", err);
     /* Eq. (39) of Thuburn et al. (2009), coriolis.c:393-427 */
     #pragma omp parallel for
@@ -1020,7 +1028,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
             for (int k = 0; k < 10; ++k) if (ti[10*fi + k] == i) second = 10*fi + k;
             if (second == -1) { err = 38; continue; }
             double v1 = nd[nsh + fi]/(rescale_1d*nd_dual[i])*tw[second];
-            if (fabs(v0 + v1) > EPS_SECURITY) err = 39;
+            if (fabs(v0 + v1) > tol) err = 39;
         }
     }
     if (err) FAIL(err, "TRSK antisymmetry check failed (reference problem code %d)", err);
@@ -1404,7 +1412,8 @@ int game_grid_generate(const game_grid_params_t *P, const game_dims_t *dm,
     rc = rhombus_averaging(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d, X->vorticity_indices_rhombi,
                            P->rhombus_tolerance > 0 ? P->rhombus_tolerance : EPS_SECURITY);
     if (rc) goto done;
-    rc = coriolis(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d);
+    rc = coriolis(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d,
+                  P->trsk_tolerance > 0 ? P->trsk_tolerance : EPS_SECURITY);
     if (rc) goto done;
     /* derived metrics, src/io/set_grid_properties.c:196-221 */
     for (int i = 0; i < 6*nsh; ++i) if (g->adjacent_vector_indices_h[i] == -1) g->adjacent_vector_indices_h[i] = 0;

@@ -36,6 +36,10 @@ typedef struct game_grid_params {
     /* bound on |sum of a rhombus's density weights - 1| (grid_generator/src/
        rhombus_averaging.c:262-265); 0 selects the reference's 1e-10 */
     double rhombus_tolerance;
+    /* bound of the TRSK checks 30 (kite areas sum to the cell area) and 39
+       (Eq. (39) antisymmetry), grid_generator/src/coriolis.c:188-194, 393-427;
+       0 selects the reference's 1e-10 */
+    double trsk_tolerance;
 } game_grid_params_t;
 
 /* arrays of the grid file that are not members of struct grid / dualgrid */

@@ -1381,8 +1381,11 @@ __global__ void __launch_bounds__(VR_THREADS) k_ver_waves_run(DevGrid G, DevStat
      2. one lane per column runs the Thomas elimination on LDS;
      3. every thread writes the clipped solution back, coalesced.
    Each coefficient is formed with the reference's arithmetic. */
-#define GD_NCOL 64
 #define GD_THREADS 256
+/* columns per block: 64 at 26 layers (53 KiB of LDS, three blocks per CU);
+   32 for deeper columns, so that two blocks still fit in LDS at 55 or 64
+   layers (56 / 64 KiB) -- the lanes of a 32-column group are half a wave */
+__host__ __device__ constexpr int gd_ncol(int lt) { return lt == 26 ? 64 : 32; }
 
 /* vertical fluxes of tracer k through interface jj of column i
    (column_solvers.c:414-465): fi with the old wind, fr with the new wind
@@ -1412,6 +1415,7 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
                                                               StepConsts K)
 {
     extern __shared__ double lds[];
+    constexpr int GD_NCOL = gd_ncol(LT);
     const int L = LT > 0 ? LT : G.L, nsh = G.nsh, ncc = G.n_cond;
     /* the tracer is the fast block index: the blocks of one group of columns
        run side by side on one XCD and share the wind, area and volume columns in L2 */
@@ -2198,18 +2202,9 @@ static void gen_densities_only(const DevGrid &G, const DevState &so, const DevSt
         else LAUNCH("k_gen_densities", k_gen_densities_stream<0>, g, b, st, G, so, sn, td, K);
         return;
     }
-    dim3 g(((G.nsh + GD_NCOL - 1)/GD_NCOL)*(G.n_const - 1)), b(GD_THREADS);
-    size_t lds = (size_t)4*G.L*GD_NCOL*sizeof(double);   /* 53 KiB at 26 layers, 128 KiB at 64 */
-    if (lds > 65536) {
-        static bool raised = false;
-        if (!raised) {
-            (void)hipFuncSetAttribute((const void *)k_gen_densities<55>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(4*GAME_MAX_LAYERS*GD_NCOL*sizeof(double)));
-            (void)hipFuncSetAttribute((const void *)k_gen_densities<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(4*GAME_MAX_LAYERS*GD_NCOL*sizeof(double)));
-            raised = true;
-        }
-    }
+    const int ncol = gd_ncol(G.L == 26 ? 26 : (G.L == 55 ? 55 : 0));
+    dim3 g(((G.nsh + ncol - 1)/ncol)*(G.n_const - 1)), b(GD_THREADS);
+    size_t lds = (size_t)4*G.L*ncol*sizeof(double);   /* 53 KiB at 26 layers, 56 KiB at 55, 64 KiB at 64 */
     if (G.L == 26) LAUNCH_SHM("k_gen_densities", k_gen_densities<26>, g, b, lds, st, G, so, sn, td, K);
     else if (G.L == 55) LAUNCH_SHM("k_gen_densities", k_gen_densities<55>, g, b, lds, st, G, so, sn, td, K);
     else LAUNCH_SHM("k_gen_densities", k_gen_densities<0>, g, b, lds, st, G, so, sn, td, K);
@@ -2221,6 +2216,9 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
     if (rk == 0 && G.n_cond > 0) {
         if (G.n_cond == 4 && G.L == 26 && !G.sh_per_box)
             LAUNCH("k_sed_heat", (k_sed_heat_col<double, 4, 26>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
+                   K);
+        else if (G.n_cond == 4 && G.L == 55 && !G.sh_per_box)
+            LAUNCH("k_sed_heat", (k_sed_heat_col<double, 4, 55>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
                    K);
         else if (G.n_cond == 4)
             LAUNCH("k_sed_heat", (k_sed_heat<double, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);
@@ -2235,6 +2233,9 @@ void gen_densities(const DevGrid &G, const DevState &so, const DevState &sn, con
     if (rk == 0 && G.n_cond > 0) {
         if (G.n_cond == 4 && G.L == 26 && !G.sh_per_box)
             LAUNCH("k_sed_heat", (k_sed_heat_col<float, 4, 26>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
+                   K);
+        else if (G.n_cond == 4 && G.L == 55 && !G.sh_per_box)
+            LAUNCH("k_sed_heat", (k_sed_heat_col<float, 4, 55>), GRID1(G.nsh), dim3(TPB), st, G, so, sn, T, cpc, sed_heat,
                    K);
         else if (G.n_cond == 4)
             LAUNCH("k_sed_heat", (k_sed_heat<float, 4>), GRID1(G.s), dim3(TPB), st, G, so, sn, T, cpc, sed_heat, K);

@@ -45,7 +45,8 @@ class Grid:
 
     def __init__(self, res_id, n_layers=26, n_oro_layers=23, moisture_on=0, n_lloyd=2000,
                  toa=41152.0, stretching=1.3, orography="flat", oro_h0=2000.0, oro_half_width=1500e3,
-                 oro_lat_deg=30.0, oro_lon_deg=90.0, radius_rescale=1.0, cache=True, rhombus_tolerance=0.0):
+                 oro_lat_deg=30.0, oro_lon_deg=90.0, radius_rescale=1.0, cache=True, rhombus_tolerance=0.0,
+                 trsk_tolerance=0.0):
         t0 = time.time()
         self.dims = N.make_dims(res_id, n_layers, moisture_on, n_oro_layers)
         d = self.dims
@@ -82,6 +83,8 @@ class Grid:
         p.generator_latlon = N.ptr(gen)
         # 0: the reference's bound on the rhombus weight sums (1e-10)
         p.rhombus_tolerance = rhombus_tolerance
+        # 0: the reference's bound on the TRSK kite-area and antisymmetry checks (1e-10)
+        p.trsk_tolerance = trsk_tolerance
         self._gen = gen
         self.orography = orography
         self.n_lloyd = n_lloyd
