@@ -1957,33 +1957,45 @@ __global__ void __launch_bounds__(TPB) k_lincomb_vector(i64 n, const double *a, 
     o[i] = c0*a[i] + c1*b[i];
 }
 
+/* merge of two compensated partial sums (s, c) += (s2, c2): the sum of the
+   leading parts with its exact rounding error (Neumaier's two-sum), the
+   compensations added */
+__device__ __forceinline__ void neumaier_merge(double &s, double &c, double s2, double c2)
+{
+    const double t = s + s2;
+    double err;
+    if (fabs(s) >= fabs(s2)) err = (s - t) + s2;
+    else err = (s2 - t) + s;
+    s = t;
+    c = (c + c2) + err;
+}
+
 /* per-block compensated partial sums of x*volume (write_output.c:1659-1668 with
-   Neumaier summation); the host adds the block partials, also compensated */
+   Neumaier summation).  Each lane sums a grid-strided subset; the 64 lanes of
+   a wavefront are merged by butterfly shuffles (six steps of __shfl_xor, no
+   LDS), the four wavefronts of the block through LDS; the host adds the
+   block partials, also compensated. */
 __global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *x, const double *y, const double *vol,
                                                            double *partials)
 {
-    __shared__ double ss[TPB], sc[TPB];
+    __shared__ double ss[TPB/64], sc[TPB/64];
     double s = 0.0, c = 0.0;
     for (i64 i = (i64)blockIdx.x*TPB + threadIdx.x; i < n; i += (i64)gridDim.x*TPB) {
         double v = x[i]*vol[i];
         if (y) v = x[i]*y[i]*vol[i];
-        double t = s + v;
-        if (fabs(s) >= fabs(v)) c += (s - t) + v;
-        else c += (v - t) + s;
-        s = t;
+        neumaier_merge(s, c, v, 0.0);
     }
-    ss[threadIdx.x] = s;
-    sc[threadIdx.x] = c;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(c, off, 64);
+        neumaier_merge(s, c, s2, c2);
+    }
+    const int wv = threadIdx.x/64, lane = threadIdx.x%64;
+    if (lane == 0) { ss[wv] = s; sc[wv] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double S = 0.0, Cc = 0.0;
-        for (int k = 0; k < TPB; ++k) {
-            double v = ss[k], t = S + v;
-            if (fabs(S) >= fabs(v)) Cc += (S - t) + v;
-            else Cc += (v - t) + S;
-            S = t;
-            Cc += sc[k];
-        }
+        double S = ss[0], Cc = sc[0];
+        for (int k = 1; k < TPB/64; ++k) neumaier_merge(S, Cc, ss[k], sc[k]);
         partials[2*blockIdx.x] = S;
         partials[2*blockIdx.x + 1] = Cc;
     }
