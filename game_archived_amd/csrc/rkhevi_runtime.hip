@@ -105,6 +105,13 @@ struct Model {
     double *partials = nullptr, *h_partials = nullptr;
     double *tmp_s = nullptr;
     EventTimer timer;
+    /* HIP graphs of one steady step (not the first), one per assignment of
+       the two state buffers to old/new, keyed by the old state's density
+       buffer; rebuilt when dt or the configuration changes */
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const void *graph_key[2] = {nullptr, nullptr};
+    double graph_dt = -1.0;
+    int use_graphs = 1;
 };
 
 Model *M = nullptr;
@@ -180,10 +187,23 @@ const char *game_gpu_last_error(void) { return g_err; }
 
 long long game_gpu_bytes_allocated(void) { return M ? M->bytes : 0; }
 
+static void drop_graphs(void)
+{
+    /* a replay still in flight finishes before its executable graph goes */
+    if (M->gexec[0] || M->gexec[1]) (void)hipStreamSynchronize(M->stream);
+    for (int p = 0; p < 2; ++p) {
+        if (M->gexec[p]) (void)hipGraphExecDestroy(M->gexec[p]);
+        M->gexec[p] = nullptr;
+        M->graph_key[p] = nullptr;
+    }
+    M->graph_dt = -1.0;
+}
+
 int game_gpu_finalize(void)
 {
     if (!M) return 0;
     (void)hipStreamSynchronize(M->stream);
+    drop_graphs();
     for (void *p : M->allocs) (void)hipFree(p);
     if (M->h_partials) (void)hipHostFree(M->h_partials);
     for (auto e : M->timer.pool) (void)hipEventDestroy(e);
@@ -247,6 +267,7 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
     G.vw_rows = getenv("GAME_GPU_VW_ROWS") ? 1 : 0;
     G.sh_per_box = getenv("GAME_GPU_SH_PER_BOX") ? 1 : 0;
     G.unfused_phase = getenv("GAME_GPU_UNFUSED_PHASE") ? 1 : 0;
+    M->use_graphs = getenv("GAME_GPU_NO_GRAPH") ? 0 : 1;
     G.split_edge = 2;
     if (const char *e = getenv("GAME_GPU_SPLIT_EDGE")) {
         int v = atoi(e);
@@ -372,6 +393,7 @@ int game_gpu_set_config(const game_config_t *config)
     int rc = check_config(config);
     if (rc) return rc;
     M->cfg = *config;
+    drop_graphs();
     M->K.impl_weight = config->impl_thermo_weight;
     M->K.snow_v = config->snow_velocity;
     M->K.rain_v = config->rain_velocity;
@@ -540,6 +562,72 @@ int game_gpu_linear_combine_two_states(int k0, int k1, int ko, double c0, double
     return 0;
 }
 
+}  /* extern "C" */
+
+/* one steady step (not the first) replayed from a HIP graph: the step's
+   launches depend only on the state buffers, dt and the configuration, so
+   one captured step per assignment of the two state buffers serves every
+   later step with the same dt.  The tracer quotient table of a new dt is
+   refreshed before the capture (outside the graph); no memory set or copy is
+   recorded.  Returns 1 when the step must run eagerly instead. */
+static int step_from_graph(double dt, int *rc)
+{
+    *rc = 0;
+    if (!M->use_graphs || g_kernel_timer) return 1;
+    hipStream_t st = M->stream;
+    if (M->G.n_const > 1 && dt != M->idv_dt) {
+        col::tracer_quotients(M->G, dt, st);
+        M->idv_dt = dt;
+    }
+    if (dt != M->graph_dt) {
+        drop_graphs();
+        M->graph_dt = dt;
+    }
+    const void *key = M->s[GAME_STATE_OLD].rho;
+    int p = M->graph_key[0] == key ? 0 : (M->graph_key[1] == key ? 1 : -1);
+    if (p < 0) {
+        p = M->graph_key[0] == nullptr ? 0 : 1;
+        if (M->gexec[p]) {
+            (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(M->gexec[p]);
+            M->gexec[p] = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            snprintf(g_err, sizeof g_err, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+            *rc = GAME_ERR_HIP;
+            return 0;
+        }
+        int r = M->precision == GAME_MIXED ? step_t(M->WF, M->WF.td_wind, dt, 0)
+                                           : step_t(M->W, M->s[GAME_STATE_TENDENCY].wind, dt, 0);
+        e = hipStreamEndCapture(st, &g);
+        if (r) { if (g) (void)hipGraphDestroy(g); *rc = r; return 0; }
+        if (e != hipSuccess) {
+            snprintf(g_err, sizeof g_err, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+            *rc = GAME_ERR_HIP;
+            return 0;
+        }
+        e = hipGraphInstantiate(&M->gexec[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            M->gexec[p] = nullptr;
+            snprintf(g_err, sizeof g_err, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            *rc = GAME_ERR_HIP;
+            return 0;
+        }
+        M->graph_key[p] = key;
+    }
+    hipError_t e = hipGraphLaunch(M->gexec[p], st);
+    if (e != hipSuccess) {
+        snprintf(g_err, sizeof g_err, "hipGraphLaunch failed: %s", hipGetErrorString(e));
+        *rc = GAME_ERR_HIP;
+    }
+    return 0;
+}
+
+extern "C" {
+
 int game_gpu_run_steps(int n, double dt, int first)
 {
     if (!M) return GAME_ERR_NOT_INIT;
@@ -550,7 +638,9 @@ int game_gpu_run_steps(int n, double dt, int first)
         DevState t = M->s[GAME_STATE_OLD];
         M->s[GAME_STATE_OLD] = M->s[GAME_STATE_NEW];
         M->s[GAME_STATE_NEW] = t;
-        int rc = game_gpu_manage_rkhevi(dt, first);
+        int rc = 0;
+        /* the first step takes other branches (pgrad_old, rk-0 advection): eager */
+        if (first || step_from_graph(dt, &rc)) rc = game_gpu_manage_rkhevi(dt, first);
         if (rc) return rc;
         first = 0;
     }
