@@ -140,16 +140,66 @@ __global__ void __launch_bounds__(ATPB) k_divv_adj(DevGrid G, AdjTables A, const
 /* ------------------------------------------------------------------ */
 /* curl on triangles^T                                                  */
 /* ------------------------------------------------------------------ */
+/* one thread per (edge, chunk of layers) walking down the layers: the two
+   triangles' contributions of layer l (to layer l itself and, in the terrain
+   layers, to the layer its interpolation reads) are formed once and kept for
+   the layers above and below; wind points at w levels get 0 (the forward
+   reads none).  The sum for layer lp adds, per triangle, the terms of layers
+   lp-1, lp, lp+1 in that order. */
+struct CurlAdjTerm {
+    double s[2], o[2];   /* base*self (to layer l) and base*other (to layer l -+ 1) */
+    int d[2];            /* -1: the other term goes to l-1, +1: to l+1, 0: none */
+};
+
+__device__ __forceinline__ CurlAdjTerm curl_adj_term(const DevGrid &G, const double *zb, int e, const int *tri,
+                                                     const double *sg, int l)
+{
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
+    const i64 vi = nsh + (i64)l*vpl + e;
+    const i64 vu = l > 0 ? vi - vpl : vi, vd = l < L - 1 ? vi + vpl : vu;
+    const double nd = G.nd[vi], zv = G.zv[vi], zu = G.zv[vu], zd = G.zv[vd];
+    double z[2], da[2], dzv[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const i64 di = G.nvh + (i64)l*G.dvpl + tri[p];
+        z[p] = zb[(i64)l*G.nt + tri[p]];
+        da[p] = G.darea[di];
+        dzv[p] = G.dzv[di];
+    }
+    CurlAdjTerm c;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        double lrf = 1.0, self = 1.0, other = 0.0;
+        int d = 0;
+        if (l >= L - G.n_oro) {
+            lrf = (G.radius + dzv[p])/(G.radius + zv);
+            const double dz = dzv[p] - zv;
+            const bool up = dz > 0 || l == L - 1;
+            const double qq = dz/(zv - (up ? zu : zd));
+            self = 1.0 + qq;
+            other = -qq;
+            d = up ? -1 : 1;
+        }
+        const double base = z[p]*lrf*nd*sg[p]/da[p];
+        c.s[p] = base*self;
+        c.o[p] = base*other;
+        c.d[p] = d;
+    }
+    return c;
+}
+
 __global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const double *zb, double *wb)
 {
-    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
-    if (t >= G.v) return;
-    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro;
-    const int lp = (int)(t/vpl), h = (int)(t - (i64)lp*vpl);
-    if (h < nsh) { wb[t] = 0.0; return; }
-    const int e = h - nsh;
-    /* the edge's two triangles at layers lp-1, lp, lp+1: every load first
-       (layer indices clamped in range, out-of-range layers masked) */
+    const int t = blockIdx.x*ATPB + threadIdx.x;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
+    int l0, l1;
+    adj_chunk(L, l0, l1);
+    if (t < nsh) {
+        for (int l = l0; l < l1; ++l) wb[(i64)l*vpl + t] = 0.0;
+        if (l1 == L) wb[(i64)L*vpl + t] = 0.0;
+    }
+    if (t >= G.nvh) return;
+    const int e = t;
     int tri[2];
     double sg[2];
 #pragma unroll
@@ -157,45 +207,22 @@ __global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const
         tri[p] = A.tri_of_edge[2*e + p];
         sg[p] = (double)G.vst[3*tri[p] + A.tri_slot[2*e + p]];
     }
-    double z[2][3], nd[3], da[2][3], dzv[2][3], zv[3], zu[3], zd[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int l = lp - 1 + q, lc = l < 0 ? 0 : (l > L - 1 ? L - 1 : l);
-        const i64 vi = nsh + (i64)lc*vpl + e;
-        const i64 vu = lc > 0 ? vi - vpl : vi, vd = lc < L - 1 ? vi + vpl : vu;
-        nd[q] = G.nd[vi]; zv[q] = G.zv[vi]; zu[q] = G.zv[vu]; zd[q] = G.zv[vd];
+    CurlAdjTerm prev = {}, cur = curl_adj_term(G, zb, e, tri, sg, l0), nxt = {};
+    if (l0 >= 1) prev = curl_adj_term(G, zb, e, tri, sg, l0 - 1);
+    if (l0 + 1 <= L - 1) nxt = curl_adj_term(G, zb, e, tri, sg, l0 + 1);
+    for (int lp = l0; lp < l1; ++lp) {
+        double r = 0.0;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const i64 di = G.nvh + (i64)lc*G.dvpl + tri[p];
-            z[p][q] = zb[(i64)lc*G.nt + tri[p]];
-            da[p][q] = G.darea[di];
-            dzv[p][q] = G.dzv[di];
+            if (lp >= 1 && prev.d[p] == 1) r += prev.o[p];
+            r += cur.s[p];
+            if (lp + 1 <= L - 1 && nxt.d[p] == -1) r += nxt.o[p];
         }
+        wb[nsh + (i64)lp*vpl + e] = r;
+        prev = cur;
+        cur = nxt;
+        if (lp + 2 <= L - 1 && lp + 1 < l1) nxt = curl_adj_term(G, zb, e, tri, sg, lp + 2);
     }
-    double r = 0.0;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int l = lp - 1 + q;
-            if (l < 0 || l > L - 1) continue;
-            double lrf = 1.0, self = 1.0, other = 0.0;
-            int other_layer = -1;
-            if (l >= L - no) {
-                lrf = (G.radius + dzv[p][q])/(G.radius + zv[q]);
-                const double dz = dzv[p][q] - zv[q];
-                const bool up = dz > 0 || l == L - 1;
-                const double qq = dz/(zv[q] - (up ? zu[q] : zd[q]));
-                self = 1.0 + qq;
-                other = -qq;
-                other_layer = up ? l - 1 : l + 1;
-            }
-            const double base = z[p][q]*lrf*nd[q]*sg[p]/da[p][q];
-            if (l == lp) r += base*self;
-            if (other_layer == lp) r += base*other;
-        }
-    }
-    wb[t] = r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -414,7 +441,7 @@ void divv_h_adjoint(const DevGrid &G, const AdjTables &A, const double *db, doub
 { LAUNCH("k_divv_adj", k_divv_adj, AGRID(G.v), dim3(ATPB), st, G, A, db, Fb); }
 
 void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st)
-{ LAUNCH("k_curl_adj", k_curl_adj, AGRID(G.v), dim3(ATPB), st, G, A, zb, wb); }
+{ LAUNCH("k_curl_adj", k_curl_adj, AGRID2(G.nvh > G.nsh ? G.nvh : G.nsh), dim3(ATPB), st, G, A, zb, wb); }
 
 void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream_t st)
 { LAUNCH("k_rhombus", k_rhombus, AGRID((i64)G.L*G.nvh), dim3(ATPB), st, G, rho, out); }
