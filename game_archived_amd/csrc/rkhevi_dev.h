@@ -129,16 +129,20 @@ __device__ __forceinline__ double density_total_n(const DevGrid &G, const double
 }
 
 /* dictionary.c:111-190 */
-__device__ __forceinline__ double c_p_cond(int k, double t)
+__device__ __forceinline__ double c_p_ice(double t)
 {
-    if ((k & 1) == 0) {
-        if (t > GC_T_0) t = GC_T_0;
-        if (t < 20.0) t = 20.0;
-        double q = t/125.1;
-        double r = -2.0572 + 0.14644*t + 0.06163*t*exp(-(q*q));
-        return r/GC_M_V;
-    }
+    if (t > GC_T_0) t = GC_T_0;
+    if (t < 20.0) t = 20.0;
+    double q = t/125.1;
+    double r = -2.0572 + 0.14644*t + 0.06163*t*exp(-(q*q));
+    return r/GC_M_V;
+}
+
+__device__ __forceinline__ double c_p_water(double t)
+{
     double tc = t - GC_T_0, r;
+    /* a branch, not a select: a wave whose boxes are all above or all below
+       freezing evaluates one of the two polynomials */
     if (tc >= 0.0) {
         if (tc > 35.0) tc = 35.0;
         double a = tc - 35.0;
@@ -148,6 +152,12 @@ __device__ __forceinline__ double c_p_cond(int k, double t)
         r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
     }
     return 4186.8*r;
+}
+
+/* even k: ice phases (snow, cloud ice), odd k: liquid (rain, cloud water) */
+__device__ __forceinline__ double c_p_cond(int k, double t)
+{
+    return (k & 1) == 0 ? c_p_ice(t) : c_p_water(t);
 }
 
 /* c_v_mass_weighted_air with the condensates' c_p taken from a table formed

@@ -1819,21 +1819,32 @@ __device__ __forceinline__ void phase_trans_point(const double *r, double T, con
 {
     const double rk0 = r[0], rk1 = r[1], rk2 = r[2], rk3 = r[3], rk4 = r[4], rk5 = r[5];
     const double mcwc = 0.2e-3;
-    double sp = (T >= GC_T_0) ? sat_water(T) : sat_ice(T);
     double wvp = rk5*GC_R_V*T;
     double dp = (rk4 - rk5)*GC_R_D*T;
     double ap = dp + wvp;
-    double ef = (T >= GC_T_0) ? 1.00071*exp(0.000000045*ap) : 0.99882*exp(0.00000008*ap);
+    const int warm = T >= GC_T_0;
+    /* saturation pressure, enhancement factor, the phase's latent heat and the
+       saturation pressure's derivative, over water or over ice: a branch, not
+       a select, so a wave whose boxes are all above or all below freezing
+       evaluates one set of exponentials */
+    double sp, ef, pth, ds;
+    if (warm) {
+        sp = sat_water(T);
+        ef = 1.00071*exp(0.000000045*ap);
+        pth = phase_trans_heat(0, T);
+        ds = dsat_water(T);
+    } else {
+        sp = sat_ice(T);
+        ef = 0.99882*exp(0.00000008*ap);
+        pth = phase_trans_heat(1, T);
+        ds = dsat_ice(T);
+    }
     sp = ef*sp;
     /* c_v_mass_weighted_air with the condensates' c_p (derived_quantities.c:97-115) */
     double cvm = 0.0;
     for (int k = 0; k < 4; ++k) cvm += r[k]*cp[k];
     cvm += rk4*GC_C_D_V;
     cvm += rk5*GC_C_V_V;
-    int warm = T >= GC_T_0;
-    int dir = warm ? 0 : 1;
-    double pth = phase_trans_heat(dir, T);
-    double ds = warm ? dsat_water(T) : dsat_ice(T);
     double a = -GC_R_V*pth/cvm;
     double b = GC_R_V*T - GC_R_V*rk5*pth/cvm + ef*ds*pth/cvm;
     double cc = wvp - sp;
@@ -1924,9 +1935,12 @@ __global__ void __launch_bounds__(TPB) k_temp_phase(DevGrid G, DevState s, TW *T
     t = t/(1.0 + r[5]/r[4]*(GC_M_D/GC_M_V - 1.0));
     T[i] = (TW)t;
     const double tt = (double)(TW)t;
+    /* dictionary.c:111-190: snow and cloud ice share the ice formula, rain and
+       cloud water the liquid one; each is evaluated once */
+    const TW c_ice = (TW)c_p_ice(tt), c_liq = (TW)c_p_water(tt);
     double cp[4];
     for (int k = 0; k < 4; ++k) {
-        const TW c = (TW)c_p_cond(k, tt);   /* dictionary.c:111-190 */
+        const TW c = (k & 1) == 0 ? c_ice : c_liq;
         cpc[k*S + i] = c;
         cp[k] = (double)c;
     }
