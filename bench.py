@@ -383,10 +383,20 @@ def main():
         a_, b_ = res["mixed"]["final"], res["fp64"]["final"]
         gas = slice(d.n_cond * d.s, (d.n_cond + 1) * d.s)
         rel = lambda u, v: float(np.linalg.norm(u - v) / np.linalg.norm(v))
+        relmax = lambda u, v: float(np.abs(u - v).max() / np.abs(v).max())
+        # wind components: w at the NSH vertical points of every level, u at the edges
+        k = np.arange(d.v) % d.vpl
+        is_w = k < d.nsh
+        fields = {"rho_gas": (a_.rho[gas], b_.rho[gas]), "rho_all_constituents": (a_.rho, b_.rho),
+                  "rhotheta_v": (a_.rhotheta_v, b_.rhotheta_v), "w": (a_.wind[is_w], b_.wind[is_w]),
+                  "u": (a_.wind[~is_w], b_.wind[~is_w])}
         form_cfg["mixed"]["rel_l2_vs_fp64"] = {
-            "rho_gas": rel(a_.rho[gas], b_.rho[gas]), "rhotheta_v": rel(a_.rhotheta_v, b_.rhotheta_v),
+            "rho_gas": rel(*fields["rho_gas"]), "rhotheta_v": rel(*fields["rhotheta_v"]),
             "wind": rel(a_.wind, b_.wind), "after_steps": warmup + steps,
             "after_sim_days": round((warmup + steps) * args.dt / 86400.0, 4)}
+        # per prognostic field: relative L2 and max |difference| / max |FP64 value|
+        form_cfg["mixed"]["error_vs_fp64"] = {
+            name: {"rel_l2": rel(u, v), "rel_max": relmax(u, v)} for name, (u, v) in fields.items()}
         form_cfg["mixed"]["speedup_vs_fp64"] = round(res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"], 3)
     finite = all(x["finite"] for x in res.values())
     out = {

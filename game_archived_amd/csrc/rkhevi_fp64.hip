@@ -196,23 +196,34 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
     }
 }
 
-/* scalar_times_vector(rho_gas, wind) into the mass flux density (multiplications.c:35-117) */
+/* scalar_times_vector(rho_gas, wind) into the mass flux density
+   (multiplications.c:35-117).  One thread per edge and chunk of layers (the
+   edge's from/to cells are read once), and one per cell and chunk of w
+   levels, in the same launch. */
 template <typename TW>
 __global__ void __launch_bounds__(TPB) k_mass_flux(DevGrid G, const double *rho_gas, const double *wind, TW *flux)
 {
-    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
-    i64 ne = (i64)G.L*G.nvh, nw = (i64)(G.L - 1)*G.nsh;
-    if (t < ne) {
-        int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-        i64 vi = G.nsh + (i64)l*G.vpl + e;
-        double sv = 0.5*(rho_gas[G.to[e] + (i64)l*G.nsh] + rho_gas[G.from[e] + (i64)l*G.nsh]);
-        flux[vi] = (TW)(sv*wind[vi]);
-    } else if (t < ne + nw) {
-        t -= ne;
-        int lev = (int)(t/G.nsh) + 1, h = (int)(t - (i64)(lev - 1)*G.nsh);
-        i64 w = (i64)lev*G.vpl + h;
-        double sv = 0.5*(rho_gas[h + (i64)(lev - 1)*G.nsh] + rho_gas[h + (i64)lev*G.nsh]);
-        flux[w] = (TW)(sv*wind[w]);
+    const Blk bk = xcd_block();
+    const int t = bk.x*TPB + threadIdx.x;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
+    int l0, l1;
+    layer_chunk(L, bk.y, l0, l1);
+    if (t < G.nvh) {
+        const int e = t, fr = G.from[e], to = G.to[e];
+        for (int l = l0; l < l1; ++l) {
+            const i64 vi = nsh + (i64)l*vpl + e;
+            const double sv = 0.5*(rho_gas[to + (i64)l*nsh] + rho_gas[fr + (i64)l*nsh]);
+            flux[vi] = (TW)(sv*wind[vi]);
+        }
+    }
+    if (t < nsh) {
+        /* interior w levels 1..L-1: level lev of this chunk's layers [l0, l1) */
+        const int h = t;
+        for (int lev = l0 > 1 ? l0 : 1; lev < l1; ++lev) {
+            const i64 w = (i64)lev*vpl + h;
+            const double sv = 0.5*(rho_gas[h + (i64)(lev - 1)*nsh] + rho_gas[h + (i64)lev*nsh]);
+            flux[w] = (TW)(sv*wind[w]);
+        }
     }
 }
 
@@ -2071,8 +2082,8 @@ template <typename TW>
 static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
                         hipStream_t st)
 {
-    LAUNCH("k_mass_flux", k_mass_flux<TW>, GRID1((i64)G.L*G.nvh + (i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, rho_gas,
-           wind, W.flux);
+    LAUNCH("k_mass_flux", k_mass_flux<TW>, GRID2(G.nvh > G.nsh ? G.nvh : G.nsh, G.split_edge), dim3(TPB), st, G,
+           rho_gas, wind, W.flux);
     LAUNCH("k_rel_vort_tri", k_rel_vort_tri<TW>, GRID1(G.nt), dim3(TPB), st, G, wind, W.rvt);
     FLAT_DISPATCH("k_pot_vort", k_pot_vort, GRID2(G.nvh, G.split_edge), G, wind, rho_gas, (const TW *)W.rvt, W.pv);
     LAUNCH("k_vorticity_flux_e", k_vorticity_flux_e<TW>, GRID2(G.nvh, G.split_edge), dim3(TPB), st, G, (const TW *)W.flux,
