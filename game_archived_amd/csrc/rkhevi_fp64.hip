@@ -1247,6 +1247,9 @@ __global__ void __launch_bounds__(VR_THREADS) k_ver_waves_run(DevGrid G, DevStat
     constexpr int L = LT, NI = LT - 1;
     constexpr int NW = VR_THREADS/VR_NCOL;
     constexpr int PER1 = (NI + NW - 1)/NW, PER3 = (L + NW - 1)/NW;
+    /* every wave's run of rows and of boxes starts inside the column, so
+       each of its first loads is in range */
+    static_assert((NW - 1)*PER1 < NI && (NW - 1)*PER3 < L, "a wave of the block would have an empty run");
     const int nsh = G.nsh, vpl = G.vpl;
     const i64 gi = (i64)G.n_cond*G.s;
     const double dt = K.dt, iw = K.impl_weight;
@@ -1489,45 +1492,50 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
             fi = jj < L - 1 ? wo : 0.0;
             fr = jj < L - 1 ? doi*wn : 0.0;
         };
-        double fi_m = 0.0, fr_m = 0.0;
-        if (j0 > 0) { const In xm = load(j0 - 1); iface(xm, j0 - 1, fi_m, fr_m); }
-        In cur = load(j0);
+        /* a wave whose run starts past the last layer (more waves than runs
+           when L is small against the waves of the block) has no row: it
+           loads nothing */
+        if (j0 < j1) {
+            double fi_m = 0.0, fr_m = 0.0;
+            if (j0 > 0) { const In xm = load(j0 - 1); iface(xm, j0 - 1, fi_m, fr_m); }
+            In cur = load(j0);
 #pragma unroll
-        for (int t = 0; t < PER_MAX; ++t) {
-            const bool valid = j0 + t < j1;
-            const int j = valid ? j0 + t : j1 - 1;
-            const In nxt = load(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1);
-            const int idx = j*GD_NCOL + col;
-            double fi, fr;
-            iface(cur, j, fi, fr);
-            const double q = cur.q, v = cur.v;   /* q = iw*dt/vol[b] */
-            double d;
-            if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - q*fi;
-            else if (j == L - 1) {
-                d = (fi_m >= 0.0) ? 1.0 + q*fi_m : 1.0;
-                if (k < ncc) d += iw*vsink*dt*a_sfc/v;
-            } else {
-                d = 1.0;
-                if (fi_m >= 0.0) d += q*fi_m;
-                if (fi < 0.0) d -= q*fi;
+            for (int t = 0; t < PER_MAX; ++t) {
+                const bool valid = j0 + t < j1;
+                const int j = valid ? j0 + t : j1 - 1;
+                const In nxt = load(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1);
+                const int idx = j*GD_NCOL + col;
+                double fi, fr;
+                iface(cur, j, fi, fr);
+                const double q = cur.q, v = cur.v;   /* q = iw*dt/vol[b] */
+                double d;
+                if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - q*fi;
+                else if (j == L - 1) {
+                    d = (fi_m >= 0.0) ? 1.0 + q*fi_m : 1.0;
+                    if (k < ncc) d += iw*vsink*dt*a_sfc/v;
+                } else {
+                    d = 1.0;
+                    if (fi_m >= 0.0) d += q*fi_m;
+                    if (fi < 0.0) d -= q*fi;
+                }
+                double r = cur.rb + dt*cur.tb;
+                if (j == 0) r += ew*dt*fr/v;
+                else if (j == L - 1) {
+                    r += -ew*dt*fr_m/v;
+                    if (k < ncc) r += -ew*vsink*dt*r_low*a_sfc/v;
+                } else r += ew*dt*(-fr_m + fr)/v;
+                /* off-diagonals of interface j (column_solvers.c:475-488);
+                   -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
+                double c = 0.0, e = 0.0;
+                if (j < L - 1) {
+                    if (fi >= 0.0) e = (-q)*fi;
+                    else c = cur.qlo*fi;
+                }
+                if (valid) { cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; }
+                fi_m = fi;
+                fr_m = fr;
+                cur = nxt;
             }
-            double r = cur.rb + dt*cur.tb;
-            if (j == 0) r += ew*dt*fr/v;
-            else if (j == L - 1) {
-                r += -ew*dt*fr_m/v;
-                if (k < ncc) r += -ew*vsink*dt*r_low*a_sfc/v;
-            } else r += ew*dt*(-fr_m + fr)/v;
-            /* off-diagonals of interface j (column_solvers.c:475-488);
-               -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
-            double c = 0.0, e = 0.0;
-            if (j < L - 1) {
-                if (fi >= 0.0) e = (-q)*fi;
-                else c = cur.qlo*fi;
-            }
-            if (valid) { cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; }
-            fi_m = fi;
-            fr_m = fr;
-            cur = nxt;
         }
     }
     __syncthreads();
