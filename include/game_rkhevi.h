@@ -245,7 +245,12 @@ int game_gpu_linear_combine_two_states(int slot_0, int slot_1, int slot_out,
                                        double coeff_0, double coeff_1);
 
 /* The reference time loop body (src/coordinator.c:302-390 minus output):
-   for n steps: copy new->old, manage_rkhevi, clear the first-step flag. */
+   for n steps: copy new->old, manage_rkhevi, clear the first-step flag.
+   The copy is an exchange of the two state buffers (the step's inputs are
+   the same).  Every step but a first one replays a HIP graph of the step's
+   launches, captured on first use for each assignment of the two buffers
+   and rebuilt when delta_t or the configuration changes; with kernel timing
+   on, or GAME_GPU_NO_GRAPH set at init, the steps launch eagerly (same bits). */
 int game_gpu_run_steps(int n_steps, double delta_t, int first_step_bool);
 
 /* inner_product (src/spatial_operators/inner_product.c:13-43) of two host
