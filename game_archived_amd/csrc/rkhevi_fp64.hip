@@ -1226,8 +1226,8 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
     }
 }
 
-/* The 26-layer sound-wave solve with each wave walking a run of layers of a
-   block of VR_NCOL columns (lane = column): a row of the three-band system
+/* The 26- and 55-layer sound-wave solve with each wave walking a run of
+   layers of a block of vr_ncol(L) columns (lane = column): a row of the three-band system
    needs the layers above and below its interface, so walking down a run
    forms each layer's linearisation (column_layer, nine divisions at rk 1)
    once instead of twice and loads each box once; the inputs of the next
@@ -1236,14 +1236,18 @@ __global__ void __launch_bounds__(VW_THREADS) k_ver_waves_3p(DevGrid G, DevState
    runs of boxes the same way, carrying the box above's values (its new
    density is the one its own step formed).  Every quantity keeps the
    arithmetic of k_ver_waves. */
-#define VR_NCOL 64
 #define VR_THREADS 256
+/* columns per block: 64 at 26 layers (50 KiB of LDS, three blocks per CU),
+   32 at 55 layers (55 KiB, two blocks per CU; the lanes of a column group
+   are half a wave) */
+__host__ __device__ constexpr int vr_ncol(int lt) { return lt == 26 ? 64 : 32; }
 
 template <int LT, typename TWIND>
 __global__ void __launch_bounds__(VR_THREADS) k_ver_waves_run(DevGrid G, DevState so, DevState snr, DevState sn,
                                                               DevState td, const TWIND *td_wind, StepConsts K, int rk)
 {
     extern __shared__ double lds[];
+    constexpr int VR_NCOL = vr_ncol(LT);
     constexpr int L = LT, NI = LT - 1;
     constexpr int NW = VR_THREADS/VR_NCOL;
     constexpr int PER1 = (NI + NW - 1)/NW, PER3 = (L + NW - 1)/NW;
@@ -2214,10 +2218,14 @@ static void ver_waves_t(const DevGrid &G, const DevState &so, const DevState &sn
         else LAUNCH("k_ver_waves", (k_ver_waves<0, TWIND>), g, b, st, G, so, snr, sn, td, td_wind, K, rk);
         return;
     }
-    if (G.L == 26 && !G.vw_rows) {
-        dim3 g((G.nsh + VR_NCOL - 1)/VR_NCOL), b(VR_THREADS);
-        size_t lds = (size_t)4*(G.L - 1)*VR_NCOL*sizeof(double);   /* 50 KiB */
-        LAUNCH_SHM("k_ver_waves", (k_ver_waves_run<26, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
+    if ((G.L == 26 || G.L == 55) && !G.vw_rows) {
+        const int ncol = vr_ncol(G.L);
+        dim3 g((G.nsh + ncol - 1)/ncol), b(VR_THREADS);
+        size_t lds = (size_t)4*(G.L - 1)*ncol*sizeof(double);   /* 50 KiB at 26 layers, 55 KiB at 55 */
+        if (G.L == 26)
+            LAUNCH_SHM("k_ver_waves", (k_ver_waves_run<26, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
+        else
+            LAUNCH_SHM("k_ver_waves", (k_ver_waves_run<55, TWIND>), g, b, lds, st, G, so, snr, sn, td, td_wind, K, rk);
         return;
     }
     dim3 g((G.nsh + VW_NCOL - 1)/VW_NCOL), b(VW_THREADS);
