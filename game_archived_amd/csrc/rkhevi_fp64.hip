@@ -1872,7 +1872,10 @@ __device__ __forceinline__ void phase_trans_point(const double *r, double T, con
     double b = GC_R_V*T - GC_R_V*rk5*pth/cvm + ef*ds*pth/cvm;
     double cc = wvp - sp;
     double p = b/a, q = cc/a;
-    double dd = -0.5*p - pow(0.25*(p*p) - q, 0.5);
+    /* the reference's pow(x, 0.5) (phase_trans.c:92-99): the square root,
+       which IEEE rounds correctly (glibc's pow is within 0.52 ulp of it);
+       the discriminant is never -0.0 or -inf, where the two differ */
+    double dd = -0.5*p - sqrt(0.25*(p*p) - q);
     double r0, r1, r2, r3, r4, heat;
     if (sp >= wvp) {
         if (warm) {
