@@ -143,13 +143,17 @@ __device__ __forceinline__ double c_p_water(double t)
     double tc = t - GC_T_0, r;
     /* a branch, not a select: a wave whose boxes are all above or all below
        freezing evaluates one of the two polynomials */
+    /* the reference's integer powers pow(x, 3.0) and pow(x, 4.0) as products
+       (within about one ulp of glibc's pow, as the device's pow is) */
     if (tc >= 0.0) {
         if (tc > 35.0) tc = 35.0;
         double a = tc - 35.0;
-        r = 0.9979 + 3.1e-6*(a*a) + 3.8e-9*pow(a, 4.0);
+        const double a2 = a*a;
+        r = 0.9979 + 3.1e-6*(a*a) + 3.8e-9*(a2*a2);
     } else {
         if (tc < -37.0) tc = -37.0;
-        r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*pow(tc, 3.0) + 2.7136e-7*pow(tc, 4.0);
+        const double t2 = tc*tc;
+        r = 1.000938 - 2.7052e-3*tc - 2.3235e-5*(tc*tc) + 4.3778e-6*(t2*tc) + 2.7136e-7*(t2*t2);
     }
     return 4186.8*r;
 }
