@@ -1486,6 +1486,20 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
             x.v = vol[b]; x.rb = rho_o[b]; x.tb = trho[b];
             return x;
         };
+        /* the same for row j + 1 of the walk, given row j: its density above
+           the interface, its own density and its quotient are row j's
+           density below and quotient below (for the last row the first is
+           not used), so three of the ten values come from registers */
+        auto load_next = [&](int j, const In &prev) {
+            In x;
+            const int jc = j < L - 1 ? j : L - 2;
+            const i64 wi = i + (i64)(jc + 1)*vpl, bc = i + (i64)jc*nsh, b = i + (i64)j*nsh;
+            x.wo = so.wind[wi]; x.wn = sn.wind[wi]; x.a = area[wi];
+            x.dlo = rho_o[bc + nsh]; x.dup = prev.dlo;
+            x.q = prev.qlo; x.qlo = idv[j < L - 1 ? b + nsh : b];
+            x.v = vol[b]; x.rb = prev.dlo; x.tb = trho[b];
+            return x;
+        };
         /* interface fluxes (column_solvers.c:414-465) */
         auto iface = [&](const In &x, int jj, double &fi, double &fr) {
             double wo = x.wo, wn = x.wn;
@@ -1507,7 +1521,8 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
             for (int t = 0; t < PER_MAX; ++t) {
                 const bool valid = j0 + t < j1;
                 const int j = valid ? j0 + t : j1 - 1;
-                const In nxt = load(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1);
+                /* past the run's end the walk repeats its last row (unused) */
+                const In nxt = load_next(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1, cur);
                 const int idx = j*GD_NCOL + col;
                 double fi, fr;
                 iface(cur, j, fi, fr);
