@@ -683,7 +683,9 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
     }
     int l0, l1;
     layer_chunk(L, bk.y, l0, l1);
-    for (int l = l0; l < l1; ++l) {
+    /* one layer's tendencies: with NC known they are returned in orho/ort
+       (stored by the caller), otherwise stored here */
+    auto layer = [&](int l, double *orho, double &ort) {
         const i64 i = (i64)l*nsh + h;
         double a[6], u[6];
 #pragma unroll
@@ -808,9 +810,33 @@ __global__ void __launch_bounds__(TPB) k_scalar_tend(DevGrid G, DevState s, DevW
         }
         if (NC > 0) {
 #pragma unroll
-            for (int k = 0; k < (NC > 0 ? NC : 1); ++k) td_rho[(i64)k*S + i] = out_rho[k];
-        }
-        td_rt[i] = out_rt;
+            for (int k = 0; k < (NC > 0 ? NC : 1); ++k) orho[k] = out_rho[k];
+            ort = out_rt;
+        } else td_rt[i] = out_rt;
+    };
+    auto store = [&](int l, const double *orho, double ort) {
+        const i64 i = (i64)l*nsh + h;
+#pragma unroll
+        for (int k = 0; k < (NC > 0 ? NC : 1); ++k) td_rho[(i64)k*S + i] = orho[k];
+        td_rt[i] = ort;
+    };
+    constexpr int RUN = 3;
+    if (NC > 0 && l1 - l0 <= RUN) {
+        /* a short chunk: the loads and arithmetic of all its layers come
+           before any store, so no store orders a later layer's loads (a step
+           past the chunk repeats its last layer and stores nothing) */
+        double orho[RUN][NC > 0 ? NC : 1], ort[RUN];
+#pragma unroll
+        for (int t = 0; t < RUN; ++t) layer(l0 + t < l1 ? l0 + t : l1 - 1, orho[t], ort[t]);
+#pragma unroll
+        for (int t = 0; t < RUN; ++t)
+            if (l0 + t < l1) store(l0 + t, orho[t], ort[t]);
+        return;
+    }
+    for (int l = l0; l < l1; ++l) {
+        double orho[NC > 0 ? NC : 1], ort = 0.0;
+        layer(l, orho, ort);
+        if (NC > 0) store(l, orho, ort);
     }
 }
 
