@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
     layer_chunk(L + 1, bk.y, l0, l1);
     double Fm[6], Wm[6];
     load_layer(l0 >= 1 ? l0 - 1 : 0, Fm, Wm);
-    for (int l = l0; l < l1; ++l) {
+    auto level = [&](int l) {
         double Fc[6], Wc[6], q[6];
         load_layer(l <= L - 1 ? l : L - 1, Fc, Wc);
 #pragma unroll
@@ -508,10 +508,22 @@ __global__ void __launch_bounds__(TPB) k_vorticity_flux_w(DevGrid G, const TW *F
                 o += t;
             }
         }
-        out[(i64)l*vpl + h] = (TW)o;
 #pragma unroll
         for (int j = 0; j < 6; ++j) { Fm[j] = Fc[j]; Wm[j] = Wc[j]; }
+        return o;
+    };
+    if (l1 - l0 <= 3) {
+        /* a short chunk: every level before any store, so no store orders a
+           later level's loads (a step past the chunk repeats its last level) */
+        double o3[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) o3[t] = level(l0 + t < l1 ? l0 + t : l1 - 1);
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            if (l0 + t < l1) out[(i64)(l0 + t)*vpl + h] = (TW)o3[t];
+        return;
     }
+    for (int l = l0; l < l1; ++l) out[(i64)l*vpl + h] = (TW)level(l);
 }
 
 /* inner_product.c:13-43, one thread per cell, all layers */
@@ -531,8 +543,7 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
     layer_chunk(G.L, bk.y, l0, l1);
     /* the w point below layer l is the one above layer l + 1: carried */
     double au = a[h + (i64)l0*G.vpl], bu = b[h + (i64)l0*G.vpl];
-    for (int l = l0; l < l1; ++l) {
-        i64 i = (i64)l*G.nsh + h;
+    auto layer = [&](int l) {
         const double *w = G.ipwT + (i64)l*8*G.nsh + h;   /* weight j at w[j*nsh] */
         const double al = a[h + (i64)(l + 1)*G.vpl], bl = b[h + (i64)(l + 1)*G.vpl];
         double o = 0;
@@ -545,10 +556,22 @@ __global__ void __launch_bounds__(TPB) k_inner_product(DevGrid G, const double *
         }
         o += w[6*(i64)G.nsh]*au*bu;
         o += w[7*(i64)G.nsh]*al*bl;
-        out[i] = (TW)o;
         au = al;
         bu = bl;
+        return o;
+    };
+    if (l1 - l0 <= 3) {
+        /* a short chunk: every layer before any store, so no store orders a
+           later layer's loads (a step past the chunk repeats its last layer) */
+        double o3[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) o3[t] = layer(l0 + t < l1 ? l0 + t : l1 - 1);
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            if (l0 + t < l1) out[(i64)(l0 + t)*G.nsh + h] = (TW)o3[t];
+        return;
     }
+    for (int l = l0; l < l1; ++l) out[(i64)l*G.nsh + h] = (TW)layer(l);
 }
 
 /* grad (gradient_operators.c:56-74) of a scalar field: edges (one thread per
