@@ -29,6 +29,13 @@
 #define R_D_REF 287.057811
 
 static char g_err[256];
+
+/* the error flag of a parallel construction loop: threads record the first
+   failure they meet and skip their remaining work; atomic accesses keep the
+   flag free of data races (any code a thread records makes the loop fail) */
+#define SET_ERR(e, v) do { _Pragma("omp atomic write") (e) = (v); } while (0)
+static inline int get_err_(const int *e) { int v; _Pragma("omp atomic read") v = *e; return v; }
+#define GET_ERR(e) get_err_(&(e))
 const char *game_grid_last_error(void) { return g_err; }
 #define FAIL(code, ...) do { snprintf(g_err, sizeof g_err, __VA_ARGS__); return (code); } while (0)
 
@@ -901,7 +908,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
     double bad_ratio = 0.0;   /* the failing sum ratio of check 30, for the message */
     #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < nvh; ++i) {
-        if (err) continue;
+        if (GET_ERR(err)) continue;
         int offset = 0;
         for (int k = 0; k < 10; ++k) {
             if (k == 0 || k == 5) offset = 0;
@@ -909,7 +916,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
             if (k < 5) { index_offset = 0; sign_0 = -1; cell = from[i]; }
             else { index_offset = 5; sign_0 = 1; cell = to[i]; }
             if (adj[6*cell + k - index_offset] == i) offset += 1;
-            if (offset > 1) { err = 21; break; }
+            if (offset > 1) { SET_ERR(err, 21); break; }
             ti[10*i + k] = adj[6*cell + k - index_offset + offset];
             if (ti[10*i + k] == -1) { tw[10*i + k] = 0; continue; }
             int sign_1 = (from[ti[10*i + k]] == cell) ? 1 : -1;
@@ -923,7 +930,7 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
                 if (!in_list(a, vi, ne)) { vi[counter] = a; lv[counter] = lat_d[a]; ov[counter] = lon_d[a]; ++counter; }
                 if (!in_list(b, vi, ne)) { vi[counter] = b; lv[counter] = lat_d[b]; ov[counter] = lon_d[b]; ++counter; }
             }
-            if (counter != ne) { err = 22; break; }
+            if (counter != ne) { SET_ERR(err, 22); break; }
             sort_edge_indices(lv, ov, ne, srt);
             for (int l = 0; l < ne; ++l) vr[l] = vi[srt[l]];
             for (int l = 0; l < ne; ++l)
@@ -946,8 +953,9 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
                 check_sum += areas[l];
             }
             if (fabs(check_sum/(rescale_2d*area[cell]) - 1) > tol) {
+                #pragma omp critical(gridgen_trsk_ratio)
                 bad_ratio = check_sum/(rescale_2d*area[cell]);
-                err = 30;
+                SET_ERR(err, 30);
                 break;
             }
             int first_index = -1, last_index = -1;
@@ -963,11 +971,11 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
                 for (int l = 0; l <= last_index; ++l) sum += areas[l];
             }
             sum = sum/(rescale_2d*area[cell]);
-            if (sum < 0 || sum > 1) { err = 34; break; }
+            if (sum < 0 || sum > 1) { SET_ERR(err, 34); break; }
             tw[10*i + k] = sign_0*(sum - 0.5)*sign_1;
             tw[10*i + k] = -rescale_1d*nd_dual[ti[10*i + k]]/nd[nsh + i]*tw[10*i + k];
         }
-        if (err) continue;
+        if (GET_ERR(err)) continue;
         /* Gassmann (2018) resorting, coriolis.c:223-378 */
         int pre_i[10];
         double pre_w[10];
@@ -994,24 +1002,24 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
             }
             if (used_n != ne - 1) { err = 42 + side; break; }
         }
-        if (err) continue;
+        if (GET_ERR(err)) continue;
         int *m = &tmc[10*i];
         int *t = &ti[10*i];
         m[0] = (to[i] < N_PENT) ? t[8] : t[9];
         m[1] = t[0];
         if (from[i] < N_PENT) {
             m[2] = t[3]; m[3] = t[5]; m[4] = 0;
-            if (tw[10*i + 4] != 0) { err = 40; continue; }
+            if (tw[10*i + 4] != 0) { SET_ERR(err, 40); continue; }
         } else { m[2] = t[2]; m[3] = t[4]; m[4] = t[5]; }
         m[5] = (from[i] < N_PENT) ? t[3] : t[4];
         m[6] = t[5];
         if (to[i] < N_PENT) {
             m[7] = t[8]; m[8] = t[0]; m[9] = 0;
-            if (tw[10*i + 9] != 0) { err = 41; continue; }
+            if (tw[10*i + 9] != 0) { SET_ERR(err, 41); continue; }
         } else { m[7] = t[7]; m[8] = t[9]; m[9] = t[0]; }
         for (int j = 0; j < 10; ++j)
             for (int k = j + 1; k < 10; ++k)
-                if (t[j] == t[k] && tw[10*i + j] != 0 && tw[10*i + k] != 0) err = 29;
+                if (t[j] == t[k] && tw[10*i + j] != 0 && tw[10*i + k] != 0) SET_ERR(err, 29);
     }
     if (err == 30)
         FAIL(err, "TRSK construction failed (reference problem code 30: kite areas / cell area = %.17g, "
@@ -1026,9 +1034,9 @@ static int coriolis(const game_dims_t *dm, game_grid_view_t *g, game_dualgrid_vi
             double v0 = nd[nsh + i]/(rescale_1d*nd_dual[fi])*tw[10*i + j];
             int second = -1;
             for (int k = 0; k < 10; ++k) if (ti[10*fi + k] == i) second = 10*fi + k;
-            if (second == -1) { err = 38; continue; }
+            if (second == -1) { SET_ERR(err, 38); continue; }
             double v1 = nd[nsh + fi]/(rescale_1d*nd_dual[i])*tw[second];
-            if (fabs(v0 + v1) > tol) err = 39;
+            if (fabs(v0 + v1) > tol) SET_ERR(err, 39);
         }
     }
     if (err) FAIL(err, "TRSK antisymmetry check failed (reference problem code %d)", err);
@@ -1050,7 +1058,7 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
     int err = 0;
     #pragma omp parallel for
     for (int i = 0; i < nvh; ++i) {
-        if (err) continue;
+        if (GET_ERR(err)) continue;
         int pre[6], list[4], dbl[2] = {-1, -1};
         for (int j = 0; j < 3; ++j) pre[j] = vit[3*to_d[i] + j];
         for (int j = 0; j < 3; ++j) pre[3 + j] = vit[3*from_d[i] + j];
@@ -1060,7 +1068,7 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
         int c = 0;
         for (int j = 0; j < 6; ++j)
             if (j != dbl[0] && j != dbl[1]) { if (c < 4) list[c] = pre[j]; c++; }
-        if (c != 4) { err = 51; continue; }
+        if (c != 4) { SET_ERR(err, 51); continue; }
         for (int j = 0; j < 4; ++j) vir[4*i + j] = list[j];
         int dpre[4] = {-1, -1, -1, -1}, cc = 0;
         for (int j = 0; j < 4; ++j) {
@@ -1069,7 +1077,7 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
             cand = to[vir[4*i + j]];
             if (!in_list(cand, dpre, 4)) { if (cc < 4) dpre[cc] = cand; ++cc; }
         }
-        if (cc != 4) { err = 52; continue; }
+        if (cc != 4) { SET_ERR(err, 52); continue; }
         for (int j = 0; j < 4; ++j) dri[4*i + j] = dpre[j];
         double rh_area = dg->area[nvh + from_d[i]] + dg->area[nvh + to_d[i]];
         int first_case = 0, second_case = 0;
@@ -1103,13 +1111,13 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
                 drw[4*i + j] = pow(radius + g->z_vector[nsh], 2)*(t0 + t1)/rh_area;
             }
         }
-        if (first_case != 2 || second_case != 2) { err = 53; continue; }
+        if (first_case != 2 || second_case != 2) { SET_ERR(err, 53); continue; }
         double check = 0;
         for (int j = 0; j < 4; ++j) {
             check += drw[4*i + j];
-            if (drw[4*i + j] <= 0 || drw[4*i + j] >= 1) err = 54;
+            if (drw[4*i + j] <= 0 || drw[4*i + j] >= 1) SET_ERR(err, 54);
         }
-        if (fabs(check - 1) > rh_tol) { err = 55; fprintf(stderr, "rhombus %d sum %.17g w %g %g %g %g\n", i, check, drw[4*i], drw[4*i+1], drw[4*i+2], drw[4*i+3]); }
+        if (fabs(check - 1) > rh_tol) { SET_ERR(err, 55); fprintf(stderr, "rhombus %d sum %.17g w %g %g %g %g\n", i, check, drw[4*i], drw[4*i+1], drw[4*i+2], drw[4*i+3]); }
     }
     if (err) FAIL(err, "rhombus weights failed (code %d)", err);
     return 0;

@@ -2079,14 +2079,15 @@ __device__ __forceinline__ void neumaier_merge(double &s, double &c, double s2, 
    a wavefront are merged by butterfly shuffles (six steps of __shfl_xor, no
    LDS), the four wavefronts of the block through LDS; the host adds the
    block partials, also compensated. */
-__global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *x, const double *y, const double *vol,
+template <typename TY>
+__global__ void __launch_bounds__(TPB) k_integral_partials(i64 n, const double *x, const TY *y, const double *vol,
                                                            double *partials)
 {
     __shared__ double ss[TPB/64], sc[TPB/64];
     double s = 0.0, c = 0.0;
     for (i64 i = (i64)blockIdx.x*TPB + threadIdx.x; i < n; i += (i64)gridDim.x*TPB) {
         double v = x[i]*vol[i];
-        if (y) v = x[i]*y[i]*vol[i];
+        if (y) v = x[i]*(double)y[i]*vol[i];
         neumaier_merge(s, c, v, 0.0);
     }
 #pragma unroll
@@ -2385,9 +2386,16 @@ void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, cons
 
 int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st)
 {
-    int blocks = 1024;
-    LAUNCH("k_integral_partials", k_integral_partials, dim3(blocks), dim3(TPB), st, n, x, y, vol, partials);
-    return blocks;
+    LAUNCH("k_integral_partials", k_integral_partials<double>, dim3(INTEGRAL_BLOCKS), dim3(TPB), st, n, x, y, vol,
+           partials);
+    return INTEGRAL_BLOCKS;
+}
+
+int integral_partials(i64 n, const double *x, const float *y, const double *vol, double *partials, hipStream_t st)
+{
+    LAUNCH("k_integral_partials", k_integral_partials<float>, dim3(INTEGRAL_BLOCKS), dim3(TPB), st, n, x, y, vol,
+           partials);
+    return INTEGRAL_BLOCKS;
 }
 
 }  // namespace col

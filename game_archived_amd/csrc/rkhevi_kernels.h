@@ -62,7 +62,11 @@ void linear_combine(const DevGrid &G, const DevState &a, const DevState &b, cons
                     hipStream_t st);
 /* G.idv[b] = (0.5 dt)/vol[b] */
 void tracer_quotients(const DevGrid &G, double dt, hipStream_t st);
+/* per-block compensated (sum, error) pairs of sum_i x[i]*y[i]*vol[i] (y may be
+   null: sum_i x[i]*vol[i]) into partials[0 .. 2*INTEGRAL_BLOCKS) */
+const int INTEGRAL_BLOCKS = 1024;
 int integral_partials(i64 n, const double *x, const double *y, const double *vol, double *partials, hipStream_t st);
+int integral_partials(i64 n, const double *x, const float *y, const double *vol, double *partials, hipStream_t st);
 }
 
 #endif

@@ -28,6 +28,10 @@ namespace {
 
 char g_err[512];
 
+/* integrals formed by one game_gpu_integrals call: the densities (at most six
+   constituents), rhotheta_v and three energies */
+const int INTEGRALS_MAX = 16;
+
 #define HIP_OK(call)                                                                        \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
@@ -190,7 +194,7 @@ long long game_gpu_bytes_allocated(void) { return M ? M->bytes : 0; }
 static void drop_graphs(void)
 {
     /* a replay still in flight finishes before its executable graph goes */
-    if (M->gexec[0] || M->gexec[1]) (void)hipStreamSynchronize(M->stream);
+    if ((M->gexec[0] || M->gexec[1]) && M->stream) (void)hipStreamSynchronize(M->stream);
     for (int p = 0; p < 2; ++p) {
         if (M->gexec[p]) (void)hipGraphExecDestroy(M->gexec[p]);
         M->gexec[p] = nullptr;
@@ -202,23 +206,36 @@ static void drop_graphs(void)
 int game_gpu_finalize(void)
 {
     if (!M) return 0;
-    (void)hipStreamSynchronize(M->stream);
+    if (M->stream) (void)hipStreamSynchronize(M->stream);
     drop_graphs();
     for (void *p : M->allocs) (void)hipFree(p);
     if (M->h_partials) (void)hipHostFree(M->h_partials);
     for (auto e : M->timer.pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(M->stream);
+    if (M->stream) (void)hipStreamDestroy(M->stream);
     g_kernel_timer = nullptr;
     delete M;
     M = nullptr;
     return 0;
 }
 
+static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const game_dualgrid_view_t *dg,
+                      const game_config_t *config, int precision);
+
+/* M is either fully built or null: a failure part-way through the set-up
+   releases what it had allocated, keeping the error message */
 int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game_dualgrid_view_t *dg,
                   const game_config_t *config, int precision)
 {
     g_err[0] = 0;
     if (M) game_gpu_finalize();
+    int rc = init_model(dims, g, dg, config, precision);
+    if (rc && M) game_gpu_finalize();
+    return rc;
+}
+
+static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const game_dualgrid_view_t *dg,
+                      const game_config_t *config, int precision)
+{
     if (!dims || !g || !dg || !config) { snprintf(g_err, sizeof g_err, "null argument"); return GAME_ERR_ARG; }
     game_dims_t d = *dims;
     game_dims_derive(&d);
@@ -374,8 +391,9 @@ int game_gpu_init(const game_dims_t *dims, const game_grid_view_t *g, const game
         if ((rc = dzeros(&W.flux2, 8))) return rc;
         if ((rc = dzeros(&W.flux_divv, S))) return rc;
     }
-    if ((rc = dzeros(&M->partials, 2*4096))) return rc;
-    HIP_OK(hipHostMalloc((void **)&M->h_partials, sizeof(double)*2*4096, hipHostMallocDefault));
+    if ((rc = dzeros(&M->partials, (size_t)2*col::INTEGRAL_BLOCKS*INTEGRALS_MAX))) return rc;
+    HIP_OK(hipHostMalloc((void **)&M->h_partials, sizeof(double)*2*col::INTEGRAL_BLOCKS*INTEGRALS_MAX,
+                         hipHostMallocDefault));
     StepConsts &K = M->K;
     K.impl_weight = config->impl_thermo_weight;
     K.damping_start = config->damping_start_height_over_toa*g->z_vector[0];
@@ -655,6 +673,7 @@ int game_gpu_temperature_diagnostics(int k)
     if (M->precision == GAME_MIXED) col::temperature(M->G, *s, M->WF.temp, (float *)nullptr, nullptr, M->stream);
     else col::temperature(M->G, *s, M->W.temp, (double *)nullptr, nullptr, M->stream);
     HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(M->stream));
     return 0;
 }
 
@@ -671,44 +690,61 @@ int game_gpu_inner_product(const double *a, const double *b, double *out)
     return 0;
 }
 
-static int integral(i64 n, const double *x, const double *y, double *out)
+}  /* extern "C" */
+
+/* the compensated sum of integral q's block partials (host side) */
+static double merge_partials(int q)
 {
-    int blocks = col::integral_partials(n, x, y, M->G.vol, M->partials, M->stream);
-    HIP_OK(hipMemcpyAsync(M->h_partials, M->partials, sizeof(double)*2*(size_t)blocks, hipMemcpyDeviceToHost,
-                          M->stream));
-    HIP_OK(hipStreamSynchronize(M->stream));
+    const double *p = M->h_partials + (size_t)2*col::INTEGRAL_BLOCKS*q;
     double s = 0.0, c = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-        double v = M->h_partials[2*b], t = s + v;
+    for (int b = 0; b < col::INTEGRAL_BLOCKS; ++b) {
+        double v = p[2*b], t = s + v;
         if (fabs(s) >= fabs(v)) c += (s - t) + v;
         else c += (v - t) + s;
         s = t;
-        c += M->h_partials[2*b + 1];
+        c += p[2*b + 1];
     }
-    *out = s + c;
+    return s + c;
+}
+
+/* write_output.c:1515-1596 with compensated sums: every integral's block
+   partials are formed by one launch each, copied back together and merged on
+   the host.  The internal energy multiplies by the diagnostics temperature,
+   as the reference does (diagnostics->temperature, write_output.c:1584-1588):
+   the temperature of the state the last step started from (manage_rkhevi.c:29),
+   or of the slot last passed to game_gpu_temperature_diagnostics. */
+template <class WK>
+static int integrals_t(const WK &W, const DevState &s, double *out)
+{
+    const DevGrid &G = M->G;
+    const int nint = G.n_const + 4;
+    double *P = M->partials;
+    const size_t stride = (size_t)2*col::INTEGRAL_BLOCKS;
+    for (int c = 0; c < G.n_const; ++c)
+        col::integral_partials(G.s, s.rho + (i64)c*G.s, (const double *)nullptr, G.vol, P + stride*c, M->stream);
+    col::integral_partials(G.s, s.rt, (const double *)nullptr, G.vol, P + stride*G.n_const, M->stream);
+    const double *rg = s.rho + (i64)G.n_cond*G.s;
+    hor::inner_product(G, s.wind, s.wind, M->tmp_s, M->stream);
+    col::integral_partials(G.s, rg, M->tmp_s, G.vol, P + stride*(G.n_const + 1), M->stream);
+    col::integral_partials(G.s, rg, G.phi, G.vol, P + stride*(G.n_const + 2), M->stream);
+    col::integral_partials(G.s, rg, W.temp, G.vol, P + stride*(G.n_const + 3), M->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(M->h_partials, P, sizeof(double)*stride*(size_t)nint, hipMemcpyDeviceToHost, M->stream));
+    HIP_OK(hipStreamSynchronize(M->stream));
+    for (int q = 0; q < nint; ++q) out[q] = merge_partials(q);
+    out[G.n_const + 1] *= 0.5;
+    out[G.n_const + 3] *= GC_C_D_V;
     return 0;
 }
 
-/* write_output.c:1515-1596 with compensated sums */
+extern "C" {
+
 int game_gpu_integrals(int k, double *out)
 {
     if (!M) return GAME_ERR_NOT_INIT;
     DevState *s = slot(k);
     if (!s || !out) return GAME_ERR_ARG;
-    const DevGrid &G = M->G;
-    int rc;
-    for (int c = 0; c < G.n_const; ++c)
-        if ((rc = integral(G.s, s->rho + (i64)c*G.s, nullptr, &out[c]))) return rc;
-    if ((rc = integral(G.s, s->rt, nullptr, &out[G.n_const]))) return rc;
-    const double *rg = s->rho + (i64)G.n_cond*G.s;
-    hor::inner_product(G, s->wind, s->wind, M->tmp_s, M->stream);
-    if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 1]))) return rc;
-    out[G.n_const + 1] *= 0.5;
-    if ((rc = integral(G.s, rg, G.phi, &out[G.n_const + 2]))) return rc;
-    col::temperature(G, *s, M->tmp_s, (double *)nullptr, nullptr, M->stream);
-    if ((rc = integral(G.s, rg, M->tmp_s, &out[G.n_const + 3]))) return rc;
-    out[G.n_const + 3] *= GC_C_D_V;
-    return 0;
+    return M->precision == GAME_MIXED ? integrals_t(M->WF, *s, out) : integrals_t(M->W, *s, out);
 }
 
 }  /* extern "C" */

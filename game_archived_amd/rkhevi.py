@@ -66,11 +66,19 @@ class DeviceModel:
         return i
 
     def start(self, state0):
-        """The driver's preparation (src/coordinator.c:292-299): old = state0,
-        new = 1*old + 0*old (which re-derives theta_v_pert), first-step flag set."""
+        """The driver's preparation (src/coordinator.c:250, 292-299): old =
+        state0, its temperature diagnosed (the diagnostics temperature the
+        initial integrals read), new = 1*old + 0*old (which re-derives
+        theta_v_pert), first-step flag set."""
         self.set_state(OLD, state0)
+        self.temperature_diagnostics(OLD)
         self.linear_combine_two_states(OLD, OLD, NEW, 1.0, 0.0)
         self.first_step = 1
+
+    def temperature_diagnostics(self, slot):
+        """temperature_diagnostics (src/constituents/derived_quantities.c:15-40)
+        of a slot into the diagnostics temperature"""
+        self._check(self.lib.game_gpu_temperature_diagnostics(slot))
 
     # -- stepping ---------------------------------------------------------
     def manage_rkhevi(self, dt):
@@ -89,6 +97,10 @@ class DeviceModel:
         self._check(self.lib.game_gpu_synchronize())
 
     def integrals(self, slot=NEW):
+        """write_out_integral's sums (src/io/write_output.c:1515-1596) of a slot:
+        masses per constituent, rhotheta_v, kinetic, potential and internal
+        energy; the internal energy multiplies by the diagnostics temperature
+        (of the state the last step started from), as the reference does"""
         out = np.zeros(self.dims.n_const + 4)
         self._check(self.lib.game_gpu_integrals(slot, N.ptr(out)))
         return out

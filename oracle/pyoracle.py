@@ -95,7 +95,9 @@ class OracleModel:
         self.irrev = Irrev(d)
         self._views = (self.old.view(), self.new.view(), self.tend.view(), self.diag.view(),
                        self.forc.view(), self.irrev.view())
-        # the driver's preparation (src/coordinator.c:292): new = 1*old + 0*old
+        # the driver's preparation (src/coordinator.c:250, 292): the initial
+        # state's temperature into the diagnostics, new = 1*old + 0*old
+        lib().or_temperature_diagnostics(self.c, self._views[0], N.ptr(self.diag.temperature))
         lib().or_linear_combine_two_states(self.c, self._views[0], self._views[0], self._views[1], 1.0, 0.0)
 
     def run(self, n_steps):
@@ -104,9 +106,10 @@ class OracleModel:
             raise RuntimeError(f"oracle step failed ({rc})")
 
     def integrals(self, naive=False):
+        """write_out_integral (src/io/write_output.c:1515-1596) of the new state;
+        the internal energy reads diagnostics->temperature (the temperature of
+        the state the last step started from), as the reference does"""
         d = self.grid.dims
-        T = np.zeros(d.s)
-        lib().or_temperature_diagnostics(self.c, self._views[1], N.ptr(T))
         out = np.zeros(d.n_const + 4)
-        lib().or_integrals(self.c, self._views[1], N.ptr(T), N.ptr(out), 1 if naive else 0)
+        lib().or_integrals(self.c, self._views[1], N.ptr(self.diag.temperature), N.ptr(out), 1 if naive else 0)
         return out

@@ -39,3 +39,25 @@ def oracle_state(grid, moist=False, seed=20261202):
 def rel_err(a, b):
     scale = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale)
+
+
+CONSTITUENT_NAMES = ("snow", "rain", "cloud_ice", "cloud_water", "gas", "vapour")
+
+
+def state_errors(dims, a, b, fields=("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind")):
+    """max |a - b| / max |b| per prognostic field, with the densities compared
+    one constituent at a time (each against its own maximum: a condensate
+    error is not hidden under the scale of the gas density)"""
+    out = {}
+    for f in fields:
+        x, y = getattr(a, f), getattr(b, f)
+        if f == "rho":
+            names = CONSTITUENT_NAMES if dims.n_const == 6 else ("gas",)
+            for k, name in enumerate(names):
+                sl = slice(k * dims.s, (k + 1) * dims.s)
+                yk = y[sl]
+                # an all-zero constituent (a dry-started condensate) must stay exactly zero
+                out[f"rho_{name}"] = rel_err(x[sl], yk) if np.abs(yk).max() > 0 else float(np.abs(x[sl]).max())
+        else:
+            out[f] = rel_err(x, y)
+    return out

@@ -58,3 +58,27 @@ def test_dropin_library_exports_reference_entry_points(layout):
     assert not missing, missing
     lib.game_abi_fingerprint.restype = C.c_ulonglong
     assert lib.game_abi_fingerprint() != 0
+
+
+def test_failed_init_leaves_no_model():
+    """A game_gpu_init that fails (a physics switch outside the path, or no HIP
+    device on this host) leaves no half-built model behind: every later entry
+    point answers GAME_ERR_NOT_INIT instead of launching on missing buffers."""
+    import numpy as np
+    from conftest import get_grid
+    from game_archived_amd import _native as N
+    lib = N.gpu_lib()
+    g = get_grid(3, "flat")
+    bad = N.default_config()
+    bad.rad_on = 1
+    assert lib.game_gpu_init(g.dims, g.grid_view(), g.dualgrid_view(), bad, 0) == 5   # GAME_ERR_UNSUPPORTED
+    assert b"physics switch" in lib.game_gpu_last_error()
+    ok = N.default_config()
+    rc = lib.game_gpu_init(g.dims, g.grid_view(), g.dualgrid_view(), ok, 7)
+    assert rc == 2                                                                    # GAME_ERR_ARG: no such form
+    out = np.zeros(g.dims.n_const + 4)
+    assert lib.game_gpu_run_steps(1, 60.0, 1) == 3                                    # GAME_ERR_NOT_INIT
+    assert lib.game_gpu_manage_rkhevi(60.0, 1) == 3
+    assert lib.game_gpu_integrals(1, N.ptr(out)) == 3
+    assert lib.game_gpu_bytes_allocated() == 0
+    assert lib.game_gpu_finalize() == 0

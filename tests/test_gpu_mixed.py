@@ -70,4 +70,12 @@ def test_mixed_moist_close_to_fp64():
     # transitions, conserved in sum up to the sedimentation out of the bottom
     tot_m = sum(m1[:d.n_const]) - sum(m0[:d.n_const])
     tot_r = sum(r1[:d.n_const]) - sum(r0[:d.n_const])
+    total = sum(r0[:d.n_const])
     print("total mass change mixed / fp64", tot_m, tot_r)
+    # both forms move the same mass out of the bottom: the mixed form's change
+    # is the FP64 form's up to its own relative error, plus the conservation floor
+    assert abs(tot_m - tot_r) <= 1e-13 * total + 1e-4 * abs(tot_r), (tot_m, tot_r)
+    # the dry-air mass (gas minus vapour) is conserved in both forms
+    for i0, i1 in ((m0, m1), (r0, r1)):
+        dry0, dry1 = i0[d.n_cond] - i0[d.n_cond + 1], i1[d.n_cond] - i1[d.n_cond + 1]
+        assert abs(dry1 - dry0) <= 1e-13 * dry0, (dry0, dry1)
