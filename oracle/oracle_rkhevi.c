@@ -365,6 +365,14 @@ static double ref_z_vector_read(const or_ctx_t *c, long long k)
     abort();
 }
 
+double or_density_to_rhombus(const or_ctx_t *c, const double *rho, int l, int e)
+{   /* vorticities.c:33-43 */
+    double dv = 0;
+    for (int j = 0; j < 4; ++j)
+        dv += c->g.density_to_rhombi_weights[4*e + j]*rho[(long long)l*NSH + c->g.density_to_rhombi_indices[4*e + j]];
+    return dv;
+}
+
 void or_calc_pot_vort(const or_ctx_t *c, const double *w, const double *rho, game_diag_view_t *dg_)
 {   /* vorticities.c:18-85 */
     or_calc_rel_vort(c, w, dg_);
@@ -376,10 +384,7 @@ void or_calc_pot_vort(const or_ctx_t *c, const double *w, const double *rho, gam
         int l = (int)(i/(2*NVH)), h = (int)(i - (long long)l*2*NVH);
         double dv;
         if (h >= NVH) {
-            int e = h - NVH;
-            dv = 0;
-            for (int j = 0; j < 4; ++j)
-                dv += c->g.density_to_rhombi_weights[4*e + j]*rho[(long long)l*NSH + c->g.density_to_rhombi_indices[4*e + j]];
+            dv = or_density_to_rhombus(c, rho, l, h - NVH);
         } else {
             if (l == 0) {
                 dv = 0.5*(rho[fr[h]] + rho[to[h]])

@@ -41,6 +41,9 @@ void or_calc_rel_vort_on_triangles(const or_ctx_t *c, const double *wind, double
 void or_calc_rel_vort(const or_ctx_t *c, const double *wind, game_diag_view_t *diag);
 void or_add_f_to_rel_vort(const or_ctx_t *c, const double *rel_vort, double *out);
 void or_calc_pot_vort(const or_ctx_t *c, const double *wind, const double *density, game_diag_view_t *diag);
+/* the density calc_pot_vort divides the rhombus PV of edge h in layer by
+   (vorticities.c:33-43) */
+double or_density_to_rhombus(const or_ctx_t *c, const double *density, int layer, int h);
 void or_vorticity_flux(const or_ctx_t *c, const double *flux, const double *pv, double *out);
 void or_linear_combine_two_states(const or_ctx_t *c, const game_state_view_t *s0, const game_state_view_t *s1,
                                   game_state_view_t *out, double c0, double c1);
