@@ -221,5 +221,6 @@ def gpu_lib():
         getattr(lib, f"game_gpu_op_{name}").argtypes = [VP] * n
         getattr(lib, f"game_gpu_op_{name}_adjoint").argtypes = [VP] * n
     lib.game_gpu_op_thomas.argtypes = [C.c_int, C.c_int] + [VP] * 6
+    lib.game_gpu_op_thomas_noalloc.argtypes = [C.c_int, C.c_int] + [VP] * 5
     lib.game_gpu_op_thomas_adjoint.argtypes = [C.c_int, C.c_int] + [VP] * 10
     return lib

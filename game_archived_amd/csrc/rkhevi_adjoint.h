@@ -41,6 +41,15 @@ void thomas(int n, int ncol, const double *c, const double *d, const double *e, 
             double *work, hipStream_t st);
 void thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
                     const double *xb, double *rb, double *cb, double *db, double *eb, double *work, hipStream_t st);
+/* the device address of the solver scratch (THOMAS_SCRATCH_DOUBLES doubles) on
+   the current device: resolved once per device by thomas_scratch_resolve
+   (0, or -1 with a message), then returned by thomas_scratch (null when not
+   resolved on this device) */
+int thomas_scratch_resolve(char *err, size_t nerr);
+double *thomas_scratch(void);
 }
+/* capacity of the allocation-free three-band solve, in doubles (64 MiB):
+   n*ncol elements, e.g. every column of a res 7 x 26 grid */
+#define THOMAS_SCRATCH_DOUBLES (1LL << 23)
 
 #endif

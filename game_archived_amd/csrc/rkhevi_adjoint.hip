@@ -469,3 +469,53 @@ void thomas_adjoint(int n, int ncol, const double *c, const double *d, const dou
 { LAUNCH("k_thomas_adj", k_thomas_adj, AGRID(ncol), dim3(ATPB), st, n, ncol, c, d, e, x, xb, rb, cb, db, eb, work); }
 
 }  // namespace adj
+
+/* ------------------------------------------------------------------ */
+/* scratch of the allocation-free three-band solve                     */
+/* ------------------------------------------------------------------ */
+/* The elimination quotients of game_gpu_op_thomas_noalloc live here instead
+   of in a caller's work buffer: THOMAS_SCRATCH_DOUBLES doubles (64 MiB) that
+   every GPU the library runs on holds from module load to process end.  In
+   host code `scratch` names a host-side stand-in, never the device memory:
+   only the address hipGetSymbolAddress returns for the current device (kept
+   per device below) may be passed to a kernel. */
+static __device__ __align__(256) double scratch[THOMAS_SCRATCH_DOUBLES];
+
+namespace {
+const int SCRATCH_DEVICES = 64;
+double *g_scratch_ptr[SCRATCH_DEVICES] = {nullptr};
+}
+
+namespace adj {
+
+int thomas_scratch_resolve(char *err, size_t nerr)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        snprintf(err, nerr, "hipGetDevice failed: %s", hipGetErrorString(e));
+        return -1;
+    }
+    if (dev < 0 || dev >= SCRATCH_DEVICES) {
+        snprintf(err, nerr, "device %d is beyond the %d devices the solver scratch supports", dev, SCRATCH_DEVICES);
+        return -1;
+    }
+    if (g_scratch_ptr[dev]) return 0;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        snprintf(err, nerr, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        return -1;
+    }
+    g_scratch_ptr[dev] = (double *)p;
+    return 0;
+}
+
+double *thomas_scratch(void)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_DEVICES) return nullptr;
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace adj

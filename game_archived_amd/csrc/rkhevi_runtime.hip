@@ -391,6 +391,8 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
         if ((rc = dzeros(&W.flux2, 8))) return rc;
         if ((rc = dzeros(&W.flux_divv, S))) return rc;
     }
+    /* the device address of the allocation-free solver's scratch on this device */
+    if (adj::thomas_scratch_resolve(g_err, sizeof g_err)) return GAME_ERR_HIP;
     if ((rc = dzeros(&M->partials, (size_t)2*col::INTEGRAL_BLOCKS*INTEGRALS_MAX))) return rc;
     HIP_OK(hipHostMalloc((void **)&M->h_partials, sizeof(double)*2*col::INTEGRAL_BLOCKS*INTEGRALS_MAX,
                          hipHostMallocDefault));
@@ -1060,6 +1062,22 @@ int game_gpu_op_thomas(int n, int ncol, const double *c, const double *d, const 
     if (!M) return GAME_ERR_NOT_INIT;
     if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
     adj::thomas(n, ncol, c, d, e, r, x, work, M->stream);
+    return op_done();
+}
+
+int game_gpu_op_thomas_noalloc(int n, int ncol, const double *c, const double *d, const double *e, const double *r,
+                               double *x)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    if ((long long)n*(long long)ncol > THOMAS_SCRATCH_DOUBLES) {
+        snprintf(g_err, sizeof g_err, "game_gpu_op_thomas_noalloc: %d systems of %d rows need %lld doubles of "
+                 "scratch, more than its %lld", ncol, n, (long long)n*(long long)ncol, (long long)THOMAS_SCRATCH_DOUBLES);
+        return GAME_ERR_ARG;
+    }
+    /* resolved at init on the model's device; resolved here for another current device */
+    if (adj::thomas_scratch_resolve(g_err, sizeof g_err)) return GAME_ERR_HIP;
+    adj::thomas(n, ncol, c, d, e, r, x, adj::thomas_scratch(), M->stream);
     return op_done();
 }
 

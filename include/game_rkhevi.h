@@ -318,6 +318,17 @@ int game_gpu_op_inner_product_adjoint(const double *b, const double *out_bar, do
    work: n*ncol doubles */
 int game_gpu_op_thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r,
                        double *x, double *work);
+/* The same solve without a caller-supplied work buffer, for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+   It calls no allocating routine: the elimination quotients go to a
+   file-scope device array of 2^23 doubles (64 MiB) that each GPU the library
+   runs on holds until the process ends; n*ncol above 2^23 is rejected with
+   GAME_ERR_ARG.  Every element of it that a call reads, the same call wrote
+   first, so nothing needs clearing; calls that overlap on one GPU share the
+   array and race. */
+int game_gpu_op_thomas_noalloc(int n, int ncol, const double *c, const double *d, const double *e, const double *r,
+                               double *x);
 /* its adjoint: r_bar = A^-T x_bar, c_bar/d_bar/e_bar = -(r_bar x^T) on the bands */
 int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
                                const double *x_bar, double *r_bar, double *c_bar, double *d_bar, double *e_bar,
