@@ -5,10 +5,12 @@ moist (6 constituents), flat surface, 2000 Lloyd iterations, the analytic
 balanced state with seeded noise, at the reference's automatic time step
 (1.61e-3 x effective resolution, 90.1 s at res 7).  A step is one RKHEVI
 time step (manage_rkhevi plus the driver's state copy) on inputs resident in
-HBM.  Both precision forms run from the same state for one simulated day
-(warm-up plus timed steps); the JSON value is the mixed form's simulated days
-per wall-hour, the FP64 form's numbers and the mixed form's difference from
-it after the day sit in config.forms.  Prints one JSON line.
+HBM.  Both precision forms run from the same state: warm-up steps, the timed
+steps, then untimed steps up to one simulated day, after which the mixed form
+is compared with the FP64 form (whatever --steps is).  The JSON value is the
+mixed form's simulated days per wall-hour over the timed steps; the FP64
+form's numbers and the mixed form's difference from it after the day sit in
+config.forms.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 7] [--precision both|fp64|mixed] [--no-cpu]
   python bench.py --copy-probe 1024     # plain 1 GiB device copy, for counter calibration
@@ -181,10 +183,11 @@ def print_kernel_table(stats, d, wb):
         log(f"{name:24s} {n:8d} {ms / n * 1e3:9.1f} {ms / total_ms * 100:5.1f}% {gbs:8.1f}")
 
 
-def time_form(grid, state0, precision, steps, warmup, dt, moist):
+def time_form(grid, state0, precision, steps, warmup, dt, moist, compare_steps):
     """warm-up steps, then `steps` timed steps of one precision form with the
-    state resident in HBM; per-kernel times from HIP events on the library's
-    stream"""
+    state resident in HBM; then, untimed, the run continues to `compare_steps`
+    steps (one simulated day) for the state the forms are compared on;
+    per-kernel times from HIP events on the library's stream"""
     from game_archived_amd.rkhevi import DeviceModel, NEW
     d = grid.dims
     dm = DeviceModel(grid, precision=precision)
@@ -199,6 +202,10 @@ def time_form(grid, state0, precision, steps, warmup, dt, moist):
         dm.run(steps, dt)
         dm.synchronize()
         wall = time.perf_counter() - t0
+        done = warmup + steps
+        if done < compare_steps:
+            dm.run(compare_steps - done, dt)
+            done = compare_steps
         i1 = dm.integrals()
         final = dm.get_state(NEW)
         # per-kernel times: further steps of the same run, each launch
@@ -215,6 +222,7 @@ def time_form(grid, state0, precision, steps, warmup, dt, moist):
     dry0 = i0[d.n_cond] - (i0[d.n_cond + 1] if moist else 0.0)
     dry1 = i1[d.n_cond] - (i1[d.n_cond + 1] if moist else 0.0)
     return {"ms_per_step": wall / steps * 1e3, "stats": stats, "final": final, "bytes_allocated": nbytes,
+            "compared_after_steps": done,
             "mass_drift": float((i1[d.n_cond] - i0[d.n_cond]) / i0[d.n_cond]),
             "dry_drift": float((dry1 - dry0) / dry0),
             "finite": bool(np.isfinite(final.rho).all() and np.isfinite(final.wind).all())}
@@ -305,6 +313,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the operator adjoint/forward timing")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--compare-days", type=float, default=None,
+                    help="simulated days after which the forms are compared (default 1; untimed steps added "
+                         "after the timed ones)")
     ap.add_argument("--lloyd", type=int, default=2000)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precision forms to time; the JSON value is the mixed form's when it runs")
@@ -345,7 +356,10 @@ def main():
         args.dt = 1.61e-3 * eff_hor_res
     args.dt = float(args.dt)
     warmup = args.warmup
-    steps = args.steps if args.steps is not None else max(1, math.ceil(86400.0 / args.dt) - warmup)
+    day_steps = math.ceil(86400.0 / args.dt)
+    steps = args.steps if args.steps is not None else max(1, day_steps - warmup)
+    # the forms are compared after one simulated day (or the timed run, if longer)
+    compare_steps = day_steps if args.compare_days is None else math.ceil(args.compare_days * 86400.0 / args.dt)
     from game_archived_amd.rkhevi import FP64, MIXED
     t0 = time.time()
     dm = DeviceModel(grid, precision=FP64)
@@ -355,7 +369,7 @@ def main():
     forms = {"fp64": FP64, "mixed": MIXED} if args.precision == "both" else {args.precision: {"fp64": FP64, "mixed": MIXED}[args.precision]}
     res = {}
     for name, prec in forms.items():
-        res[name] = time_form(grid, state0, prec, steps, warmup, args.dt, moist)
+        res[name] = time_form(grid, state0, prec, steps, warmup, args.dt, moist, compare_steps)
         r = res[name]
         log(f"[{name}] {r['ms_per_step']:.4f} ms/step, device memory {r['bytes_allocated'] / 2**30:.2f} GiB")
         print_kernel_table(r["stats"], d, 4 if prec == MIXED else 8)
@@ -392,8 +406,8 @@ def main():
                   "u": (a_.wind[~is_w], b_.wind[~is_w])}
         form_cfg["mixed"]["rel_l2_vs_fp64"] = {
             "rho_gas": rel(*fields["rho_gas"]), "rhotheta_v": rel(*fields["rhotheta_v"]),
-            "wind": rel(a_.wind, b_.wind), "after_steps": warmup + steps,
-            "after_sim_days": round((warmup + steps) * args.dt / 86400.0, 4)}
+            "wind": rel(a_.wind, b_.wind), "after_steps": res["mixed"]["compared_after_steps"],
+            "after_sim_days": round(res["mixed"]["compared_after_steps"] * args.dt / 86400.0, 4)}
         # per prognostic field: relative L2 and max |difference| / max |FP64 value|
         form_cfg["mixed"]["error_vs_fp64"] = {
             name: {"rel_l2": rel(u, v), "rel_max": relmax(u, v)} for name, (u, v) in fields.items()}
