@@ -191,3 +191,65 @@ int game_init_phase2(const game_init_params_t *P, const game_dims_t *d, const ga
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* restart: the state from densities and temperature                     */
+/* ------------------------------------------------------------------ */
+/* saturation pressure in Pa over plane water, Huang (2018) Eq. (17) above
+   freezing, Pruppacher and Klett (2010) Eq. (A.4-1) for super-cooled water
+   (src/constituents/dictionary.c:287-333) */
+static double sat_pressure_water(double temperature)
+{
+    double tc = temperature - T_0_REF;
+    if (tc > 100.0) tc = 100.0;
+    if (tc > 0.0) return exp(34.494 - 4924.99/(tc + 237.1))/pow(tc + 105.0, 1.57);
+    if (tc < -50.0) tc = -50.0;
+    double r = 6.107799961 + 4.436518521e-1*tc + 1.428945805e-2*pow(tc, 2) + 2.650648471e-4*pow(tc, 3)
+               + 3.031240396e-6*pow(tc, 4) + 2.034080948e-8*pow(tc, 5) + 6.136820929e-11*pow(tc, 6);
+    return 100.0*r;
+}
+
+/* over plane ice, Huang (2018) Eq. (18) (dictionary.c:361-385) */
+static double sat_pressure_ice(double temperature)
+{
+    double tc = temperature - T_0_REF;
+    if (tc < -80.0) tc = -80.0;
+    if (tc > 0.0) tc = 0.0;
+    return exp(43.494 - 6545.8/(tc + 278.0))/pow(tc + 868.0, 2);
+}
+
+double game_rel_humidity(double abs_humidity, double temperature)
+{
+    double vapour_pressure = abs_humidity*R_V_REF*temperature;
+    double saturation = temperature > T_0_REF ? sat_pressure_water(temperature) : sat_pressure_ice(temperature);
+    return vapour_pressure/saturation;
+}
+
+int game_state_from_temperature(const game_dims_t *d, const game_grid_view_t *g, double *rho,
+                                const double *temperature, game_state_view_t *st)
+{
+    const long long S = (long long)d->nsh*d->n_layers;
+    const int n_cond = 4*d->moisture_on, n_const = 1 + d->moisture_on + n_cond;
+    double *gas = rho + (long long)n_cond*S, *vap = d->moisture_on ? rho + (long long)(n_cond + 1)*S : NULL;
+    if (vap) {
+        #pragma omp parallel for
+        for (long long i = 0; i < S; ++i) {
+            double rh = game_rel_humidity(vap[i], temperature[i]);
+            if (rh > 1.0) vap[i] = vap[i]/game_rel_humidity(vap[i], temperature[i]);
+        }
+    }
+    /* dry, the reference reads the member behind rho there (rhotheta_v of the
+       state being filled, zero), so the virtual temperature is the temperature */
+    #pragma omp parallel for
+    for (long long i = 0; i < S; ++i) {
+        double tv = vap ? temperature[i]*(1.0 + vap[i]/gas[i]*(M_D_REF/M_V_REF - 1.0)) : temperature[i];
+        double pressure = gas[i]*R_D_REF*tv;
+        double pot_temp_v = tv*pow(P_0_REF/pressure, R_D_REF/C_D_P_REF);
+        st->rhotheta_v[i] = gas[i]*pot_temp_v;
+        st->theta_v_pert[i] = pot_temp_v - g->theta_v_bg[i];
+        st->exner_pert[i] = tv/(g->theta_v_bg[i] + st->theta_v_pert[i]) - g->exner_bg[i];
+    }
+    for (long long i = 0; i < (long long)n_const*S; ++i)
+        if (rho[i] < 0) return 1;
+    return 0;
+}

@@ -45,6 +45,17 @@ int game_init_phase1(const game_init_params_t *p, const game_dims_t *d, const ga
 int game_init_phase2(const game_init_params_t *p, const game_dims_t *d, const game_grid_view_t *g,
                      const double *v_squared, const double *pot_vort_tend,
                      game_state_view_t *state, game_init_work_t *w);
+/* The re-derivation of a state read from a file (read_init_data,
+   src/io/set_initial_state.c:244-336): given the densities (n_const x S) and
+   the temperature (S), the water vapour is limited to saturation in place
+   (rel_humidity <= 1, :281-293), then rhotheta_v, theta_v_pert and
+   exner_pert follow from the equation of state (:296-314).  rho and wind of
+   `state` are not touched; returns 1 if a density is negative (the
+   reference aborts there, :318-327). */
+int game_state_from_temperature(const game_dims_t *d, const game_grid_view_t *g, double *rho,
+                                const double *temperature, game_state_view_t *state);
+/* rel_humidity (src/constituents/derived_quantities.c:63-82) */
+double game_rel_humidity(double abs_humidity, double temperature);
 double game_gauss(uint64_t seed, long long cell, int layer);
 double game_uniform01(uint64_t seed, long long cell, int layer, int stream);
 
