@@ -209,6 +209,8 @@ def gpu_lib():
     lib.game_gpu_run_steps.argtypes = [C.c_int, C.c_double, C.c_int]
     lib.game_gpu_temperature_diagnostics.argtypes = [C.c_int]
     lib.game_gpu_integrals.argtypes = [C.c_int, DP]
+    lib.game_gpu_write_integrals.argtypes = [C.c_int, C.c_double, C.c_char_p]
+    lib.game_gpu_run_steps_write_integrals.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_char_p]
     lib.game_gpu_enable_timing.argtypes = [C.c_int]
     lib.game_gpu_kernel_stats.argtypes = [C.c_int, P(C.c_char_p), P(C.c_longlong), P(C.c_double)]
     lib.game_gpu_last_error.restype = C.c_char_p
@@ -222,5 +224,8 @@ def gpu_lib():
         getattr(lib, f"game_gpu_op_{name}_adjoint").argtypes = [VP] * n
     lib.game_gpu_op_thomas.argtypes = [C.c_int, C.c_int] + [VP] * 6
     lib.game_gpu_op_thomas_noalloc.argtypes = [C.c_int, C.c_int] + [VP] * 5
+    lib.game_gpu_window_forward.argtypes = [C.c_int, C.c_double] + [VP] * 7
+    lib.game_gpu_window_vjp.argtypes = [C.c_int, C.c_double, C.c_int] + [VP] * 10
+    lib.game_gpu_window_gradient.argtypes = [C.c_int, C.c_double, C.c_int] + [VP] * 4 + [DP] + [VP] * 3
     lib.game_gpu_op_thomas_adjoint.argtypes = [C.c_int, C.c_int] + [VP] * 10
     return lib

@@ -51,7 +51,7 @@ def build_host():
 def build_gpu():
     os.makedirs(LIB, exist_ok=True)
     objs = []
-    for src in ("rkhevi_fp64.hip", "rkhevi_adjoint.hip", "rkhevi_runtime.hip"):
+    for src in ("rkhevi_fp64.hip", "rkhevi_adjoint.hip", "rkhevi_runtime.hip", "rkhevi_window.hip"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

@@ -21,102 +21,15 @@
 #include "rkhevi_dev.h"
 #include "rkhevi_kernels.h"
 #include "rkhevi_adjoint.h"
+#include "step_adjoint.h"
+#include "adj_tables_host.h"
+#include "runtime_internal.h"
 
 KernelTimer *g_kernel_timer = nullptr;
 
-namespace {
+namespace rt {
 
 char g_err[512];
-
-/* integrals formed by one game_gpu_integrals call: the densities (at most six
-   constituents), rhotheta_v and three energies */
-const int INTEGRALS_MAX = 16;
-
-#define HIP_OK(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess) {                                                             \
-            snprintf(g_err, sizeof g_err, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                     __FILE__, __LINE__);                                                   \
-            return GAME_ERR_HIP;                                                            \
-        }                                                                                   \
-    } while (0)
-
-struct Stat {
-    std::string name;
-    long long launches = 0;
-    double ms = 0.0;
-};
-
-struct EventTimer : KernelTimer {
-    std::vector<Stat> stats;
-    struct Pending { int idx; hipEvent_t a, b; };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> pool;
-    int cur = -1;
-    hipEvent_t cur_a = nullptr;
-
-    hipEvent_t get() {
-        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        return e;
-    }
-    int index(const char *name) {
-        for (size_t k = 0; k < stats.size(); ++k) if (stats[k].name == name) return (int)k;
-        Stat s; s.name = name; stats.push_back(s);
-        return (int)stats.size() - 1;
-    }
-    void begin(const char *name, hipStream_t st) override {
-        cur = index(name);
-        cur_a = get();
-        (void)hipEventRecord(cur_a, st);
-    }
-    void end(hipStream_t st) override {
-        hipEvent_t b = get();
-        (void)hipEventRecord(b, st);
-        pending.push_back({cur, cur_a, b});
-    }
-    void collect() {
-        for (auto &p : pending) {
-            float ms = 0.f;
-            (void)hipEventSynchronize(p.b);
-            (void)hipEventElapsedTime(&ms, p.a, p.b);
-            stats[p.idx].launches += 1;
-            stats[p.idx].ms += ms;
-            pool.push_back(p.a);
-            pool.push_back(p.b);
-        }
-        pending.clear();
-    }
-};
-
-struct Model {
-    game_dims_t d;
-    game_config_t cfg;
-    int precision = GAME_FP64;
-    DevGrid G;
-    DevState s[3];
-    DevWork W;      /* FP64 form */
-    DevWorkF WF;    /* mixed form */
-    AdjTables A;    /* reverse tables of the operator adjoints, built on first use */
-    int adj_ready = 0;
-    StepConsts K;
-    double idv_dt = -1.0;   /* the dt G.idv holds quotients for */
-    hipStream_t stream = nullptr;
-    std::vector<void *> allocs;
-    long long bytes = 0;
-    double *partials = nullptr, *h_partials = nullptr;
-    double *tmp_s = nullptr;
-    EventTimer timer;
-    /* HIP graphs of one steady step (not the first), one per assignment of
-       the two state buffers to old/new, keyed by the old state's density
-       buffer; rebuilt when dt or the configuration changes */
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    const void *graph_key[2] = {nullptr, nullptr};
-    double graph_dt = -1.0;
-    int use_graphs = 1;
-};
 
 Model *M = nullptr;
 
@@ -128,6 +41,18 @@ int dalloc(void **p, size_t n)
     M->allocs.push_back(*p);
     M->bytes += (long long)n;
     return 0;
+}
+
+void dfree(void *p)
+{
+    if (!p || !M) return;
+    for (size_t k = 0; k < M->allocs.size(); ++k)
+        if (M->allocs[k] == p) {
+            (void)hipStreamSynchronize(M->stream);
+            (void)hipFree(p);
+            M->allocs.erase(M->allocs.begin() + (long)k);
+            return;
+        }
 }
 
 template <class T>
@@ -183,7 +108,9 @@ int check_config(const game_config_t *c)
 
 DevState *slot(int k) { return (k >= 0 && k < 3) ? &M->s[k] : nullptr; }
 
-}  // namespace
+}  // namespace rt
+
+using namespace rt;
 
 extern "C" {
 
@@ -254,7 +181,9 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
     int ndev = 0;
     HIP_OK(hipGetDeviceCount(&ndev));
     if (ndev < 1) { snprintf(g_err, sizeof g_err, "no HIP device"); return GAME_ERR_HIP; }
+    static long long generations = 0;
     M = new Model();
+    M->generation = ++generations;
     M->d = d;
     M->cfg = *config;
     M->precision = precision;
@@ -510,13 +439,36 @@ static int get_irrev_t(const WK &W, game_irrev_view_t *h)
    state_old: the step then also runs on a state_new buffer that holds an
    older state, which lets game_gpu_run_steps swap the two buffers instead of
    copying. */
+/* the five fields of a state, device to device */
+static int copy_state_d2d(const DevState &dst, const DevState &src, hipStream_t st)
+{
+    const size_t S = (size_t)M->d.s*sizeof(double), V = (size_t)M->d.v*sizeof(double);
+    HIP_OK(hipMemcpyAsync(dst.rho, src.rho, S*(size_t)M->d.n_const, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(dst.rt, src.rt, S, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(dst.tp, src.tp, S, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(dst.ep, src.ep, S, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(dst.wind, src.wind, V, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+static int copy_tendency(double *rho, double *rt_, double *wind, const DevState &td, hipStream_t st)
+{
+    const size_t S = (size_t)M->d.s*sizeof(double), V = (size_t)M->d.v*sizeof(double);
+    HIP_OK(hipMemcpyAsync(rho, td.rho, S, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(rt_, td.rt, S, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(wind, td.wind, V, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
 template <class WK, typename TW>
-static int step_t(const WK &W, TW *td_wind, double dt, int first)
+static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape *tape = nullptr)
 {
     hipStream_t st = M->stream;
     const DevGrid &G = M->G;
     DevState &so = M->s[GAME_STATE_OLD], &sn = M->s[GAME_STATE_NEW], &td = M->s[GAME_STATE_TENDENCY];
     M->K.dt = dt;
+    int rc;
+    if (tape && (rc = copy_state_d2d(tape->s, so, st))) return rc;
     if (G.n_const > 1 && dt != M->idv_dt) {
         col::tracer_quotients(G, dt, st);
         M->idv_dt = dt;
@@ -539,9 +491,19 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first)
         if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
         else col::ver_waves(G, so, snr, sn, td, (const float *)td_wind, M->K, rk, st);
         if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.cpc, W.sed_heat, M->K, rk, st);
+        if (tape && rk == 0) {
+            if ((rc = copy_state_d2d(tape->s1, sn, st))) return rc;
+            if ((rc = copy_tendency(tape->td0_rho, tape->td0_rt, tape->td0_wind, td, st))) return rc;
+        }
+        if (tape && rk == 1 && (rc = copy_tendency(tape->td1_rho, tape->td1_rt, tape->td1_wind, td, st))) return rc;
     }
     HIP_OK(hipGetLastError());
     return 0;
+}
+
+int rt::step_fp64(double dt, int first, const StepTape *tape)
+{
+    return step_t(M->W, M->s[GAME_STATE_TENDENCY].wind, dt, first, tape);
 }
 
 extern "C" {
@@ -749,6 +711,50 @@ int game_gpu_integrals(int k, double *out)
     return M->precision == GAME_MIXED ? integrals_t(M->WF, *s, out) : integrals_t(M->W, *s, out);
 }
 
+/* the records write_out_integral appends (src/io/write_output.c:1515-1596):
+   "masses" (time, every constituent's mass), "potential_temperature_density"
+   (time, the rhotheta_v integral) and "energy" (time, kinetic, potential,
+   internal), tab-separated with the reference's %lf format */
+int game_gpu_write_integrals(int k, double time_since_init, const char *directory)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    if (!directory) { snprintf(g_err, sizeof g_err, "null directory"); return GAME_ERR_ARG; }
+    double out[INTEGRALS_MAX];
+    int rc = game_gpu_integrals(k, out);
+    if (rc) return rc;
+    const int nc = M->d.n_const;
+    const char *names[3] = {"masses", "potential_temperature_density", "energy"};
+    for (int f = 0; f < 3; ++f) {
+        std::string path = std::string(directory) + "/" + names[f];
+        FILE *fp = fopen(path.c_str(), "a");
+        if (!fp) { snprintf(g_err, sizeof g_err, "cannot open %s for appending", path.c_str()); return GAME_ERR_ARG; }
+        fprintf(fp, "%lf\t", time_since_init);
+        if (f == 0)
+            for (int c = 0; c < nc; ++c) fprintf(fp, c == nc - 1 ? "%lf\n" : "%lf\t", out[c]);
+        else if (f == 1)
+            fprintf(fp, "%lf\n", out[nc]);
+        else
+            fprintf(fp, "%lf\t%lf\t%lf\n", out[nc + 1], out[nc + 2], out[nc + 3]);
+        if (fclose(fp)) { snprintf(g_err, sizeof g_err, "cannot write %s", path.c_str()); return GAME_ERR_ARG; }
+    }
+    return 0;
+}
+
+/* the reference's time loop with its integral output (src/coordinator.c:302-336):
+   per step the driver's copy and manage_rkhevi (as game_gpu_run_steps), then
+   the three records of the new state at time t_start + (k + 1) dt */
+int game_gpu_run_steps_write_integrals(int n, double dt, int first, double t_start, const char *directory)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    for (int k = 0; k < n; ++k) {
+        int rc = game_gpu_run_steps(1, dt, first);
+        if (rc) return rc;
+        first = 0;
+        if ((rc = game_gpu_write_integrals(GAME_STATE_NEW, t_start + (k + 1)*dt, directory))) return rc;
+    }
+    return 0;
+}
+
 }  /* extern "C" */
 
 template <class WK>
@@ -833,13 +839,15 @@ int game_gpu_kernel_stats(int k, const char **name, long long *launches, double 
 /* ------------------------------------------------------------------ */
 }  /* extern "C" */
 
+namespace rt {
 static int upload_ints(const int **dst, const std::vector<int> &v)
 {
     return upload<int>(dst, v.data(), v.size());
 }
+}  // namespace rt
 
 /* reverse index tables of the adjoint gathers, from the forward tables */
-static int build_adjoint_tables(void)
+int rt::build_adjoint_tables(void)
 {
     if (M->adj_ready) return 0;
     const DevGrid &G = M->G;
@@ -853,101 +861,24 @@ static int build_adjoint_tables(void)
     HIP_OK(hipMemcpyAsync(drw.data(), G.drw, 4*NVH*sizeof(double), hipMemcpyDeviceToHost, M->stream));
     HIP_OK(hipMemcpyAsync(tw.data(), G.tw, 10*NVH*sizeof(double), hipMemcpyDeviceToHost, M->stream));
     HIP_OK(hipStreamSynchronize(M->stream));
-    std::vector<int> sf(NVH, -1), stt(NVH, -1);
-    for (size_t c = 0; c < NSH; ++c) {
-        int ne = c < GAME_N_PENT ? 5 : 6;
-        for (int j = 0; j < ne; ++j) {
-            int e = adj[6*c + j];
-            if (from[e] == (int)c) sf[e] = j;
-            else if (to[e] == (int)c) stt[e] = j;
-        }
-    }
-    std::vector<int> toe(2*NVH, -1), tsl(2*NVH, -1), cnt(NVH, 0);
-    for (size_t t = 0; t < NT; ++t)
-        for (int k = 0; k < 3; ++k) {
-            int e = vit[3*t + k];
-            if (e < 0 || (size_t)e >= NVH || cnt[e] >= 2) {
-                snprintf(g_err, sizeof g_err, "triangle %zu lists edge %d a third time", t, e);
-                return GAME_ERR_ARG;
-            }
-            toe[2*e + cnt[e]] = (int)t;
-            tsl[2*e + cnt[e]] = k;
-            ++cnt[e];
-        }
-    for (size_t e = 0; e < NVH; ++e)
-        if (sf[e] < 0 || stt[e] < 0 || cnt[e] != 2) {
-            snprintf(g_err, sizeof g_err, "edge %zu: inconsistent adjacency or triangle tables", e);
-            return GAME_ERR_ARG;
-        }
-    /* CSR lists in ascending (edge, slot) order */
-    std::vector<int> roff(NSH + 1, 0), rent;
-    for (size_t e = 0; e < NVH; ++e) for (int j = 0; j < 4; ++j) ++roff[dri[4*e + j] + 1];
-    for (size_t c = 0; c < NSH; ++c) roff[c + 1] += roff[c];
-    rent.resize(2*(size_t)roff[NSH]);
-    {
-        std::vector<int> fill(roff.begin(), roff.end() - 1);
-        for (size_t e = 0; e < NVH; ++e)
-            for (int j = 0; j < 4; ++j) {
-                int c = dri[4*e + j], k = fill[c]++;
-                rent[2*(size_t)k] = (int)e;
-                rent[2*(size_t)k + 1] = j;
-            }
-    }
-    /* TRSK slots the forward reads: 0..3 / 5..8 beside a pentagon, else 0..4 / 5..9 */
-    auto used = [&](size_t e, int j) {
-        if (j < 5) return from[e] < GAME_N_PENT ? j < 4 : true;
-        return to[e] < GAME_N_PENT ? j < 9 : true;
-    };
-    std::vector<int> toff(NVH + 1, 0), tent;
-    for (size_t e = 0; e < NVH; ++e) for (int j = 0; j < 10; ++j) if (used(e, j)) ++toff[ti[10*e + j] + 1];
-    for (size_t e = 0; e < NVH; ++e) toff[e + 1] += toff[e];
-    tent.resize(2*(size_t)toff[NVH]);
-    {
-        std::vector<int> fill(toff.begin(), toff.end() - 1);
-        for (size_t e = 0; e < NVH; ++e)
-            for (int j = 0; j < 10; ++j) {
-                if (!used(e, j)) continue;
-                int k = fill[ti[10*e + j]]++;
-                tent[2*(size_t)k] = (int)e;
-                tent[2*(size_t)k + 1] = j;
-            }
-    }
-    /* padded slot-major copies of both lists, with the weights (and PV
-       indices) of each entry; a TRSK term that half-averages PV with the
-       edge's own (slots 2 and 7 away from pentagons) stores its PV index m
-       as -(m + 1) */
-    std::vector<int> rhp_e((size_t)ADJ_RH_MAX*NSH, 0), trp_e((size_t)ADJ_TR_MAX*NVH, 0), trp_m((size_t)ADJ_TR_MAX*NVH, 0);
-    std::vector<double> rhp_w((size_t)ADJ_RH_MAX*NSH, 0.0), trp_w((size_t)ADJ_TR_MAX*NVH, 0.0);
-    for (size_t c = 0; c < NSH; ++c) {
-        const int n = roff[c + 1] - roff[c];
-        if (n > ADJ_RH_MAX) { snprintf(g_err, sizeof g_err, "cell %zu in %d rhombi", c, n); return GAME_ERR_ARG; }
-        for (int k = 0; k < n; ++k) {
-            const int e = rent[2*(size_t)(roff[c] + k)], j = rent[2*(size_t)(roff[c] + k) + 1];
-            rhp_e[(size_t)k*NSH + c] = e;
-            rhp_w[(size_t)k*NSH + c] = drw[4*(size_t)e + j];
-        }
-    }
-    for (size_t ep = 0; ep < NVH; ++ep) {
-        const int n = toff[ep + 1] - toff[ep];
-        if (n > ADJ_TR_MAX) { snprintf(g_err, sizeof g_err, "edge %zu in %d TRSK stencils", ep, n); return GAME_ERR_ARG; }
-        for (int k = 0; k < ADJ_TR_MAX; ++k) trp_e[(size_t)k*NVH + ep] = (int)ep;
-        for (int k = 0; k < n; ++k) {
-            const int e = tent[2*(size_t)(toff[ep] + k)], j = tent[2*(size_t)(toff[ep] + k) + 1];
-            const bool half = (j == 2 && from[e] >= GAME_N_PENT) || (j == 7 && to[e] >= GAME_N_PENT);
-            trp_e[(size_t)k*NVH + ep] = e;
-            trp_m[(size_t)k*NVH + ep] = half ? -(tmc[10*(size_t)e + j] + 1) : tmc[10*(size_t)e + j];
-            trp_w[(size_t)k*NVH + ep] = tw[10*(size_t)e + j];
-        }
-    }
+    AdjTablesHost T;
+    if (build_adj_tables_host(G.nsh, G.nvh, G.nt, GAME_N_PENT, adj.data(), from.data(), to.data(), vit.data(),
+                              dri.data(), drw.data(), ti.data(), tmc.data(), tw.data(), ADJ_RH_MAX, ADJ_TR_MAX,
+                              PVR_MAX, T, g_err, sizeof g_err))
+        return GAME_ERR_ARG;
     AdjTables &A = M->A;
     int rc;
-    if ((rc = upload_ints(&A.rhp_e, rhp_e)) || (rc = upload_ints(&A.trp_e, trp_e)) || (rc = upload_ints(&A.trp_m, trp_m))
-        || (rc = upload(&A.rhp_w, rhp_w.data(), rhp_w.size())) || (rc = upload(&A.trp_w, trp_w.data(), trp_w.size())))
+    if ((rc = upload_ints(&A.rhp_e, T.rhp_e)) || (rc = upload_ints(&A.trp_e, T.trp_e)) || (rc = upload_ints(&A.trp_m, T.trp_m))
+        || (rc = upload(&A.rhp_w, T.rhp_w.data(), T.rhp_w.size())) || (rc = upload(&A.trp_w, T.trp_w.data(), T.trp_w.size())))
         return rc;
-    if ((rc = upload_ints(&A.slot_from, sf)) || (rc = upload_ints(&A.slot_to, stt))
-        || (rc = upload_ints(&A.tri_of_edge, toe)) || (rc = upload_ints(&A.tri_slot, tsl))
-        || (rc = upload_ints(&A.rh_off, roff)) || (rc = upload_ints(&A.rh_ent, rent))
-        || (rc = upload_ints(&A.tr_off, toff)) || (rc = upload_ints(&A.tr_ent, tent)))
+    if ((rc = upload_ints(&A.slot_from, T.slot_from)) || (rc = upload_ints(&A.slot_to, T.slot_to))
+        || (rc = upload_ints(&A.tri_of_edge, T.tri_of_edge)) || (rc = upload_ints(&A.tri_slot, T.tri_slot))
+        || (rc = upload_ints(&A.rh_off, T.rh_off)) || (rc = upload_ints(&A.rh_ent, T.rh_ent))
+        || (rc = upload_ints(&A.tr_off, T.tr_off)) || (rc = upload_ints(&A.tr_ent, T.tr_ent)))
+        return rc;
+    PvReverse &R = M->PR;
+    if ((rc = upload_ints(&R.e, T.pvr_e)) || (rc = upload_ints(&R.f, T.pvr_f))
+        || (rc = upload(&R.w, T.pvr_w.data(), T.pvr_w.size())))
         return rc;
     HIP_OK(hipStreamSynchronize(M->stream));
     M->adj_ready = 1;

@@ -1,0 +1,839 @@
+/*
+ * step_adjoint.h -- the reverse (adjoint) of the dry RKHEVI step, as
+ * element functors: each struct holds its arguments and computes, in
+ * operator()(t), every output element owned by index t.  rkhevi_step_adjoint.hip
+ * launches them (one thread per t) in the order of the reverse sweep.  The
+ * reference has no adjoint; each functor cites the forward lines it reverses.
+ *
+ * Conventions: a cotangent array named xb holds d(J)/d(x) for the forward
+ * array x (same layout); "+=" functors accumulate, every element owned by one
+ * thread, so no atomics are needed.  Everything is FP64.
+ */
+#ifndef GAME_STEP_ADJOINT_H
+#define GAME_STEP_ADJOINT_H
+
+#include "rkhevi_dev.h"
+#include "rkhevi_adjoint.h"
+
+#define GAME_HD __host__ __device__
+
+/* reverse table of the rhombus PV read by the TRSK sums (vorticity_flux.c:46-109):
+   for rhombus m, the (output edge, weight, flux edge) of every term that reads
+   PV m, padded slot-major to PVR_MAX entries with weight 0; the hexagon
+   half-averages contribute one entry with half the weight for the modified
+   index and one for the edge's own rhombus */
+#define PVR_MAX 20
+struct PvReverse {
+    const int *e;        /* PVR_MAX x NVH: output edge */
+    const int *f;        /* PVR_MAX x NVH: flux edge */
+    const double *w;     /* PVR_MAX x NVH: weight */
+};
+
+namespace sadj {
+
+GAME_HD inline i64 eidx(const DevGrid &G, int l, int e) { return G.nsh + (i64)l*G.vpl + e; }
+GAME_HD inline i64 widx(const DevGrid &G, int lev, int h) { return (i64)lev*G.vpl + h; }
+GAME_HD inline i64 sidx(const DevGrid &G, int l, int h) { return (i64)l*G.nsh + h; }
+
+/* ------------------------------------------------------------------ */
+/* three_band_solver_ver_waves, column_solvers.c:19-383: one column     */
+/* ------------------------------------------------------------------ */
+/* Forward per column (L layers, interfaces j = 0..L-2 between layers j and
+   j+1 at level j+1):
+     re = rho_o + dt t_rho, rte = rt_o + dt t_rt,
+     alpha, beta, gamma (old state at rk 0; the mean of old and new at rk 1),
+     theta_e = tp_o + dt vol (alpha t_rho + beta t_rt), pi_e = ep_o + dt vol gamma t_rt,
+     rho_i = mean of rho_o, rho_ie = mean of re, theta_i = mean of rt_n/rho_n,
+     the three bands c, d, e and the right-hand side r, x = A^-1 r, damping,
+     then rho, rt, w, tp, ep of the new state.
+   The reverse recomputes the forward into a workspace (WS_N arrays of L
+   doubles per column, array k of column c element j at (k*L + j)*ncol + c,
+   so lanes of consecutive columns touch consecutive words) and walks back. */
+enum { WS_RE, WS_RTE, WS_AL, WS_BE, WS_GA, WS_TPE, WS_EPE, WS_RIO, WS_RIE, WS_TIN, WS_C, WS_D, WS_E, WS_R, WS_X,
+       WS_EP, WS_RNEW, WS_RTNEW,
+       /* reverse */
+       WS_BAL, WS_BBE, WS_BGA, WS_BTPE, WS_BEPE, WS_BRIO, WS_BRIE, WS_BTIN, WS_BRE, WS_BRTE, WS_BX, WS_BR, WS_N };
+
+struct VerWavesAdj {
+    DevGrid G;
+    int rk;
+    double dt, iw;
+    /* forward inputs: old state, new state as read (rk 1; rk 0: the old), tendency */
+    const double *rho_o, *rt_o, *tp_o, *ep_o, *wind_o;
+    const double *rho_n, *rt_n, *ep_n;
+    const double *t_rho, *t_rt, *t_wind;
+    /* cotangents of the outputs: rho, rt, w (interior levels), tp, ep of the new state */
+    const double *b_rho, *b_rt, *b_wind, *b_tp, *b_ep;
+    /* accumulated: cotangents of the old state, of the new state as read (rk 1
+       only; at rk 0 they are added to the old state's), of the tendency */
+    double *bo_rho, *bo_rt, *bo_tp, *bo_ep, *bo_wind;
+    double *bn_rho, *bn_rt, *bn_ep;
+    double *bt_rho, *bt_rt, *bt_wind;
+    double *ws;
+
+    GAME_HD double &W(int k, int j, int c) const { return ws[((i64)k*G.L + j)*G.nsh + c]; }
+
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.nsh) return;
+        const int c = (int)t, L = G.L, n = L - 1;
+        const double pdw = 0.5;
+        /* ---- forward ---- */
+        for (int j = 0; j < L; ++j) {
+            const i64 b = sidx(G, j, c);
+            const double ro = rho_o[b], rto = rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
+            const double tr = t_rho[b], tt = t_rt[b];
+            double al, be, ga;
+            if (rk == 0) {
+                al = -rto/(ro*ro)/vol;
+                be = 1.0/ro/vol;
+                ga = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b])/vol;
+            } else {
+                const double rn = rho_n[b], rtn = rt_n[b];
+                const double ao = -rto/(ro*ro), bo = 1.0/ro, go = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b]);
+                const double an = -rtn/(rn*rn), bn = 1.0/rn, gn = GC_R_D/(GC_C_D_V*rtn)*(exbg + ep_n[b]);
+                al = ((1.0 - pdw)*ao + pdw*an)/vol;
+                be = ((1.0 - pdw)*bo + pdw*bn)/vol;
+                ga = ((1.0 - pdw)*go + pdw*gn)/vol;
+            }
+            W(WS_RE, j, c) = ro + dt*tr;
+            W(WS_RTE, j, c) = rto + dt*tt;
+            W(WS_AL, j, c) = al;
+            W(WS_BE, j, c) = be;
+            W(WS_GA, j, c) = ga;
+            W(WS_TPE, j, c) = tp_o[b] + dt*vol*(al*tr + be*tt);
+            W(WS_EPE, j, c) = ep_o[b] + dt*vol*ga*tt;
+        }
+        const double *rn_ = rk == 0 ? rho_o : rho_n, *rtn_ = rk == 0 ? rt_o : rt_n;
+        for (int j = 0; j < n; ++j) {
+            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
+            W(WS_RIO, j, c) = 0.5*(rho_o[b] + rho_o[lo]);
+            W(WS_RIE, j, c) = 0.5*(W(WS_RE, j, c) + W(WS_RE, j + 1, c));
+            W(WS_TIN, j, c) = 0.5*(rtn_[b]/rn_[b] + rtn_[lo]/rn_[lo]);
+        }
+        for (int j = 0; j < n; ++j) {
+            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
+            const i64 wi = widx(G, j + 1, c);
+            const double tin = W(WS_TIN, j, c), rio = W(WS_RIO, j, c), wo = wind_o[wi];
+            const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
+            const double q = dz/(iw*(dt*dt)*GC_C_D_P);
+            W(WS_D, j, c) = -(tin*tin)*(W(WS_GA, j, c) + W(WS_GA, j + 1, c))
+                            + 0.5*dex*(W(WS_AL, j + 1, c) - W(WS_AL, j, c) + tin*(W(WS_BE, j + 1, c) - W(WS_BE, j, c)))
+                            - dz/(iw*(dt*dt)*GC_C_D_P*rio)*(2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]));
+            W(WS_R, j, c) = -(wo + dt*t_wind[wi])*q + tin*(W(WS_EPE, j, c) - W(WS_EPE, j + 1, c))/dt
+                            + 0.5/dt*(W(WS_TPE, j, c) + W(WS_TPE, j + 1, c))*dex - q*wo*W(WS_RIE, j, c)/rio;
+            if (j < n - 1) {
+                const i64 l2 = sidx(G, j + 2, c);
+                const double dz1 = G.zs[lo] - G.zs[l2], dex1 = G.exbg[lo] - G.exbg[l2];
+                const double tin1 = W(WS_TIN, j + 1, c);
+                W(WS_C, j, c) = tin1*W(WS_GA, j + 1, c)*tin + 0.5*dex1*(W(WS_AL, j + 1, c) + W(WS_BE, j + 1, c)*tin)
+                                - dz1/(iw*dt*GC_C_D_P)*0.5*wind_o[widx(G, j + 2, c)]/(G.vol[lo]*W(WS_RIO, j + 1, c));
+                W(WS_E, j, c) = tin*W(WS_GA, j + 1, c)*tin1 - 0.5*dex*(W(WS_AL, j + 1, c) + W(WS_BE, j + 1, c)*tin1)
+                                + dz/(iw*dt*GC_C_D_P)*0.5*wo/(G.vol[lo]*rio);
+            }
+        }
+        /* Thomas, column_solvers.c:641-667 */
+        {
+            double ep = n > 1 ? W(WS_E, 0, c)/W(WS_D, 0, c) : 0.0, rp = W(WS_R, 0, c)/W(WS_D, 0, c);
+            W(WS_EP, 0, c) = ep;
+            W(WS_X, 0, c) = rp;
+            for (int j = 1; j < n; ++j) {
+                const double den = W(WS_D, j, c) - ep*W(WS_C, j - 1, c);
+                if (j < n - 1) ep = W(WS_E, j, c)/den;
+                rp = (W(WS_R, j, c) - rp*W(WS_C, j - 1, c))/den;
+                W(WS_EP, j, c) = ep;
+                W(WS_X, j, c) = rp;
+            }
+            double x = W(WS_X, n - 1, c);
+            for (int j = n - 2; j >= 0; --j) {
+                x = W(WS_X, j, c) - W(WS_EP, j, c)*x;
+                W(WS_X, j, c) = x;
+            }
+        }
+        /* new rho and rt (needed by the reverse of tp, ep and w) */
+        for (int j = 0; j < L; ++j) {
+            const i64 b = sidx(G, j, c);
+            const double su = j < n ? W(WS_X, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]) : 0.0;
+            const double sd = j > 0 ? W(WS_X, j - 1, c)/(1.0 + dt*G.damp[sidx(G, j - 1, c)]) : 0.0;
+            const double tu = j < n ? W(WS_TIN, j, c) : 0.0, td = j > 0 ? W(WS_TIN, j - 1, c) : 0.0;
+            W(WS_RNEW, j, c) = W(WS_RE, j, c) + dt*(-sd + su)/G.vol[b];
+            W(WS_RTNEW, j, c) = W(WS_RTE, j, c) + dt*(-td*sd + tu*su)/G.vol[b];
+        }
+        /* ---- reverse ---- */
+        for (int j = 0; j < L; ++j) {
+            W(WS_BAL, j, c) = 0.0; W(WS_BBE, j, c) = 0.0; W(WS_BGA, j, c) = 0.0;
+            W(WS_BTPE, j, c) = 0.0; W(WS_BEPE, j, c) = 0.0;
+            W(WS_BRIO, j, c) = 0.0; W(WS_BRIE, j, c) = 0.0; W(WS_BTIN, j, c) = 0.0;
+        }
+        /* ep_new = ep_o + vol ga (rt_new - rt_o); tp_new = rt_new/rho_new - thbg */
+        for (int j = 0; j < L; ++j) {
+            const i64 b = sidx(G, j, c);
+            const double vol = G.vol[b], ga = W(WS_GA, j, c), rnew = W(WS_RNEW, j, c), rtnew = W(WS_RTNEW, j, c);
+            const double bep = b_ep[b], btp = b_tp[b];
+            W(WS_BGA, j, c) += vol*(rtnew - rt_o[b])*bep;
+            double brt = b_rt[b] + vol*ga*bep + btp/rnew;
+            double brho = b_rho[b] - btp*rtnew/(rnew*rnew);
+            W(WS_BRTE, j, c) = brt;     /* cotangent of rt_new for now */
+            W(WS_BRE, j, c) = brho;     /* cotangent of rho_new for now */
+        }
+        /* w_new[j] = (2 sol[j]/A - din wo)/rho_i, din = mean of rho_new */
+        for (int j = 0; j < n; ++j) {
+            const i64 wi = widx(G, j + 1, c);
+            const double rio = W(WS_RIO, j, c), wo = wind_o[wi], bw = b_wind[wi];
+            const double sol = W(WS_X, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]);
+            const double din = 0.5*(W(WS_RNEW, j, c) + W(WS_RNEW, j + 1, c));
+            const double wnew = (2.0*sol/G.area[wi] - din*wo)/rio;
+            W(WS_BX, j, c) = 2.0/(G.area[wi]*rio)*bw;       /* cotangent of sol for now */
+            bo_wind[wi] += -din/rio*bw;
+            const double bdin = -wo/rio*bw;
+            W(WS_BRIO, j, c) += -wnew/rio*bw;
+            W(WS_BRE, j, c) += 0.5*bdin;
+            W(WS_BRE, j + 1, c) += 0.5*bdin;
+        }
+        /* rt_new and rho_new from sol */
+        for (int j = 0; j < n; ++j) {
+            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
+            const double sol = W(WS_X, j, c)/(1.0 + dt*G.damp[b]);
+            const double grt = W(WS_BRTE, j, c)/G.vol[b] - W(WS_BRTE, j + 1, c)/G.vol[lo];
+            const double grho = W(WS_BRE, j, c)/G.vol[b] - W(WS_BRE, j + 1, c)/G.vol[lo];
+            W(WS_BTIN, j, c) += dt*sol*grt;
+            W(WS_BX, j, c) += dt*W(WS_TIN, j, c)*grt + dt*grho;
+        }
+        /* damping, then the transposed solve A^T rb = xb (A^T: lower e, upper c) */
+        for (int j = 0; j < n; ++j) W(WS_BX, j, c) = W(WS_BX, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]);
+        {
+            /* forward sweep of A^T into WS_BR (quotients reuse WS_EP as scratch) */
+            double ep = n > 1 ? W(WS_C, 0, c)/W(WS_D, 0, c) : 0.0, rp = W(WS_BX, 0, c)/W(WS_D, 0, c);
+            W(WS_EP, 0, c) = ep;
+            W(WS_BR, 0, c) = rp;
+            for (int j = 1; j < n; ++j) {
+                const double den = W(WS_D, j, c) - ep*W(WS_E, j - 1, c);
+                if (j < n - 1) ep = W(WS_C, j, c)/den;
+                rp = (W(WS_BX, j, c) - rp*W(WS_E, j - 1, c))/den;
+                W(WS_EP, j, c) = ep;
+                W(WS_BR, j, c) = rp;
+            }
+            double x = W(WS_BR, n - 1, c);
+            for (int j = n - 2; j >= 0; --j) {
+                x = W(WS_BR, j, c) - W(WS_EP, j, c)*x;
+                W(WS_BR, j, c) = x;
+            }
+        }
+        /* the new state's rho and rt cotangents now feed re and rte */
+        /* coefficients: bands -(rb x^T), right-hand side rb */
+        for (int j = 0; j < n; ++j) {
+            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
+            const i64 wi = widx(G, j + 1, c);
+            const double x = W(WS_X, j, c), rb = W(WS_BR, j, c);
+            const double bd = -rb*x, br = rb;
+            const double tin = W(WS_TIN, j, c), rio = W(WS_RIO, j, c), wo = wind_o[wi];
+            const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
+            const double q = dz/(iw*(dt*dt)*GC_C_D_P);
+            const double ga0 = W(WS_GA, j, c), ga1 = W(WS_GA, j + 1, c);
+            const double be0 = W(WS_BE, j, c), be1 = W(WS_BE, j + 1, c);
+            /* r = -(wo + dt tw) q + tin (pe_j - pe_j+1)/dt + 0.5/dt (te_j + te_j+1) dex - q wo rie/rio */
+            double bwo = -q*br - q*W(WS_RIE, j, c)/rio*br;
+            bt_wind[wi] += -dt*q*br;
+            double btin = (W(WS_EPE, j, c) - W(WS_EPE, j + 1, c))/dt*br;
+            W(WS_BEPE, j, c) += tin/dt*br;
+            W(WS_BEPE, j + 1, c) -= tin/dt*br;
+            W(WS_BTPE, j, c) += 0.5/dt*dex*br;
+            W(WS_BTPE, j + 1, c) += 0.5/dt*dex*br;
+            W(WS_BRIE, j, c) += -q*wo/rio*br;
+            double brio = q*wo*W(WS_RIE, j, c)/(rio*rio)*br;
+            /* d = -tin^2 (ga_j + ga_j+1) + 0.5 dex (al_j+1 - al_j + tin (be_j+1 - be_j))
+                   - dz/(iw dt^2 cp rio) (2/A + dt wo 0.5 (-1/vol_j + 1/vol_j+1)) */
+            const double kk = 2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]);
+            btin += (-2.0*tin*(ga0 + ga1) + 0.5*dex*(be1 - be0))*bd;
+            W(WS_BGA, j, c) += -(tin*tin)*bd;
+            W(WS_BGA, j + 1, c) += -(tin*tin)*bd;
+            W(WS_BAL, j + 1, c) += 0.5*dex*bd;
+            W(WS_BAL, j, c) -= 0.5*dex*bd;
+            W(WS_BBE, j + 1, c) += 0.5*dex*tin*bd;
+            W(WS_BBE, j, c) -= 0.5*dex*tin*bd;
+            brio += dz/(iw*(dt*dt)*GC_C_D_P*rio*rio)*kk*bd;
+            bwo += -dz/(iw*(dt*dt)*GC_C_D_P*rio)*dt*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo])*bd;
+            if (j < n - 1) {
+                /* row j+1's lower band c[j] = tin_j+1 ga_j+1 tin_j + 0.5 dex1 (al_j+1 + be_j+1 tin_j)
+                       - dz1/(iw dt cp) 0.5 wo_j+1/(vol_j+1 rio_j+1) */
+                const i64 l2 = sidx(G, j + 2, c);
+                const double bc = -W(WS_BR, j + 1, c)*x;
+                const double be_ = -rb*W(WS_X, j + 1, c);
+                const double dz1 = G.zs[lo] - G.zs[l2], dex1 = G.exbg[lo] - G.exbg[l2];
+                const double tin1 = W(WS_TIN, j + 1, c), rio1 = W(WS_RIO, j + 1, c);
+                const double wo1 = wind_o[widx(G, j + 2, c)];
+                W(WS_BTIN, j + 1, c) += ga1*tin*bc;
+                W(WS_BGA, j + 1, c) += tin1*tin*bc;
+                btin += (tin1*ga1 + 0.5*dex1*be1)*bc;
+                W(WS_BAL, j + 1, c) += 0.5*dex1*bc;
+                W(WS_BBE, j + 1, c) += 0.5*dex1*tin*bc;
+                const double k1 = dz1/(iw*dt*GC_C_D_P)*0.5;
+                bo_wind[widx(G, j + 2, c)] += -k1/(G.vol[lo]*rio1)*bc;
+                W(WS_BRIO, j + 1, c) += k1*wo1/(G.vol[lo]*rio1*rio1)*bc;
+                /* upper band e[j] = tin_j ga_j+1 tin_j+1 - 0.5 dex (al_j+1 + be_j+1 tin_j+1)
+                       + dz/(iw dt cp) 0.5 wo_j/(vol_j+1 rio_j) */
+                btin += ga1*tin1*be_;
+                W(WS_BGA, j + 1, c) += tin*tin1*be_;
+                W(WS_BTIN, j + 1, c) += (tin*ga1 - 0.5*dex*be1)*be_;
+                W(WS_BAL, j + 1, c) -= 0.5*dex*be_;
+                W(WS_BBE, j + 1, c) -= 0.5*dex*tin1*be_;
+                const double k0 = dz/(iw*dt*GC_C_D_P)*0.5;
+                bwo += k0/(G.vol[lo]*rio)*be_;
+                brio -= k0*wo/(G.vol[lo]*rio*rio)*be_;
+            }
+            W(WS_BTIN, j, c) += btin;
+            W(WS_BRIO, j, c) += brio;
+            bo_wind[wi] += bwo;
+        }
+        /* interface means: rie, rio, tin */
+        for (int j = 0; j < n; ++j) {
+            W(WS_BRE, j, c) += 0.5*W(WS_BRIE, j, c);
+            W(WS_BRE, j + 1, c) += 0.5*W(WS_BRIE, j, c);
+        }
+        /* per layer: theta_e, pi_e, re, rte, alpha, beta, gamma */
+        for (int j = 0; j < L; ++j) {
+            const i64 b = sidx(G, j, c);
+            const double ro = rho_o[b], rto = rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
+            const double tr = t_rho[b], tt = t_rt[b];
+            const double al = W(WS_AL, j, c), be = W(WS_BE, j, c), ga = W(WS_GA, j, c);
+            double bal = W(WS_BAL, j, c), bbe = W(WS_BBE, j, c), bga = W(WS_BGA, j, c);
+            const double btpe = W(WS_BTPE, j, c), bepe = W(WS_BEPE, j, c);
+            double btr = 0.0, btt = 0.0, bro = 0.0, brto = 0.0, btpo = 0.0, bepo = 0.0;
+            double brn = 0.0, brtn = 0.0, bepn = 0.0;
+            /* ep_new's ep_o and rt_o terms */
+            bepo += b_ep[b];
+            brto += -vol*ga*b_ep[b];
+            /* theta_e = tp_o + dt vol (al tr + be tt) */
+            btpo += btpe;
+            bal += dt*vol*tr*btpe;
+            bbe += dt*vol*tt*btpe;
+            btr += dt*vol*al*btpe;
+            btt += dt*vol*be*btpe;
+            /* pi_e = ep_o + dt vol ga tt */
+            bepo += bepe;
+            bga += dt*vol*tt*bepe;
+            btt += dt*vol*ga*bepe;
+            /* re = rho_o + dt tr, rte = rt_o + dt tt */
+            const double bre = W(WS_BRE, j, c), brte = W(WS_BRTE, j, c);
+            bro += bre;
+            btr += dt*bre;
+            brto += brte;
+            btt += dt*brte;
+            /* rho_i = mean of rho_o (interfaces j-1 and j) */
+            if (j < n) bro += 0.5*W(WS_BRIO, j, c);
+            if (j > 0) bro += 0.5*W(WS_BRIO, j - 1, c);
+            /* theta_i = mean of rt_n/rho_n */
+            double btin_sum = 0.0;
+            if (j < n) btin_sum += 0.5*W(WS_BTIN, j, c);
+            if (j > 0) btin_sum += 0.5*W(WS_BTIN, j - 1, c);
+            const double rn = rk == 0 ? ro : rho_n[b], rtn = rk == 0 ? rto : rt_n[b];
+            brtn += btin_sum/rn;
+            brn += -btin_sum*rtn/(rn*rn);
+            /* alpha, beta, gamma */
+            if (rk == 0) {
+                /* al = -rto/ro^2/vol, be = 1/ro/vol, ga = R/(cv rto) (exbg + ep_o)/vol */
+                brto += -bal/(ro*ro)/vol;
+                bro += 2.0*rto/(ro*ro*ro)/vol*bal;
+                bro += -bbe/(ro*ro)/vol;
+                brto += -ga/rto*bga;
+                bepo += GC_R_D/(GC_C_D_V*rto)/vol*bga;
+            } else {
+                const double rnn = rho_n[b], rtnn = rt_n[b];
+                const double ho = (1.0 - pdw)/vol, hn = pdw/vol;
+                brto += -ho*bal/(ro*ro);
+                bro += ho*2.0*rto/(ro*ro*ro)*bal;
+                brtn += -hn*bal/(rnn*rnn);
+                brn += hn*2.0*rtnn/(rnn*rnn*rnn)*bal;
+                bro += -ho*bbe/(ro*ro);
+                brn += -hn*bbe/(rnn*rnn);
+                const double go = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b]);
+                const double gn = GC_R_D/(GC_C_D_V*rtnn)*(exbg + ep_n[b]);
+                brto += -ho*go/rto*bga;
+                bepo += ho*GC_R_D/(GC_C_D_V*rto)*bga;
+                brtn += -hn*gn/rtnn*bga;
+                bepn += hn*GC_R_D/(GC_C_D_V*rtnn)*bga;
+            }
+            bt_rho[b] += btr;
+            bt_rt[b] += btt;
+            bo_tp[b] += btpo;
+            bo_ep[b] += bepo;
+            if (rk == 0) {
+                bo_rho[b] += bro + brn;
+                bo_rt[b] += brto + brtn;
+            } else {
+                bo_rho[b] += bro;
+                bo_rt[b] += brto;
+                bn_rho[b] += brn;
+                bn_rt[b] += brtn;
+                bn_ep[b] += bepn;
+            }
+        }
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* scalar_tendencies_expl.c:107-176 for the gas (dry): t_rho = -div(F),  */
+/* t_rt = -div(theta_v F), F = mean(rho) u, theta_v = rt/rho             */
+/* ------------------------------------------------------------------ */
+/* cotangent of the vertical contravariant correction at (cell, level):
+   it enters the box below the level with +area/vol and the box above with
+   -area/vol (divergences.c:41-59) */
+GAME_HD inline double vcc_cot_h(const DevGrid &G, const double *db, int c, int lev)
+{
+    const i64 s0 = (i64)lev*G.nsh + c, s1 = s0 - G.nsh;
+    const double a = G.area[c + (i64)lev*G.vpl];
+    return db[s0]/G.vol[s0]*a - db[s1]/G.vol[s1]*a;
+}
+
+/* (divv_h^T db) at edge e of layer l (divergences.c:14-64, averaging.c:41-95) */
+GAME_HD inline double divv_adj_edge(const DevGrid &G, const AdjTables &A, const double *db, int l, int e)
+{
+    const i64 vi = eidx(G, l, e);
+    const int fr = G.from[e], to = G.to[e];
+    const i64 sf = sidx(G, l, fr), st = sidx(G, l, to);
+    double r = G.area[vi]*(db[sf]/G.vol[sf] - db[st]/G.vol[st]);
+    if (!G.flat && l >= G.L - G.n_oro) {
+        const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
+        for (int p = 0; p < 2; ++p) {
+            const int c = cell[p];
+            const double k = -0.5*G.ipw[8*sidx(G, l, c) + slot[p]]*G.slope[vi];
+            double v = vcc_cot_h(G, db, c, l);
+            if (l + 1 <= G.L - 1) v += vcc_cot_h(G, db, c, l + 1);
+            r += k*v;
+        }
+    }
+    return r;
+}
+
+/* per (layer, edge): the cotangents of F and of theta_v F; ub += mean(rho) Fb */
+struct ScalarTendAdjE {
+    DevGrid G;
+    AdjTables A;
+    const double *rho, *rt, *wind;          /* the substep's densities and updated wind */
+    const double *b_trho, *b_trt;           /* cotangents of the two tendencies */
+    double *Fb, *Fthb;                      /* L x NVH, written */
+    double *b_wind;                         /* accumulated at the edges */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= (i64)G.L*G.nvh) return;
+        const int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+        const i64 sf = sidx(G, l, G.from[e]), st = sidx(G, l, G.to[e]);
+        const double fthb = -divv_adj_edge(G, A, b_trt, l, e);
+        const double fb = -divv_adj_edge(G, A, b_trho, l, e) + 0.5*(rt[sf]/rho[sf] + rt[st]/rho[st])*fthb;
+        Fb[t] = fb;
+        Fthb[t] = fthb;
+        b_wind[eidx(G, l, e)] += 0.5*(rho[sf] + rho[st])*fb;
+    }
+};
+
+/* per (layer, cell): the densities' cotangents from F and theta_v */
+struct ScalarTendAdjC {
+    DevGrid G;
+    const double *rho, *rt, *wind;
+    const double *Fb, *Fthb;
+    double *b_rho, *b_rt;                   /* accumulated */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.s) return;
+        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        const int ne = c < GAME_N_PENT ? 5 : 6;
+        double thvb = 0.0, rb = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const int e = G.adj[6*c + j];
+            const i64 vi = eidx(G, l, e), k = (i64)l*G.nvh + e;
+            const double F = 0.5*(rho[sidx(G, l, G.from[e])] + rho[sidx(G, l, G.to[e])])*wind[vi];
+            thvb += 0.5*F*Fthb[k];
+            rb += 0.5*wind[vi]*Fb[k];
+        }
+        const double r = rho[t];
+        b_rho[t] += rb - thvb*rt[t]/(r*r);
+        b_rt[t] += thvb/r;
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* scalar_times_vector (multiplications.c:35-117), F = s v: its two     */
+/* cotangents                                                            */
+/* ------------------------------------------------------------------ */
+/* sb (per layer, cell) += factor * sum over the F's that read s of 0.5 v Fb;
+   v = v1 (+ v2 when given) */
+struct StvAdjS {
+    DevGrid G;
+    const double *v1, *v2, *Fb;
+    double factor;
+    int with_w;                             /* 1: scalar_times_vector (edges and interior w); 0: _h */
+    double *sb;                             /* accumulated */
+    GAME_HD double v(i64 i) const { return v2 ? v1[i] + v2[i] : v1[i]; }
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.s) return;
+        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        const int ne = c < GAME_N_PENT ? 5 : 6;
+        double r = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const i64 vi = eidx(G, l, G.adj[6*c + j]);
+            r += 0.5*v(vi)*Fb[vi];
+        }
+        if (with_w) {
+            if (l >= 1) r += 0.5*v(widx(G, l, c))*Fb[widx(G, l, c)];
+            if (l + 1 <= G.L - 1) r += 0.5*v(widx(G, l + 1, c))*Fb[widx(G, l + 1, c)];
+        }
+        sb[t] += factor*r;
+    }
+};
+
+/* vb (V) (+)= factor * mean(s) Fb at the edges and interior w points, s = s1 (+ s2) */
+struct StvAdjV {
+    DevGrid G;
+    const double *s1, *s2, *Fb;
+    double factor;
+    int with_w, accumulate;
+    double *vb;
+    GAME_HD double s(i64 i) const { return s2 ? s1[i] + s2[i] : s1[i]; }
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.v) return;
+        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        double r = 0.0;
+        if (h >= G.nsh) {
+            const int e = h - G.nsh;
+            r = factor*0.5*(s(sidx(G, lev, G.from[e])) + s(sidx(G, lev, G.to[e])))*Fb[t];
+        } else if (with_w && lev >= 1 && lev <= G.L - 1) {
+            r = factor*0.5*(s(sidx(G, lev - 1, h)) + s(sidx(G, lev, h)))*Fb[t];
+        }
+        if (accumulate) vb[t] += r;
+        else vb[t] = r;
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* vorticity_flux.c:16-187 in the PV, for a fixed mass flux F            */
+/* ------------------------------------------------------------------ */
+GAME_HD inline double vflux_vert_weight(const DevGrid &G, int lev)
+{
+    return (lev == 0 || lev == G.L) ? 1.0 : 0.5;
+}
+
+struct VFluxPvAdj {
+    DevGrid G;
+    AdjTables A;
+    PvReverse R;
+    const double *F, *ob;                   /* mass flux (V), cotangent of the output (V) */
+    double *pvb;                            /* curl, written */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.curl) return;
+        const int nvh = G.nvh, L = G.L;
+        const int lev = (int)(t/(2*nvh)), h = (int)(t - (i64)lev*2*nvh);
+        double r = 0.0;
+        if (h >= nvh) {
+            const int m = h - nvh;
+            for (int k = 0; k < PVR_MAX; ++k) {
+                const i64 km = (i64)k*nvh + m;
+                const double w = R.w[km];
+                if (w != 0.0) r += w*F[eidx(G, lev, R.f[km])]*ob[eidx(G, lev, R.e[km])];
+            }
+        } else {
+            const int e = h;
+            const int fr = G.from[e], to = G.to[e];
+            /* the non-standard term of the edge in layer lev (ipw slot 6, w at
+               level lev) and in layer lev - 1 (slot 7, w at level lev) */
+            if (lev <= L - 1)
+                r += -0.5*ob[eidx(G, lev, e)]*(G.ipw[8*sidx(G, lev, fr) + 6]*F[widx(G, lev, fr)]
+                                               + G.ipw[8*sidx(G, lev, to) + 6]*F[widx(G, lev, to)]);
+            if (lev >= 1)
+                r += -0.5*ob[eidx(G, lev - 1, e)]*(G.ipw[8*sidx(G, lev - 1, fr) + 7]*F[widx(G, lev, fr)]
+                                                   + G.ipw[8*sidx(G, lev - 1, to) + 7]*F[widx(G, lev, to)]);
+            /* the vertical outputs of the edge's two cells at level lev */
+            const double vw = vflux_vert_weight(G, lev);
+            const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
+            for (int p = 0; p < 2; ++p) {
+                const int c = cell[p];
+                double s = 0.0;
+                if (lev >= 1) s += G.ipw[8*sidx(G, lev - 1, c) + slot[p]]*F[eidx(G, lev - 1, e)];
+                if (lev <= L - 1) s += G.ipw[8*sidx(G, lev, c) + slot[p]]*F[eidx(G, lev, e)];
+                r += vw*s*ob[widx(G, lev, c)];
+            }
+        }
+        pvb[t] = r;
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* calc_pot_vort, vorticities.c:18-85: pv = (rel_vort + f)/density        */
+/* ------------------------------------------------------------------ */
+/* the density of curl element t (rhombus: four-cell weighted mean; half
+   level: the edge mean above and below, extrapolated at top and surface) */
+GAME_HD inline double pv_density(const DevGrid &G, const double *rho, int lev, int h)
+{
+    const int nvh = G.nvh, nsh = G.nsh, L = G.L;
+    if (h >= nvh) {
+        const int e = h - nvh;
+        double dv = 0.0;
+        for (int j = 0; j < 4; ++j) dv += G.drw[4*e + j]*rho[(i64)lev*nsh + G.dri[4*e + j]];
+        return dv;
+    }
+    const int e = h, fr = G.from[e], to = G.to[e];
+    if (lev == 0) {
+        const double m0 = 0.5*(rho[fr] + rho[to]), m1 = 0.5*(rho[fr + nsh] + rho[to + nsh]);
+        return m0 + (m0 - m1)/G.pv_top_dz1[e]*G.pv_top_dz2[e];
+    }
+    if (lev == L) {
+        const i64 a = (i64)(L - 1)*nsh, b = (i64)(L - 2)*nsh;
+        const double ma = 0.5*(rho[a + fr] + rho[a + to]), mb = 0.5*(rho[b + fr] + rho[b + to]);
+        return ma + (mb - ma)/G.pv_sfc_dz1[e]*G.pv_sfc_dz2[e];
+    }
+    const i64 uf = (i64)(lev - 1)*nsh + fr, ut = (i64)(lev - 1)*nsh + to;
+    return 0.25*(rho[uf] + rho[ut] + rho[uf + nsh] + rho[ut + nsh]);
+}
+
+struct PvDivAdj {
+    DevGrid G;
+    const double *pvb, *pv, *rho;
+    double *rvb, *dvb;                      /* curl, written */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.curl) return;
+        const int lev = (int)(t/(2*G.nvh)), h = (int)(t - (i64)lev*2*G.nvh);
+        const double dv = pv_density(G, rho, lev, h);
+        rvb[t] = pvb[t]/dv;
+        dvb[t] = -pvb[t]*pv[t]/dv;
+    }
+};
+
+/* rhombus rows: rel_vort = (A_f zeta_f + A_t zeta_t)/(A_f + A_t)
+   (vorticities.c:120-130); per (layer, triangle) over its three edges */
+struct RhombusVortAdj {
+    DevGrid G;
+    const double *rvb;                      /* curl */
+    double *zb;                             /* L x NT, written */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= (i64)G.L*G.nt) return;
+        const int l = (int)(t/G.nt), tri = (int)(t - (i64)l*G.nt);
+        const i64 base = G.nvh + (i64)l*G.dvpl;
+        const double at = G.darea[base + tri];
+        double r = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const int e = G.vit[3*tri + k];
+            r += at/(G.darea[base + G.dfrom[e]] + G.darea[base + G.dto[e]])*rvb[(i64)l*2*G.nvh + G.nvh + e];
+        }
+        zb[t] = r;
+    }
+};
+
+/* tangential rows (vorticities.c:132-163): the cotangent of the horizontal
+   covariant component of each edge; the top row copies level 1 */
+GAME_HD inline double tan_row_cot(const DevGrid &G, const double *rvb, int lev, int e)
+{
+    double r = rvb[(i64)lev*2*G.nvh + e];
+    if (lev == 1) r += rvb[e];
+    return r;
+}
+
+struct TanVortAdjE {
+    DevGrid G;
+    const double *rvb;
+    double *cvb;                            /* L x NVH, written: cotangent of the covariant component */
+    double *b_wind;                         /* accumulated at the edges */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= (i64)G.L*G.nvh) return;
+        const int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+        const i64 vi = eidx(G, l, e);
+        double r = G.nd[vi]/G.darea[(i64)(l + 1)*G.dvpl + e]*tan_row_cot(G, rvb, l + 1, e);
+        if (l >= 1) r += -G.nd[vi]/G.darea[(i64)l*G.dvpl + e]*tan_row_cot(G, rvb, l, e);
+        cvb[t] = r;
+        b_wind[vi] += r;
+    }
+};
+
+/* per (level, cell): the w cotangents of the tangential rows and of the
+   terrain part of the covariant components (averaging.c:14-39, 97-111) */
+struct TanVortAdjC {
+    DevGrid G;
+    AdjTables A;
+    const double *rvb, *cvb;
+    double *b_wind;                         /* accumulated at the w points */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= (i64)(G.L + 1)*G.nsh) return;
+        const int lev = (int)(t/G.nsh), c = (int)(t - (i64)lev*G.nsh);
+        const int ne = c < GAME_N_PENT ? 5 : 6, L = G.L, o0 = G.L - G.n_oro;
+        const i64 wi = widx(G, lev, c);
+        double r = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const int e = G.adj[6*c + j];
+            if (lev >= 1 && lev <= L - 1) {
+                const double s = G.from[e] == c ? 1.0 : -1.0;
+                r += s*G.nd[wi]/G.darea[(i64)lev*G.dvpl + e]*tan_row_cot(G, rvb, lev, e);
+            }
+            if (!G.flat) {
+                if (lev >= o0 && lev <= L - 1)
+                    r += 0.5*G.slope[eidx(G, lev, e)]*G.ipw[8*sidx(G, lev, c) + 6]*cvb[(i64)lev*G.nvh + e];
+                if (lev - 1 >= o0 && lev - 1 < L - 1)
+                    r += 0.5*G.slope[eidx(G, lev - 1, e)]*G.ipw[8*sidx(G, lev - 1, c) + 7]
+                         *cvb[(i64)(lev - 1)*G.nvh + e];
+            }
+        }
+        b_wind[wi] += r;
+    }
+};
+
+/* density cotangents of the half-level (tangential) rows, per (layer, cell) */
+struct HalfDensAdj {
+    DevGrid G;
+    const double *dvb;
+    double *b_rho;                          /* accumulated */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.s) return;
+        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        const int ne = c < GAME_N_PENT ? 5 : 6, L = G.L;
+        const i64 nv2 = 2*(i64)G.nvh;
+        double r = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const int e = G.adj[6*c + j];
+            if (l >= 1) r += 0.25*dvb[(i64)l*nv2 + e];
+            if (l + 1 <= L - 1) r += 0.25*dvb[(i64)(l + 1)*nv2 + e];
+            const double kt = G.pv_top_dz2[e]/G.pv_top_dz1[e], ks = G.pv_sfc_dz2[e]/G.pv_sfc_dz1[e];
+            if (l == 0) r += 0.5*(1.0 + kt)*dvb[e];
+            if (l == 1) r += -0.5*kt*dvb[e];
+            if (l == L - 1) r += 0.5*(1.0 - ks)*dvb[(i64)L*nv2 + e];
+            if (l == L - 2) r += 0.5*ks*dvb[(i64)L*nv2 + e];
+        }
+        b_rho[t] += r;
+    }
+};
+
+/* density cotangents of the rhombus rows (vorticities.c:33-43), per (layer, cell) */
+struct RhombusDensAdj {
+    DevGrid G;
+    AdjTables A;
+    const double *dvb;
+    double *b_rho;                          /* accumulated */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.s) return;
+        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        double r = 0.0;
+        for (int k = 0; k < ADJ_RH_MAX; ++k) {
+            const double w = A.rhp_w[(i64)k*G.nsh + c];
+            if (w != 0.0) r += w*dvb[(i64)l*2*G.nvh + G.nvh + A.rhp_e[(i64)k*G.nsh + c]];
+        }
+        b_rho[t] += r;
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* vector_tendencies_expl.c:74-130 and the wind update, manage_rkhevi.c:67-75 */
+/* ------------------------------------------------------------------ */
+/* second substep: tend1 = 0.5 tend0 + 0.5 (ohw pgrad_old - chw P + adv1) at
+   edges, 0.5 tend0 + 0.5 (-cvw P + adv1) at interior w points; its cotangent
+   tb1 (edges: dt times that of the updated wind) goes to tend0, pgrad_old, P
+   and the advection term adv1 (plus the next step's cotangent of adv) */
+struct TendAdjRk1 {
+    DevGrid G;
+    double ohw, chw, cvw;
+    const double *tb1;                      /* cotangent of tend1 (V) */
+    const double *advb_next;                /* cotangent of this step's adv from the next step */
+    double *tb0, *poldb, *Pb;               /* accumulated */
+    double *advb;                           /* written: cotangent of adv1 */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.v) return;
+        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        const bool edge = h >= G.nsh, inner_w = !edge && lev >= 1 && lev <= G.L - 1;
+        if (!edge && !inner_w) { advb[t] = advb_next[t]; return; }
+        const double b = tb1[t];
+        tb0[t] += 0.5*b;
+        if (edge) {
+            poldb[t] += 0.5*ohw*b;
+            Pb[t] += -0.5*chw*b;
+        } else {
+            Pb[t] += -0.5*cvw*b;
+        }
+        advb[t] = 0.5*b + advb_next[t];
+    }
+};
+
+/* first substep: tend0 = ohw pgrad_old - chw P + adv_prev (edges),
+   -cvw P + adv_prev (interior w) */
+struct TendAdjRk0 {
+    DevGrid G;
+    double ohw, chw, cvw;
+    const double *tb0;
+    double *poldb, *Pb;                     /* accumulated */
+    double *advb_prev;                      /* written: cotangent of the adv the step read */
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.v) return;
+        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        const bool edge = h >= G.nsh, inner_w = !edge && lev >= 1 && lev <= G.L - 1;
+        if (!edge && !inner_w) { advb_prev[t] = 0.0; return; }
+        const double b = tb0[t];
+        if (edge) {
+            poldb[t] += ohw*b;
+            Pb[t] += -chw*b;
+        } else {
+            Pb[t] += -cvw*b;
+        }
+        advb_prev[t] = b;
+    }
+};
+
+/* ------------------------------------------------------------------ */
+/* element-wise pieces                                                   */
+/* ------------------------------------------------------------------ */
+/* y (+)= a*x over n elements */
+struct Axpy {
+    i64 n;
+    double a;
+    const double *x;
+    double *y;
+    int accumulate;
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= n) return;
+        y[t] = accumulate ? y[t] + a*x[t] : a*x[t];
+    }
+};
+
+/* the wind update's edges: b_wind += ub (edges), tb (+)= dt*ub (edges) */
+struct WindUpdateAdj {
+    DevGrid G;
+    double dt;
+    const double *ub;
+    double *b_wind, *tb;
+    int accumulate_tb;
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.v) return;
+        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        if (h < G.nsh) return;
+        (void)lev;
+        b_wind[t] += ub[t];
+        tb[t] = accumulate_tb ? tb[t] + dt*ub[t] : dt*ub[t];
+    }
+};
+
+/* theta_v_pert = rt/rho - theta_bg at a step boundary (the driver's copy
+   re-derives it, linear_combine_two_states.c:26): its cotangent moves to
+   rho and rt and is cleared */
+struct ThetaBoundaryAdj {
+    DevGrid G;
+    const double *rho, *rt;
+    double *b_rho, *b_rt, *b_tp;
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= G.s) return;
+        const double r = rho[t], bt = b_tp[t];
+        b_rho[t] -= bt*rt[t]/(r*r);
+        b_rt[t] += bt/r;
+        b_tp[t] = 0.0;
+    }
+};
+
+}  // namespace sadj
+
+#endif

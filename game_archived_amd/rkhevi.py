@@ -93,6 +93,16 @@ class DeviceModel:
         if n_steps > 0:
             self.first_step = 0
 
+    def run_writing_integrals(self, n_steps, dt, t_start, directory):
+        """the time loop with the reference's integral files (src/coordinator.c:302-336)"""
+        self._check(self.lib.game_gpu_run_steps_write_integrals(n_steps, dt, self.first_step, t_start,
+                                                                 directory.encode()))
+        if n_steps > 0:
+            self.first_step = 0
+
+    def write_integrals(self, time_since_init, directory, slot=NEW):
+        self._check(self.lib.game_gpu_write_integrals(slot, time_since_init, directory.encode()))
+
     def synchronize(self):
         self._check(self.lib.game_gpu_synchronize())
 

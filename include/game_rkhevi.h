@@ -267,6 +267,17 @@ int game_gpu_temperature_diagnostics(int slot);
    out[n_const+2] potential, out[n_const+3] internal energy. */
 int game_gpu_integrals(int slot, double *out);
 
+/* The three records write_out_integral appends to the files "masses",
+   "potential_temperature_density" and "energy" in `directory`
+   (src/io/write_output.c:1515-1596; same columns and %lf format), from
+   game_gpu_integrals of a slot at time_since_init. */
+int game_gpu_write_integrals(int slot, double time_since_init, const char *directory);
+/* n steps of the reference time loop with its integral output
+   (src/coordinator.c:302-336): each step as game_gpu_run_steps, then the
+   records of the new state at time t_start + (k + 1)*delta_t. */
+int game_gpu_run_steps_write_integrals(int n_steps, double delta_t, int first_step_bool, double t_start,
+                                       const char *directory);
+
 /* v_squared (S) and pot_vort_tend (V) of a host (wind, density) pair: the
    kinetic-energy and vorticity-flux terms of the initial hydrostatic balance
    (src/io/set_initial_state.c:155-166).  Uses the tendency slot as scratch,
@@ -333,6 +344,37 @@ int game_gpu_op_thomas_noalloc(int n, int ncol, const double *c, const double *d
 int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,
                                const double *x_bar, double *r_bar, double *c_bar, double *d_bar, double *e_bar,
                                double *work);
+
+/* ------------------------------------------------------------------ */
+/* Window of steps and its gradient (FP64 form of a dry model only).     */
+/* The reference has no adjoint.  Control variables: rho (S), rhotheta_v */
+/* (S) and wind (V), DEVICE pointers; theta_v_pert and exner_pert are    */
+/* diagnosed from them by the equation of state, exner = (R_d rhotheta_v */
+/* / p_0)^(R_d/c_v) (set_initial_state.c:302-314 written for rhotheta_v). */
+/* The first step takes the first-step branches (coordinator.c:299).     */
+/* The runs use the model's state slots and carried fields, which hold   */
+/* the window's last recomputed state afterwards.                        */
+/* ------------------------------------------------------------------ */
+/* n_steps steps from the control variables; outputs (device, may be null):
+   rho, rhotheta_v, exner_pert and wind of the final state */
+int game_gpu_window_forward(int n_steps, double dt, const double *rho0, const double *rt0, const double *wind0,
+                            double *rho_out, double *rt_out, double *exner_out, double *wind_out);
+/* the vector-Jacobian product of the window map (rho, rhotheta_v, wind)_0 ->
+   (rho, rhotheta_v, exner_pert, wind)_N: given cotangents of the final
+   state (device, null for zero), the cotangents of the control variables.
+   Forward sweep with checkpoints every checkpoint_every steps (state and
+   carried forcings), then per segment a recomputation that copies every
+   step's inputs and substep results, reversed step by step with the
+   hand-written adjoint kernels. */
+int game_gpu_window_vjp(int n_steps, double dt, int checkpoint_every, const double *rho0, const double *rt0,
+                        const double *wind0, const double *rho_bar_n, const double *rt_bar_n,
+                        const double *exner_bar_n, const double *wind_bar_n, double *rho_bar, double *rt_bar,
+                        double *wind_bar);
+/* J = 1/2 sum (rho_N - rho_target)^2 (weights 1) into *cost (host) and its
+   gradient with respect to the control variables (device) */
+int game_gpu_window_gradient(int n_steps, double dt, int checkpoint_every, const double *rho0, const double *rt0,
+                             const double *wind0, const double *rho_target, double *cost, double *rho_bar,
+                             double *rt_bar, double *wind_bar);
 
 #ifdef __cplusplus
 }
