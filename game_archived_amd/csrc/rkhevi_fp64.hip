@@ -641,32 +641,67 @@ __global__ void __launch_bounds__(TPB) k_vector_tend(DevGrid G, DevWorkT<TW> W, 
    exactly as scalar_times_vector_h forms them (multiplications.c:45-67). */
 __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
 {
-    i64 t = (i64)xcd_block().x*TPB + threadIdx.x;
-    const i64 n = (i64)G.n_oro*G.nsh;
-    if (t >= n) return;
-    const int nsh = G.nsh, L = G.L;
-    int lev = L - G.n_oro + (int)(t/nsh), h = (int)(t % nsh);
-    int ne = h < GAME_N_PENT ? 5 : 6;
+    /* one thread per cell walks a chunk of the terrain levels.  The
+       correction at level lev sums the cell's edge terms of layer lev - 1 and
+       then of layer lev, starting from zero (from layer lev alone at the top
+       terrain level): the running sum of layer lev - 1's terms is carried to
+       the next level, so each layer's edge terms are formed once, and the sums
+       are the reference's in the reference's order */
+    const Blk bk = xcd_block();
+    const int h = bk.x*TPB + threadIdx.x;
+    if (h >= G.nsh) return;
+    const int nsh = G.nsh, L = G.L, no = G.n_oro, top = L - no;
+    const i64 n = (i64)no*nsh;
+    const int ne = h < GAME_N_PENT ? 5 : 6;
     const double *rg = s.rho + (i64)G.n_cond*G.s;
-    double rw = 0, r1 = 0, r2 = 0;
-    for (int pass = (lev == L - G.n_oro) ? 1 : 0; pass < 2; ++pass) {
-        int l = pass == 0 ? lev - 1 : lev;
-        for (int i = 0; i < ne; ++i) {
-            int e = G.adj[6*h + i];
-            i64 si = (i64)l*nsh + h, vi = nsh + (i64)l*G.vpl + e;
-            i64 st = (i64)l*nsh + G.to[e], sf = (i64)l*nsh + G.from[e];
-            double u = s.wind[vi];
-            double f1 = (0.5*(rg[st] + rg[sf]))*u;
-            double f2 = (0.5*(s.rt[st]/rg[st] + s.rt[sf]/rg[sf]))*f1;
-            double c = -0.5*G.ipw[8*si + i]*G.slope[vi];
-            rw += c*u;
-            r1 += c*f1;
-            r2 += c*f2;
-        }
+    int ej[6], ct[6], cf[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int e = G.adj[6*h + (j < ne ? j : 0)];
+        ej[j] = e;
+        ct[j] = G.to[e];
+        cf[j] = G.from[e];
     }
-    vcc[t] = rw;
-    vcc[n + t] = r1;
-    vcc[2*n + t] = r2;
+    int c0, c1;
+    layer_chunk(no, bk.y, c0, c1);
+    /* the edge terms of layer l added to (aw, a1, a2) */
+    auto add_layer = [&](int l, double &aw, double &a1, double &a2, double &pw, double &p1, double &p2) {
+        const i64 si = (i64)l*nsh + h;
+        double tw[6], t1[6], t2[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const i64 vi = nsh + (i64)l*G.vpl + ej[j];
+            const i64 st = (i64)l*nsh + ct[j], sf = (i64)l*nsh + cf[j];
+            const double u = s.wind[vi];
+            const double f1 = (0.5*(rg[st] + rg[sf]))*u;
+            const double f2 = (0.5*(s.rt[st]/rg[st] + s.rt[sf]/rg[sf]))*f1;
+            const double c = -0.5*G.ipw[8*si + (j < ne ? j : 0)]*G.slope[vi];
+            tw[j] = c*u;
+            t1[j] = c*f1;
+            t2[j] = c*f2;
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (j < ne) {
+                aw += tw[j]; a1 += t1[j]; a2 += t2[j];
+                pw += tw[j]; p1 += t1[j]; p2 += t2[j];
+            }
+        }
+    };
+    double pw = 0.0, p1 = 0.0, p2 = 0.0;
+    if (c0 > 0) {
+        double dw = 0.0, d1 = 0.0, d2 = 0.0;
+        add_layer(top + c0 - 1, pw, p1, p2, dw, d1, d2);
+    }
+    for (int k = c0; k < c1; ++k) {
+        double rw = pw, r1 = p1, r2 = p2;
+        pw = 0.0; p1 = 0.0; p2 = 0.0;
+        add_layer(top + k, rw, r1, r2, pw, p1, p2);
+        const i64 t = (i64)k*nsh + h;
+        vcc[t] = rw;
+        vcc[n + t] = r1;
+        vcc[2*n + t] = r2;
+    }
 }
 
 /* scalar_tendencies_expl.c:89-177 for every constituent (diffusion off): the
@@ -2189,7 +2224,7 @@ static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &t
                           hipStream_t st)
 {
     if (!G.flat)
-        LAUNCH("k_vcc", k_vcc, GRID1((i64)G.n_oro*G.nsh), dim3(TPB), st, G, s, W.vcc);
+        LAUNCH("k_vcc", k_vcc, GRID2(G.nsh, (G.n_oro + 3)/4), dim3(TPB), st, G, s, W.vcc);
     if (G.n_const == 6) {
         if (G.flat) LAUNCH("k_scalar_tend", (k_scalar_tend<true, TW, 6>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
                            W, td.rho, td.rt, rk);

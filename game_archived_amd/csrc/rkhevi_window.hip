@@ -217,7 +217,7 @@ int window_alloc(Window &w, int k, int n_ckpt)
     for (double **p : ev) if ((rc = zeros(p, L*NVH))) return rc;
     double **cv[] = {&w.pvb, &w.rvb, &w.dvb};
     for (double **p : cv) if ((rc = zeros(p, CURL))) return rc;
-    if ((rc = zeros(&w.zb, L*NT)) || (rc = zeros(&w.ws, (size_t)sadj::WS_N*L*NSH))) return rc;
+    if ((rc = zeros(&w.zb, L*NT)) || (rc = zeros(&w.ws, (size_t)sadj::VW_N*L*NSH))) return rc;
     DevWork &A = w.AW;
     if ((rc = zeros(&A.flux, V)) || (rc = zeros(&A.rvt, L*NT)) || (rc = zeros(&A.pv, CURL)) || (rc = zeros(&A.pvt, V))
         || (rc = zeros(&A.vsq, S)) || (rc = zeros(&A.vsq_grad, V)))
@@ -353,6 +353,20 @@ int advection_adjoint(const double *wind, const double *rho, const double *advb,
    step produced (whose wind edges are next->wind), w.bP the cotangent of the
    step's pressure gradient nl + l, w.bA that of its second-substep advection;
    on return w.bS, w.bP, w.bA hold the same for the step before */
+/* the reverse of the sound-wave solver of one substep (column_solvers.c:19-383) */
+static void ver_waves_adjoint(const sadj::VwArgs &a, hipStream_t st)
+{
+    const DevGrid &G = a.G;
+    const i64 ni = (i64)(G.L - 1)*G.nsh;
+    launch("k_vw_layer_fwd", sadj::VwLayerFwd{a}, G.s, st);
+    launch("k_vw_iface_fwd", sadj::VwIfaceFwd{a}, ni, st);
+    launch("k_vw_solve", sadj::VwSolve{a, 0}, G.nsh, st);
+    launch("k_vw_iface_sol_bar", sadj::VwIfaceSolBar{a}, ni, st);
+    launch("k_vw_solve_t", sadj::VwSolve{a, 1}, G.nsh, st);
+    launch("k_vw_iface_bar", sadj::VwIfaceBar{a}, ni, st);
+    launch("k_vw_layer_bar", sadj::VwLayerBar{a}, G.s, st);
+}
+
 int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
 {
     Window &w = *Wn;
@@ -369,11 +383,11 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
         return rc;
     /* ---- second substep ---- */
     {
-        sadj::VerWavesAdj f{G, 1, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s1.rho, T.s1.rt, T.s1.ep,
-                            T.td1_rho, T.td1_rt, T.td1_wind, w.bS.rho, w.bS.rt, w.bS.wind, w.bS.tp, w.bS.ep,
-                            w.bSn.rho, w.bSn.rt, w.bSn.tp, w.bSn.ep, w.bSn.wind, w.bS1.rho, w.bS1.rt, w.bS1.ep,
-                            w.bt1_rho, w.bt1_rt, w.bt1_wind, w.ws};
-        launch("k_ver_waves_adj", f, G.nsh, st);
+        sadj::VwArgs a{G, 1, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s1.rho, T.s1.rt, T.s1.ep,
+                       T.td1_rho, T.td1_rt, T.td1_wind, w.bS.rho, w.bS.rt, w.bS.wind, w.bS.tp, w.bS.ep,
+                       w.bSn.rho, w.bSn.rt, w.bSn.tp, w.bSn.ep, w.bSn.wind, w.bS1.rho, w.bS1.rt, w.bS1.ep,
+                       w.bt1_rho, w.bt1_rt, w.bt1_wind, w.ws};
+        ver_waves_adjoint(a, st);
     }
     launch("k_pass_adj", PassThroughAdj{G, w.bS.wind, w.bU, w.bSn.wind}, G.v, st);
     launch("k_scalar_tend_adj_e", sadj::ScalarTendAdjE{G, A, T.s1.rho, T.s1.rt, next.wind, w.bt1_rho, w.bt1_rt, w.Fb,
@@ -386,11 +400,11 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
     if ((rc = advection_adjoint(T.s1.wind, T.s1.rho, w.bA1, w.bS1.wind, w.bS1.rho))) return rc;
     /* ---- first substep ---- */
     {
-        sadj::VerWavesAdj f{G, 0, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s.rho, T.s.rt, T.s.ep,
-                            T.td0_rho, T.td0_rt, T.td0_wind, w.bS1.rho, w.bS1.rt, w.bS1.wind, w.bS1.tp, w.bS1.ep,
-                            w.bSn.rho, w.bSn.rt, w.bSn.tp, w.bSn.ep, w.bSn.wind, w.bSn.rho, w.bSn.rt, w.bSn.ep,
-                            w.bt0_rho, w.bt0_rt, w.bt0_wind, w.ws};
-        launch("k_ver_waves_adj", f, G.nsh, st);
+        sadj::VwArgs a{G, 0, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s.rho, T.s.rt, T.s.ep,
+                       T.td0_rho, T.td0_rt, T.td0_wind, w.bS1.rho, w.bS1.rt, w.bS1.wind, w.bS1.tp, w.bS1.ep,
+                       w.bSn.rho, w.bSn.rt, w.bSn.tp, w.bSn.ep, w.bSn.wind, w.bSn.rho, w.bSn.rt, w.bSn.ep,
+                       w.bt0_rho, w.bt0_rt, w.bt0_wind, w.ws};
+        ver_waves_adjoint(a, st);
     }
     launch("k_pass_adj", PassThroughAdj{G, w.bS1.wind, w.bU, w.bSn.wind}, G.v, st);
     launch("k_scalar_tend_adj_e", sadj::ScalarTendAdjE{G, A, T.s.rho, T.s.rt, T.s1.wind, w.bt0_rho, w.bt0_rt, w.Fb,

@@ -1,7 +1,7 @@
 /*
  * step_adjoint.h -- the reverse (adjoint) of the dry RKHEVI step, as
  * element functors: each struct holds its arguments and computes, in
- * operator()(t), every output element owned by index t.  rkhevi_step_adjoint.hip
+ * operator()(t), every output element owned by index t.  rkhevi_window.hip
  * launches them (one thread per t) in the order of the reverse sweep.  The
  * reference has no adjoint; each functor cites the forward lines it reverses.
  *
@@ -37,24 +37,29 @@ GAME_HD inline i64 sidx(const DevGrid &G, int l, int h) { return (i64)l*G.nsh + 
 
 /* ------------------------------------------------------------------ */
 /* three_band_solver_ver_waves, column_solvers.c:19-383: one column     */
+/* (L layers, interfaces j = 0..L-2 between layers j and j+1 at level   */
+/* j+1):                                                                */
+/*   re = rho_o + dt t_rho, rte = rt_o + dt t_rt,                       */
+/*   alpha, beta, gamma (old state at rk 0; the mean of old and new at  */
+/*   rk 1), theta_e = tp_o + dt vol (alpha t_rho + beta t_rt),          */
+/*   pi_e = ep_o + dt vol gamma t_rt, rho_i = mean of rho_o,            */
+/*   rho_ie = mean of re, theta_i = mean of rt_n/rho_n,                 */
+/*   the bands c, d, e and right-hand side r, x = A^-1 r, damping,      */
+/*   then rho, rt, w, tp, ep of the new state.                          */
+/* The reverse runs as seven launches: the per-layer coefficients       */
+/* (VwLayerFwd), the per-interface bands (VwIfaceFwd), the solve        */
+/* (VwSolve, one thread per column), the cotangent of the solution      */
+/* (VwIfaceSolBar), the transposed solve (VwSolveT), the interface      */
+/* cotangents (VwIfaceBar) and the per-layer cotangents (VwLayerBar).   */
+/* Every launch but the two solves has one thread per (layer, column)   */
+/* or (interface, column), and every contribution an interface makes to */
+/* its neighbours is gathered by the neighbour, so no element has two  */
+/* writers.  The workspace holds VW_N arrays of L doubles per column,   */
+/* array k of column c element j at (k*L + j)*ncol + c.                 */
 /* ------------------------------------------------------------------ */
-/* Forward per column (L layers, interfaces j = 0..L-2 between layers j and
-   j+1 at level j+1):
-     re = rho_o + dt t_rho, rte = rt_o + dt t_rt,
-     alpha, beta, gamma (old state at rk 0; the mean of old and new at rk 1),
-     theta_e = tp_o + dt vol (alpha t_rho + beta t_rt), pi_e = ep_o + dt vol gamma t_rt,
-     rho_i = mean of rho_o, rho_ie = mean of re, theta_i = mean of rt_n/rho_n,
-     the three bands c, d, e and the right-hand side r, x = A^-1 r, damping,
-     then rho, rt, w, tp, ep of the new state.
-   The reverse recomputes the forward into a workspace (WS_N arrays of L
-   doubles per column, array k of column c element j at (k*L + j)*ncol + c,
-   so lanes of consecutive columns touch consecutive words) and walks back. */
-enum { WS_RE, WS_RTE, WS_AL, WS_BE, WS_GA, WS_TPE, WS_EPE, WS_RIO, WS_RIE, WS_TIN, WS_C, WS_D, WS_E, WS_R, WS_X,
-       WS_EP, WS_RNEW, WS_RTNEW,
-       /* reverse */
-       WS_BAL, WS_BBE, WS_BGA, WS_BTPE, WS_BEPE, WS_BRIO, WS_BRIE, WS_BTIN, WS_BRE, WS_BRTE, WS_BX, WS_BR, WS_N };
+enum { VW_AL, VW_BE, VW_GA, VW_TIN, VW_C, VW_D, VW_E, VW_R, VW_X, VW_EP, VW_BX, VW_BR, VW_BTIN, VW_BRIO, VW_N };
 
-struct VerWavesAdj {
+struct VwArgs {
     DevGrid G;
     int rk;
     double dt, iw;
@@ -72,301 +77,414 @@ struct VerWavesAdj {
     double *ws;
 
     GAME_HD double &W(int k, int j, int c) const { return ws[((i64)k*G.L + j)*G.nsh + c]; }
+    /* (layer or interface, column) of a flat thread index */
+    GAME_HD void split(i64 t, int &j, int &c) const
+    {
+        const unsigned u = (unsigned)t, n = (unsigned)G.nsh;
+        j = (int)(u/n);
+        c = (int)(u - (unsigned)j*n);
+    }
+    GAME_HD const double *rn_() const { return rk == 0 ? rho_o : rho_n; }
+    GAME_HD const double *rtn_() const { return rk == 0 ? rt_o : rt_n; }
+    GAME_HD double rio(int j, int c) const
+    {
+        return 0.5*(rho_o[sidx(G, j, c)] + rho_o[sidx(G, j + 1, c)]);
+    }
+    GAME_HD double re(int j, int c) const { const i64 b = sidx(G, j, c); return rho_o[b] + dt*t_rho[b]; }
+    GAME_HD double rte(int j, int c) const { const i64 b = sidx(G, j, c); return rt_o[b] + dt*t_rt[b]; }
+    GAME_HD double tpe(int j, int c) const
+    {
+        const i64 b = sidx(G, j, c);
+        return tp_o[b] + dt*G.vol[b]*(W(VW_AL, j, c)*t_rho[b] + W(VW_BE, j, c)*t_rt[b]);
+    }
+    GAME_HD double epe(int j, int c) const
+    {
+        const i64 b = sidx(G, j, c);
+        return ep_o[b] + dt*G.vol[b]*W(VW_GA, j, c)*t_rt[b];
+    }
+    GAME_HD double damp(int j, int c) const { return 1.0 + dt*G.damp[sidx(G, j, c)]; }
+    /* the damped solution at interface j */
+    GAME_HD double sol(int j, int c) const { return W(VW_X, j, c)/damp(j, c); }
+    /* rho and rt of the new state in layer j */
+    GAME_HD void rnew(int j, int c, double &rn, double &rtn) const
+    {
+        const int n = G.L - 1;
+        const double su = j < n ? sol(j, c) : 0.0, sd = j > 0 ? sol(j - 1, c) : 0.0;
+        const double tu = j < n ? W(VW_TIN, j, c) : 0.0, td = j > 0 ? W(VW_TIN, j - 1, c) : 0.0;
+        const double vol = G.vol[sidx(G, j, c)];
+        rn = re(j, c) + dt*(-sd + su)/vol;
+        rtn = rte(j, c) + dt*(-td*sd + tu*su)/vol;
+    }
+    /* w_new[j] = (2 sol/A - din wo)/rho_i, din = mean of rho_new: the
+       cotangent it gives din */
+    GAME_HD double bdin(int j, int c) const
+    {
+        const i64 wi = widx(G, j + 1, c);
+        return -wind_o[wi]/rio(j, c)*b_wind[wi];
+    }
+    /* cotangents of rho_new and rt_new in layer j from the new state's rho,
+       rt, tp (= rt/rho - thbg), ep (= ep_o + vol ga (rt - rt_o)) and w */
+    GAME_HD void bnew(int j, int c, double rn, double rtn, double &bre, double &brte) const
+    {
+        const int n = G.L - 1;
+        const i64 b = sidx(G, j, c);
+        const double bep = b_ep[b], btp = b_tp[b];
+        brte = b_rt[b] + G.vol[b]*W(VW_GA, j, c)*bep + btp/rn;
+        bre = b_rho[b] - btp*rtn/(rn*rn);
+        if (j > 0) bre += 0.5*bdin(j - 1, c);
+        if (j < n) bre += 0.5*bdin(j, c);
+    }
+};
 
+/* alpha, beta, gamma of layer j (column_solvers.c:99-121) */
+struct VwLayerFwd {
+    VwArgs a;
     GAME_HD void operator()(i64 t) const
     {
-        if (t >= G.nsh) return;
-        const int c = (int)t, L = G.L, n = L - 1;
-        const double pdw = 0.5;
-        /* ---- forward ---- */
-        for (int j = 0; j < L; ++j) {
-            const i64 b = sidx(G, j, c);
-            const double ro = rho_o[b], rto = rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
-            const double tr = t_rho[b], tt = t_rt[b];
-            double al, be, ga;
-            if (rk == 0) {
-                al = -rto/(ro*ro)/vol;
-                be = 1.0/ro/vol;
-                ga = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b])/vol;
-            } else {
-                const double rn = rho_n[b], rtn = rt_n[b];
-                const double ao = -rto/(ro*ro), bo = 1.0/ro, go = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b]);
-                const double an = -rtn/(rn*rn), bn = 1.0/rn, gn = GC_R_D/(GC_C_D_V*rtn)*(exbg + ep_n[b]);
-                al = ((1.0 - pdw)*ao + pdw*an)/vol;
-                be = ((1.0 - pdw)*bo + pdw*bn)/vol;
-                ga = ((1.0 - pdw)*go + pdw*gn)/vol;
+        const DevGrid &G = a.G;
+        if (t >= G.s) return;
+        int j, c;
+        a.split(t, j, c);
+        const i64 b = sidx(G, j, c);
+        const double ro = a.rho_o[b], rto = a.rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
+        double al, be, ga;
+        if (a.rk == 0) {
+            al = -rto/(ro*ro)/vol;
+            be = 1.0/ro/vol;
+            ga = GC_R_D/(GC_C_D_V*rto)*(exbg + a.ep_o[b])/vol;
+        } else {
+            const double pdw = 0.5;
+            const double rn = a.rho_n[b], rtn = a.rt_n[b];
+            const double ao = -rto/(ro*ro), bo = 1.0/ro, go = GC_R_D/(GC_C_D_V*rto)*(exbg + a.ep_o[b]);
+            const double an = -rtn/(rn*rn), bn = 1.0/rn, gn = GC_R_D/(GC_C_D_V*rtn)*(exbg + a.ep_n[b]);
+            al = ((1.0 - pdw)*ao + pdw*an)/vol;
+            be = ((1.0 - pdw)*bo + pdw*bn)/vol;
+            ga = ((1.0 - pdw)*go + pdw*gn)/vol;
+        }
+        a.W(VW_AL, j, c) = al;
+        a.W(VW_BE, j, c) = be;
+        a.W(VW_GA, j, c) = ga;
+    }
+};
+
+/* theta_i, the bands and the right-hand side of interface j
+   (column_solvers.c:123-190) */
+struct VwIfaceFwd {
+    VwArgs a;
+    GAME_HD double tin(int j, int c) const
+    {
+        const double *rn = a.rn_(), *rtn = a.rtn_();
+        const i64 b = sidx(a.G, j, c), lo = sidx(a.G, j + 1, c);
+        return 0.5*(rtn[b]/rn[b] + rtn[lo]/rn[lo]);
+    }
+    GAME_HD void operator()(i64 t) const
+    {
+        const DevGrid &G = a.G;
+        const int n = G.L - 1;
+        if (t >= (i64)n*G.nsh) return;
+        int j, c;
+        a.split(t, j, c);
+        const double dt = a.dt, iw = a.iw;
+        const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c), wi = widx(G, j + 1, c);
+        const double ti = tin(j, c), rio = a.rio(j, c), wo = a.wind_o[wi];
+        const double rie = 0.5*(a.re(j, c) + a.re(j + 1, c));
+        const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
+        const double q = dz/(iw*(dt*dt)*GC_C_D_P);
+        const double ga0 = a.W(VW_GA, j, c), ga1 = a.W(VW_GA, j + 1, c);
+        const double al0 = a.W(VW_AL, j, c), al1 = a.W(VW_AL, j + 1, c);
+        const double be0 = a.W(VW_BE, j, c), be1 = a.W(VW_BE, j + 1, c);
+        a.W(VW_TIN, j, c) = ti;
+        a.W(VW_D, j, c) = -(ti*ti)*(ga0 + ga1) + 0.5*dex*(al1 - al0 + ti*(be1 - be0))
+                          - dz/(iw*(dt*dt)*GC_C_D_P*rio)*(2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]));
+        a.W(VW_R, j, c) = -(wo + dt*a.t_wind[wi])*q + ti*(a.epe(j, c) - a.epe(j + 1, c))/dt
+                          + 0.5/dt*(a.tpe(j, c) + a.tpe(j + 1, c))*dex - q*wo*rie/rio;
+        if (j < n - 1) {
+            const i64 l2 = sidx(G, j + 2, c);
+            const double dz1 = G.zs[lo] - G.zs[l2], dex1 = G.exbg[lo] - G.exbg[l2];
+            const double ti1 = tin(j + 1, c), rio1 = a.rio(j + 1, c);
+            a.W(VW_C, j, c) = ti1*ga1*ti + 0.5*dex1*(al1 + be1*ti)
+                              - dz1/(iw*dt*GC_C_D_P)*0.5*a.wind_o[widx(G, j + 2, c)]/(G.vol[lo]*rio1);
+            a.W(VW_E, j, c) = ti*ga1*ti1 - 0.5*dex*(al1 + be1*ti1) + dz/(iw*dt*GC_C_D_P)*0.5*wo/(G.vol[lo]*rio);
+        }
+    }
+};
+
+/* Thomas algorithm of column_solvers.c:641-667: A x = r (bands c below,
+   d, e above) or, with transpose, A^T x = r (e below, c above); the
+   quotients go to VW_EP */
+struct VwSolve {
+    VwArgs a;
+    int transpose;
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= a.G.nsh) return;
+        const int c = (int)t, n = a.G.L - 1;
+        const int lo_b = transpose ? VW_E : VW_C, up_b = transpose ? VW_C : VW_E;
+        const int rhs = transpose ? VW_BX : VW_R, out = transpose ? VW_BR : VW_X;
+        double ep = n > 1 ? a.W(up_b, 0, c)/a.W(VW_D, 0, c) : 0.0, rp = a.W(rhs, 0, c)/a.W(VW_D, 0, c);
+        a.W(VW_EP, 0, c) = ep;
+        a.W(out, 0, c) = rp;
+        for (int j = 1; j < n; ++j) {
+            const double lo = a.W(lo_b, j - 1, c);
+            const double den = a.W(VW_D, j, c) - ep*lo;
+            rp = (a.W(rhs, j, c) - rp*lo)/den;
+            if (j < n - 1) ep = a.W(up_b, j, c)/den;
+            a.W(VW_EP, j, c) = ep;
+            a.W(out, j, c) = rp;
+        }
+        double x = rp;
+        for (int j = n - 2; j >= 0; --j) {
+            x = a.W(out, j, c) - a.W(VW_EP, j, c)*x;
+            a.W(out, j, c) = x;
+        }
+    }
+};
+
+/* the cotangent of the solution x at interface j from the new state's
+   rho, rt (through the fluxes sol and theta_i sol) and w */
+struct VwIfaceSolBar {
+    VwArgs a;
+    GAME_HD void operator()(i64 t) const
+    {
+        const DevGrid &G = a.G;
+        const int n = G.L - 1;
+        if (t >= (i64)n*G.nsh) return;
+        int j, c;
+        a.split(t, j, c);
+        const double dt = a.dt;
+        const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c), wi = widx(G, j + 1, c);
+        double rn0, rtn0, rn1, rtn1, bre0, brte0, bre1, brte1;
+        a.rnew(j, c, rn0, rtn0);
+        a.rnew(j + 1, c, rn1, rtn1);
+        a.bnew(j, c, rn0, rtn0, bre0, brte0);
+        a.bnew(j + 1, c, rn1, rtn1, bre1, brte1);
+        const double grt = brte0/G.vol[b] - brte1/G.vol[lo];
+        const double grho = bre0/G.vol[b] - bre1/G.vol[lo];
+        double bx = 2.0/(G.area[wi]*a.rio(j, c))*a.b_wind[wi];
+        bx += dt*a.W(VW_TIN, j, c)*grt + dt*grho;
+        a.W(VW_BX, j, c) = bx/a.damp(j, c);
+    }
+};
+
+/* the cotangent interface j contributes to theta_i, rho_i and w_o of
+   interface k = j + 1 through the bands of row j + 1 (c[j]) and row j
+   (e[j]); zero for the last interface */
+struct VwNext { double btin, brio, bwo; };
+
+GAME_HD inline VwNext vw_next(const VwArgs &a, int j, int c)
+{
+    const DevGrid &G = a.G;
+    VwNext r{0.0, 0.0, 0.0};
+    if (j >= G.L - 2) return r;
+    const double dt = a.dt, iw = a.iw;
+    const int k = j + 1;
+    const i64 b = sidx(G, j, c), lo = sidx(G, k, c), l2 = sidx(G, k + 1, c);
+    const double x = a.W(VW_X, j, c), rb = a.W(VW_BR, j, c);
+    const double bc = -a.W(VW_BR, k, c)*x, be_ = -rb*a.W(VW_X, k, c);
+    const double tin = a.W(VW_TIN, j, c), ga1 = a.W(VW_GA, k, c), be1 = a.W(VW_BE, k, c);
+    const double dex = G.exbg[b] - G.exbg[lo];
+    const double dz1 = G.zs[lo] - G.zs[l2];
+    const double rio1 = a.rio(k, c), wo1 = a.wind_o[widx(G, k + 1, c)];
+    const double k1 = dz1/(iw*dt*GC_C_D_P)*0.5;
+    r.btin = ga1*tin*bc + (tin*ga1 - 0.5*dex*be1)*be_;
+    r.brio = k1*wo1/(G.vol[lo]*rio1*rio1)*bc;
+    r.bwo = -k1/(G.vol[lo]*rio1)*bc;
+    return r;
+}
+
+/* the cotangents of theta_i and rho_i at interface j, and of w_o and the w
+   tendency there: the band cotangents -(rb x^T) and the right-hand side rb
+   of row j, the parts rows j and j - 1 give interface j, and the new state's
+   rho, rt and w */
+struct VwIfaceBar {
+    VwArgs a;
+    GAME_HD void operator()(i64 t) const
+    {
+        const DevGrid &G = a.G;
+        const int n = G.L - 1;
+        if (t >= (i64)n*G.nsh) return;
+        int j, c;
+        a.split(t, j, c);
+        const double dt = a.dt, iw = a.iw;
+        const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c), wi = widx(G, j + 1, c);
+        /* the new state: theta_i sol in the rt flux, w_new */
+        double rn0, rtn0, rn1, rtn1, bre0, brte0, bre1, brte1;
+        a.rnew(j, c, rn0, rtn0);
+        a.rnew(j + 1, c, rn1, rtn1);
+        a.bnew(j, c, rn0, rtn0, bre0, brte0);
+        a.bnew(j + 1, c, rn1, rtn1, bre1, brte1);
+        const double sol = a.sol(j, c), rio = a.rio(j, c), wo = a.wind_o[wi], bw = a.b_wind[wi];
+        const double grt = brte0/G.vol[b] - brte1/G.vol[lo];
+        const double din = 0.5*(rn0 + rn1);
+        const double wnew = (2.0*sol/G.area[wi] - din*wo)/rio;
+        double btin = dt*sol*grt;
+        double brio = -wnew/rio*bw;
+        double bwo = -din/rio*bw;
+        /* row j: r and d, then c[j] and e[j] */
+        const double x = a.W(VW_X, j, c), rb = a.W(VW_BR, j, c);
+        const double bd = -rb*x, br = rb;
+        const double tin = a.W(VW_TIN, j, c);
+        const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
+        const double q = dz/(iw*(dt*dt)*GC_C_D_P);
+        const double rie = 0.5*(a.re(j, c) + a.re(j + 1, c));
+        const double ga0 = a.W(VW_GA, j, c), ga1 = a.W(VW_GA, j + 1, c);
+        const double be0 = a.W(VW_BE, j, c), be1 = a.W(VW_BE, j + 1, c);
+        bwo += -q*br - q*rie/rio*br;
+        a.bt_wind[wi] += -dt*q*br;
+        btin += (a.epe(j, c) - a.epe(j + 1, c))/dt*br;
+        brio += q*wo*rie/(rio*rio)*br;
+        const double kk = 2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]);
+        btin += (-2.0*tin*(ga0 + ga1) + 0.5*dex*(be1 - be0))*bd;
+        brio += dz/(iw*(dt*dt)*GC_C_D_P*rio*rio)*kk*bd;
+        bwo += -dz/(iw*(dt*dt)*GC_C_D_P*rio)*dt*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo])*bd;
+        if (j < n - 1) {
+            const i64 l2 = sidx(G, j + 2, c);
+            const double bc = -a.W(VW_BR, j + 1, c)*x;
+            const double be_ = -rb*a.W(VW_X, j + 1, c);
+            const double dex1 = G.exbg[lo] - G.exbg[l2];
+            const double tin1 = a.W(VW_TIN, j + 1, c);
+            btin += (tin1*ga1 + 0.5*dex1*be1)*bc;
+            btin += ga1*tin1*be_;
+            const double k0 = dz/(iw*dt*GC_C_D_P)*0.5;
+            bwo += k0/(G.vol[lo]*rio)*be_;
+            brio -= k0*wo/(G.vol[lo]*rio*rio)*be_;
+        }
+        if (j > 0) {
+            const VwNext p = vw_next(a, j - 1, c);
+            btin += p.btin;
+            brio += p.brio;
+            bwo += p.bwo;
+        }
+        a.W(VW_BTIN, j, c) = btin;
+        a.W(VW_BRIO, j, c) = brio;
+        a.bo_wind[wi] += bwo;
+    }
+};
+
+/* the per-layer cotangents of layer j: alpha, beta, gamma, theta_e, pi_e
+   from the rows of interfaces j - 1 and j, re and rte, and through them the
+   old state, the new state as read and the tendency */
+struct VwLayerBar {
+    VwArgs a;
+    GAME_HD void operator()(i64 t) const
+    {
+        const DevGrid &G = a.G;
+        if (t >= G.s) return;
+        int j, c;
+        a.split(t, j, c);
+        const int n = G.L - 1, rk = a.rk;
+        const double dt = a.dt, iw = a.iw, pdw = 0.5;
+        const i64 b = sidx(G, j, c);
+        const double ro = a.rho_o[b], rto = a.rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
+        const double tr = a.t_rho[b], tt = a.t_rt[b];
+        const double al = a.W(VW_AL, j, c), be = a.W(VW_BE, j, c), ga = a.W(VW_GA, j, c);
+        double rn, rtn, bre, brte;
+        a.rnew(j, c, rn, rtn);
+        a.bnew(j, c, rn, rtn, bre, brte);
+        double bal = 0.0, bbe = 0.0, bga = vol*(rtn - rto)*a.b_ep[b], btpe = 0.0, bepe = 0.0;
+        /* row j (the layer above interface j) */
+        if (j < n) {
+            const double tin = a.W(VW_TIN, j, c), br = a.W(VW_BR, j, c), bd = -br*a.W(VW_X, j, c);
+            const double dex = exbg - G.exbg[sidx(G, j + 1, c)];
+            bepe += tin/dt*br;
+            btpe += 0.5/dt*dex*br;
+            bga += -(tin*tin)*bd;
+            bal -= 0.5*dex*bd;
+            bbe -= 0.5*dex*tin*bd;
+            /* rho_ie = mean of re: its cotangent at interface j */
+            const i64 lo = sidx(G, j + 1, c);
+            const double q = (G.zs[b] - G.zs[lo])/(iw*(dt*dt)*GC_C_D_P);
+            bre += 0.5*(-q*a.wind_o[widx(G, j + 1, c)]/a.rio(j, c)*br);
+        }
+        /* row j - 1 (the layer below interface j - 1), and its bands c[j-1], e[j-1] */
+        if (j > 0) {
+            const int p = j - 1;
+            const i64 bp = sidx(G, p, c);
+            const double tin = a.W(VW_TIN, p, c), br = a.W(VW_BR, p, c), xp = a.W(VW_X, p, c), bd = -br*xp;
+            const double dex = G.exbg[bp] - exbg;
+            bepe -= tin/dt*br;
+            btpe += 0.5/dt*dex*br;
+            bga += -(tin*tin)*bd;
+            bal += 0.5*dex*bd;
+            bbe += 0.5*dex*tin*bd;
+            if (j < n) {
+                const double bc = -a.W(VW_BR, j, c)*xp, be_ = -br*a.W(VW_X, j, c);
+                const double tin1 = a.W(VW_TIN, j, c);
+                const double dex1 = exbg - G.exbg[sidx(G, j + 1, c)];
+                bga += tin1*tin*bc + tin*tin1*be_;
+                bal += 0.5*dex1*bc - 0.5*dex*be_;
+                bbe += 0.5*dex1*tin*bc - 0.5*dex*tin1*be_;
             }
-            W(WS_RE, j, c) = ro + dt*tr;
-            W(WS_RTE, j, c) = rto + dt*tt;
-            W(WS_AL, j, c) = al;
-            W(WS_BE, j, c) = be;
-            W(WS_GA, j, c) = ga;
-            W(WS_TPE, j, c) = tp_o[b] + dt*vol*(al*tr + be*tt);
-            W(WS_EPE, j, c) = ep_o[b] + dt*vol*ga*tt;
+            const double q = (G.zs[bp] - G.zs[b])/(iw*(dt*dt)*GC_C_D_P);
+            bre += 0.5*(-q*a.wind_o[widx(G, j, c)]/a.rio(p, c)*br);
         }
-        const double *rn_ = rk == 0 ? rho_o : rho_n, *rtn_ = rk == 0 ? rt_o : rt_n;
-        for (int j = 0; j < n; ++j) {
-            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
-            W(WS_RIO, j, c) = 0.5*(rho_o[b] + rho_o[lo]);
-            W(WS_RIE, j, c) = 0.5*(W(WS_RE, j, c) + W(WS_RE, j + 1, c));
-            W(WS_TIN, j, c) = 0.5*(rtn_[b]/rn_[b] + rtn_[lo]/rn_[lo]);
+        double btr = 0.0, btt = 0.0, bro = 0.0, brto = 0.0, btpo = 0.0, bepo = 0.0;
+        double brn = 0.0, brtn = 0.0, bepn = 0.0;
+        /* ep_new's ep_o and rt_o terms */
+        bepo += a.b_ep[b];
+        brto += -vol*ga*a.b_ep[b];
+        /* theta_e = tp_o + dt vol (al tr + be tt) */
+        btpo += btpe;
+        bal += dt*vol*tr*btpe;
+        bbe += dt*vol*tt*btpe;
+        btr += dt*vol*al*btpe;
+        btt += dt*vol*be*btpe;
+        /* pi_e = ep_o + dt vol ga tt */
+        bepo += bepe;
+        bga += dt*vol*tt*bepe;
+        btt += dt*vol*ga*bepe;
+        /* re = rho_o + dt tr, rte = rt_o + dt tt */
+        bro += bre;
+        btr += dt*bre;
+        brto += brte;
+        btt += dt*brte;
+        /* rho_i = mean of rho_o (interfaces j-1 and j) */
+        if (j < n) bro += 0.5*a.W(VW_BRIO, j, c);
+        if (j > 0) bro += 0.5*a.W(VW_BRIO, j - 1, c);
+        /* theta_i = mean of rt_n/rho_n */
+        double btin_sum = 0.0;
+        if (j < n) btin_sum += 0.5*a.W(VW_BTIN, j, c);
+        if (j > 0) btin_sum += 0.5*a.W(VW_BTIN, j - 1, c);
+        const double rnr = rk == 0 ? ro : a.rho_n[b], rtnr = rk == 0 ? rto : a.rt_n[b];
+        brtn += btin_sum/rnr;
+        brn += -btin_sum*rtnr/(rnr*rnr);
+        /* alpha, beta, gamma */
+        if (rk == 0) {
+            /* al = -rto/ro^2/vol, be = 1/ro/vol, ga = R/(cv rto) (exbg + ep_o)/vol */
+            brto += -bal/(ro*ro)/vol;
+            bro += 2.0*rto/(ro*ro*ro)/vol*bal;
+            bro += -bbe/(ro*ro)/vol;
+            brto += -ga/rto*bga;
+            bepo += GC_R_D/(GC_C_D_V*rto)/vol*bga;
+        } else {
+            const double rnn = a.rho_n[b], rtnn = a.rt_n[b];
+            const double ho = (1.0 - pdw)/vol, hn = pdw/vol;
+            brto += -ho*bal/(ro*ro);
+            bro += ho*2.0*rto/(ro*ro*ro)*bal;
+            brtn += -hn*bal/(rnn*rnn);
+            brn += hn*2.0*rtnn/(rnn*rnn*rnn)*bal;
+            bro += -ho*bbe/(ro*ro);
+            brn += -hn*bbe/(rnn*rnn);
+            const double go = GC_R_D/(GC_C_D_V*rto)*(exbg + a.ep_o[b]);
+            const double gn = GC_R_D/(GC_C_D_V*rtnn)*(exbg + a.ep_n[b]);
+            brto += -ho*go/rto*bga;
+            bepo += ho*GC_R_D/(GC_C_D_V*rto)*bga;
+            brtn += -hn*gn/rtnn*bga;
+            bepn += hn*GC_R_D/(GC_C_D_V*rtnn)*bga;
         }
-        for (int j = 0; j < n; ++j) {
-            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
-            const i64 wi = widx(G, j + 1, c);
-            const double tin = W(WS_TIN, j, c), rio = W(WS_RIO, j, c), wo = wind_o[wi];
-            const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
-            const double q = dz/(iw*(dt*dt)*GC_C_D_P);
-            W(WS_D, j, c) = -(tin*tin)*(W(WS_GA, j, c) + W(WS_GA, j + 1, c))
-                            + 0.5*dex*(W(WS_AL, j + 1, c) - W(WS_AL, j, c) + tin*(W(WS_BE, j + 1, c) - W(WS_BE, j, c)))
-                            - dz/(iw*(dt*dt)*GC_C_D_P*rio)*(2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]));
-            W(WS_R, j, c) = -(wo + dt*t_wind[wi])*q + tin*(W(WS_EPE, j, c) - W(WS_EPE, j + 1, c))/dt
-                            + 0.5/dt*(W(WS_TPE, j, c) + W(WS_TPE, j + 1, c))*dex - q*wo*W(WS_RIE, j, c)/rio;
-            if (j < n - 1) {
-                const i64 l2 = sidx(G, j + 2, c);
-                const double dz1 = G.zs[lo] - G.zs[l2], dex1 = G.exbg[lo] - G.exbg[l2];
-                const double tin1 = W(WS_TIN, j + 1, c);
-                W(WS_C, j, c) = tin1*W(WS_GA, j + 1, c)*tin + 0.5*dex1*(W(WS_AL, j + 1, c) + W(WS_BE, j + 1, c)*tin)
-                                - dz1/(iw*dt*GC_C_D_P)*0.5*wind_o[widx(G, j + 2, c)]/(G.vol[lo]*W(WS_RIO, j + 1, c));
-                W(WS_E, j, c) = tin*W(WS_GA, j + 1, c)*tin1 - 0.5*dex*(W(WS_AL, j + 1, c) + W(WS_BE, j + 1, c)*tin1)
-                                + dz/(iw*dt*GC_C_D_P)*0.5*wo/(G.vol[lo]*rio);
-            }
-        }
-        /* Thomas, column_solvers.c:641-667 */
-        {
-            double ep = n > 1 ? W(WS_E, 0, c)/W(WS_D, 0, c) : 0.0, rp = W(WS_R, 0, c)/W(WS_D, 0, c);
-            W(WS_EP, 0, c) = ep;
-            W(WS_X, 0, c) = rp;
-            for (int j = 1; j < n; ++j) {
-                const double den = W(WS_D, j, c) - ep*W(WS_C, j - 1, c);
-                if (j < n - 1) ep = W(WS_E, j, c)/den;
-                rp = (W(WS_R, j, c) - rp*W(WS_C, j - 1, c))/den;
-                W(WS_EP, j, c) = ep;
-                W(WS_X, j, c) = rp;
-            }
-            double x = W(WS_X, n - 1, c);
-            for (int j = n - 2; j >= 0; --j) {
-                x = W(WS_X, j, c) - W(WS_EP, j, c)*x;
-                W(WS_X, j, c) = x;
-            }
-        }
-        /* new rho and rt (needed by the reverse of tp, ep and w) */
-        for (int j = 0; j < L; ++j) {
-            const i64 b = sidx(G, j, c);
-            const double su = j < n ? W(WS_X, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]) : 0.0;
-            const double sd = j > 0 ? W(WS_X, j - 1, c)/(1.0 + dt*G.damp[sidx(G, j - 1, c)]) : 0.0;
-            const double tu = j < n ? W(WS_TIN, j, c) : 0.0, td = j > 0 ? W(WS_TIN, j - 1, c) : 0.0;
-            W(WS_RNEW, j, c) = W(WS_RE, j, c) + dt*(-sd + su)/G.vol[b];
-            W(WS_RTNEW, j, c) = W(WS_RTE, j, c) + dt*(-td*sd + tu*su)/G.vol[b];
-        }
-        /* ---- reverse ---- */
-        for (int j = 0; j < L; ++j) {
-            W(WS_BAL, j, c) = 0.0; W(WS_BBE, j, c) = 0.0; W(WS_BGA, j, c) = 0.0;
-            W(WS_BTPE, j, c) = 0.0; W(WS_BEPE, j, c) = 0.0;
-            W(WS_BRIO, j, c) = 0.0; W(WS_BRIE, j, c) = 0.0; W(WS_BTIN, j, c) = 0.0;
-        }
-        /* ep_new = ep_o + vol ga (rt_new - rt_o); tp_new = rt_new/rho_new - thbg */
-        for (int j = 0; j < L; ++j) {
-            const i64 b = sidx(G, j, c);
-            const double vol = G.vol[b], ga = W(WS_GA, j, c), rnew = W(WS_RNEW, j, c), rtnew = W(WS_RTNEW, j, c);
-            const double bep = b_ep[b], btp = b_tp[b];
-            W(WS_BGA, j, c) += vol*(rtnew - rt_o[b])*bep;
-            double brt = b_rt[b] + vol*ga*bep + btp/rnew;
-            double brho = b_rho[b] - btp*rtnew/(rnew*rnew);
-            W(WS_BRTE, j, c) = brt;     /* cotangent of rt_new for now */
-            W(WS_BRE, j, c) = brho;     /* cotangent of rho_new for now */
-        }
-        /* w_new[j] = (2 sol[j]/A - din wo)/rho_i, din = mean of rho_new */
-        for (int j = 0; j < n; ++j) {
-            const i64 wi = widx(G, j + 1, c);
-            const double rio = W(WS_RIO, j, c), wo = wind_o[wi], bw = b_wind[wi];
-            const double sol = W(WS_X, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]);
-            const double din = 0.5*(W(WS_RNEW, j, c) + W(WS_RNEW, j + 1, c));
-            const double wnew = (2.0*sol/G.area[wi] - din*wo)/rio;
-            W(WS_BX, j, c) = 2.0/(G.area[wi]*rio)*bw;       /* cotangent of sol for now */
-            bo_wind[wi] += -din/rio*bw;
-            const double bdin = -wo/rio*bw;
-            W(WS_BRIO, j, c) += -wnew/rio*bw;
-            W(WS_BRE, j, c) += 0.5*bdin;
-            W(WS_BRE, j + 1, c) += 0.5*bdin;
-        }
-        /* rt_new and rho_new from sol */
-        for (int j = 0; j < n; ++j) {
-            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
-            const double sol = W(WS_X, j, c)/(1.0 + dt*G.damp[b]);
-            const double grt = W(WS_BRTE, j, c)/G.vol[b] - W(WS_BRTE, j + 1, c)/G.vol[lo];
-            const double grho = W(WS_BRE, j, c)/G.vol[b] - W(WS_BRE, j + 1, c)/G.vol[lo];
-            W(WS_BTIN, j, c) += dt*sol*grt;
-            W(WS_BX, j, c) += dt*W(WS_TIN, j, c)*grt + dt*grho;
-        }
-        /* damping, then the transposed solve A^T rb = xb (A^T: lower e, upper c) */
-        for (int j = 0; j < n; ++j) W(WS_BX, j, c) = W(WS_BX, j, c)/(1.0 + dt*G.damp[sidx(G, j, c)]);
-        {
-            /* forward sweep of A^T into WS_BR (quotients reuse WS_EP as scratch) */
-            double ep = n > 1 ? W(WS_C, 0, c)/W(WS_D, 0, c) : 0.0, rp = W(WS_BX, 0, c)/W(WS_D, 0, c);
-            W(WS_EP, 0, c) = ep;
-            W(WS_BR, 0, c) = rp;
-            for (int j = 1; j < n; ++j) {
-                const double den = W(WS_D, j, c) - ep*W(WS_E, j - 1, c);
-                if (j < n - 1) ep = W(WS_C, j, c)/den;
-                rp = (W(WS_BX, j, c) - rp*W(WS_E, j - 1, c))/den;
-                W(WS_EP, j, c) = ep;
-                W(WS_BR, j, c) = rp;
-            }
-            double x = W(WS_BR, n - 1, c);
-            for (int j = n - 2; j >= 0; --j) {
-                x = W(WS_BR, j, c) - W(WS_EP, j, c)*x;
-                W(WS_BR, j, c) = x;
-            }
-        }
-        /* the new state's rho and rt cotangents now feed re and rte */
-        /* coefficients: bands -(rb x^T), right-hand side rb */
-        for (int j = 0; j < n; ++j) {
-            const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c);
-            const i64 wi = widx(G, j + 1, c);
-            const double x = W(WS_X, j, c), rb = W(WS_BR, j, c);
-            const double bd = -rb*x, br = rb;
-            const double tin = W(WS_TIN, j, c), rio = W(WS_RIO, j, c), wo = wind_o[wi];
-            const double dz = G.zs[b] - G.zs[lo], dex = G.exbg[b] - G.exbg[lo];
-            const double q = dz/(iw*(dt*dt)*GC_C_D_P);
-            const double ga0 = W(WS_GA, j, c), ga1 = W(WS_GA, j + 1, c);
-            const double be0 = W(WS_BE, j, c), be1 = W(WS_BE, j + 1, c);
-            /* r = -(wo + dt tw) q + tin (pe_j - pe_j+1)/dt + 0.5/dt (te_j + te_j+1) dex - q wo rie/rio */
-            double bwo = -q*br - q*W(WS_RIE, j, c)/rio*br;
-            bt_wind[wi] += -dt*q*br;
-            double btin = (W(WS_EPE, j, c) - W(WS_EPE, j + 1, c))/dt*br;
-            W(WS_BEPE, j, c) += tin/dt*br;
-            W(WS_BEPE, j + 1, c) -= tin/dt*br;
-            W(WS_BTPE, j, c) += 0.5/dt*dex*br;
-            W(WS_BTPE, j + 1, c) += 0.5/dt*dex*br;
-            W(WS_BRIE, j, c) += -q*wo/rio*br;
-            double brio = q*wo*W(WS_RIE, j, c)/(rio*rio)*br;
-            /* d = -tin^2 (ga_j + ga_j+1) + 0.5 dex (al_j+1 - al_j + tin (be_j+1 - be_j))
-                   - dz/(iw dt^2 cp rio) (2/A + dt wo 0.5 (-1/vol_j + 1/vol_j+1)) */
-            const double kk = 2.0/G.area[wi] + dt*wo*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo]);
-            btin += (-2.0*tin*(ga0 + ga1) + 0.5*dex*(be1 - be0))*bd;
-            W(WS_BGA, j, c) += -(tin*tin)*bd;
-            W(WS_BGA, j + 1, c) += -(tin*tin)*bd;
-            W(WS_BAL, j + 1, c) += 0.5*dex*bd;
-            W(WS_BAL, j, c) -= 0.5*dex*bd;
-            W(WS_BBE, j + 1, c) += 0.5*dex*tin*bd;
-            W(WS_BBE, j, c) -= 0.5*dex*tin*bd;
-            brio += dz/(iw*(dt*dt)*GC_C_D_P*rio*rio)*kk*bd;
-            bwo += -dz/(iw*(dt*dt)*GC_C_D_P*rio)*dt*0.5*(-1.0/G.vol[b] + 1.0/G.vol[lo])*bd;
-            if (j < n - 1) {
-                /* row j+1's lower band c[j] = tin_j+1 ga_j+1 tin_j + 0.5 dex1 (al_j+1 + be_j+1 tin_j)
-                       - dz1/(iw dt cp) 0.5 wo_j+1/(vol_j+1 rio_j+1) */
-                const i64 l2 = sidx(G, j + 2, c);
-                const double bc = -W(WS_BR, j + 1, c)*x;
-                const double be_ = -rb*W(WS_X, j + 1, c);
-                const double dz1 = G.zs[lo] - G.zs[l2], dex1 = G.exbg[lo] - G.exbg[l2];
-                const double tin1 = W(WS_TIN, j + 1, c), rio1 = W(WS_RIO, j + 1, c);
-                const double wo1 = wind_o[widx(G, j + 2, c)];
-                W(WS_BTIN, j + 1, c) += ga1*tin*bc;
-                W(WS_BGA, j + 1, c) += tin1*tin*bc;
-                btin += (tin1*ga1 + 0.5*dex1*be1)*bc;
-                W(WS_BAL, j + 1, c) += 0.5*dex1*bc;
-                W(WS_BBE, j + 1, c) += 0.5*dex1*tin*bc;
-                const double k1 = dz1/(iw*dt*GC_C_D_P)*0.5;
-                bo_wind[widx(G, j + 2, c)] += -k1/(G.vol[lo]*rio1)*bc;
-                W(WS_BRIO, j + 1, c) += k1*wo1/(G.vol[lo]*rio1*rio1)*bc;
-                /* upper band e[j] = tin_j ga_j+1 tin_j+1 - 0.5 dex (al_j+1 + be_j+1 tin_j+1)
-                       + dz/(iw dt cp) 0.5 wo_j/(vol_j+1 rio_j) */
-                btin += ga1*tin1*be_;
-                W(WS_BGA, j + 1, c) += tin*tin1*be_;
-                W(WS_BTIN, j + 1, c) += (tin*ga1 - 0.5*dex*be1)*be_;
-                W(WS_BAL, j + 1, c) -= 0.5*dex*be_;
-                W(WS_BBE, j + 1, c) -= 0.5*dex*tin1*be_;
-                const double k0 = dz/(iw*dt*GC_C_D_P)*0.5;
-                bwo += k0/(G.vol[lo]*rio)*be_;
-                brio -= k0*wo/(G.vol[lo]*rio*rio)*be_;
-            }
-            W(WS_BTIN, j, c) += btin;
-            W(WS_BRIO, j, c) += brio;
-            bo_wind[wi] += bwo;
-        }
-        /* interface means: rie, rio, tin */
-        for (int j = 0; j < n; ++j) {
-            W(WS_BRE, j, c) += 0.5*W(WS_BRIE, j, c);
-            W(WS_BRE, j + 1, c) += 0.5*W(WS_BRIE, j, c);
-        }
-        /* per layer: theta_e, pi_e, re, rte, alpha, beta, gamma */
-        for (int j = 0; j < L; ++j) {
-            const i64 b = sidx(G, j, c);
-            const double ro = rho_o[b], rto = rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
-            const double tr = t_rho[b], tt = t_rt[b];
-            const double al = W(WS_AL, j, c), be = W(WS_BE, j, c), ga = W(WS_GA, j, c);
-            double bal = W(WS_BAL, j, c), bbe = W(WS_BBE, j, c), bga = W(WS_BGA, j, c);
-            const double btpe = W(WS_BTPE, j, c), bepe = W(WS_BEPE, j, c);
-            double btr = 0.0, btt = 0.0, bro = 0.0, brto = 0.0, btpo = 0.0, bepo = 0.0;
-            double brn = 0.0, brtn = 0.0, bepn = 0.0;
-            /* ep_new's ep_o and rt_o terms */
-            bepo += b_ep[b];
-            brto += -vol*ga*b_ep[b];
-            /* theta_e = tp_o + dt vol (al tr + be tt) */
-            btpo += btpe;
-            bal += dt*vol*tr*btpe;
-            bbe += dt*vol*tt*btpe;
-            btr += dt*vol*al*btpe;
-            btt += dt*vol*be*btpe;
-            /* pi_e = ep_o + dt vol ga tt */
-            bepo += bepe;
-            bga += dt*vol*tt*bepe;
-            btt += dt*vol*ga*bepe;
-            /* re = rho_o + dt tr, rte = rt_o + dt tt */
-            const double bre = W(WS_BRE, j, c), brte = W(WS_BRTE, j, c);
-            bro += bre;
-            btr += dt*bre;
-            brto += brte;
-            btt += dt*brte;
-            /* rho_i = mean of rho_o (interfaces j-1 and j) */
-            if (j < n) bro += 0.5*W(WS_BRIO, j, c);
-            if (j > 0) bro += 0.5*W(WS_BRIO, j - 1, c);
-            /* theta_i = mean of rt_n/rho_n */
-            double btin_sum = 0.0;
-            if (j < n) btin_sum += 0.5*W(WS_BTIN, j, c);
-            if (j > 0) btin_sum += 0.5*W(WS_BTIN, j - 1, c);
-            const double rn = rk == 0 ? ro : rho_n[b], rtn = rk == 0 ? rto : rt_n[b];
-            brtn += btin_sum/rn;
-            brn += -btin_sum*rtn/(rn*rn);
-            /* alpha, beta, gamma */
-            if (rk == 0) {
-                /* al = -rto/ro^2/vol, be = 1/ro/vol, ga = R/(cv rto) (exbg + ep_o)/vol */
-                brto += -bal/(ro*ro)/vol;
-                bro += 2.0*rto/(ro*ro*ro)/vol*bal;
-                bro += -bbe/(ro*ro)/vol;
-                brto += -ga/rto*bga;
-                bepo += GC_R_D/(GC_C_D_V*rto)/vol*bga;
-            } else {
-                const double rnn = rho_n[b], rtnn = rt_n[b];
-                const double ho = (1.0 - pdw)/vol, hn = pdw/vol;
-                brto += -ho*bal/(ro*ro);
-                bro += ho*2.0*rto/(ro*ro*ro)*bal;
-                brtn += -hn*bal/(rnn*rnn);
-                brn += hn*2.0*rtnn/(rnn*rnn*rnn)*bal;
-                bro += -ho*bbe/(ro*ro);
-                brn += -hn*bbe/(rnn*rnn);
-                const double go = GC_R_D/(GC_C_D_V*rto)*(exbg + ep_o[b]);
-                const double gn = GC_R_D/(GC_C_D_V*rtnn)*(exbg + ep_n[b]);
-                brto += -ho*go/rto*bga;
-                bepo += ho*GC_R_D/(GC_C_D_V*rto)*bga;
-                brtn += -hn*gn/rtnn*bga;
-                bepn += hn*GC_R_D/(GC_C_D_V*rtnn)*bga;
-            }
-            bt_rho[b] += btr;
-            bt_rt[b] += btt;
-            bo_tp[b] += btpo;
-            bo_ep[b] += bepo;
-            if (rk == 0) {
-                bo_rho[b] += bro + brn;
-                bo_rt[b] += brto + brtn;
-            } else {
-                bo_rho[b] += bro;
-                bo_rt[b] += brto;
-                bn_rho[b] += brn;
-                bn_rt[b] += brtn;
-                bn_ep[b] += bepn;
-            }
+        a.bt_rho[b] += btr;
+        a.bt_rt[b] += btt;
+        a.bo_tp[b] += btpo;
+        a.bo_ep[b] += bepo;
+        if (rk == 0) {
+            a.bo_rho[b] += bro + brn;
+            a.bo_rt[b] += brto + brtn;
+        } else {
+            a.bo_rho[b] += bro;
+            a.bo_rt[b] += brto;
+            a.bn_rho[b] += brn;
+            a.bn_rt[b] += brtn;
+            a.bn_ep[b] += bepn;
         }
     }
 };
