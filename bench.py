@@ -299,7 +299,7 @@ def adjoint_ratios(grid, reps=20):
     return out
 
 
-def window_ratio(lloyd, reps=3):
+def window_ratio(lloyd, reps=3, all_kernels=False):
     """The assimilation window (res 6 x 26, dry, Gaussian mountain, 24 steps at
     120 s, checkpoints every 4 steps, J = 1/2 sum (rho_N - rho_target)^2 with the
     target from a second seeded state): wall time of the window's forward run
@@ -344,17 +344,40 @@ def window_ratio(lloyd, reps=3):
         for _ in range(reps):
             grad()
         t_g = (time.perf_counter() - t0) / reps
+        phases = np.zeros(4)
+        dm._check(lib.game_gpu_window_last_timing(phases.ctypes.data_as(C.POINTER(C.c_double))))
+        # the whole window taped in one segment (no recomputation): 24 taped steps fit in HBM here
+        grad_n = lambda: dm._check(lib.game_gpu_window_gradient(n, dt, n, *map(p, xa), p(target), C.byref(cost),
+                                                                *map(p, grads)))
+        grad_n()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grad_n()
+        t_gn = (time.perf_counter() - t0) / reps
+        phases_n = np.zeros(4)
+        dm._check(lib.game_gpu_window_last_timing(phases_n.ctypes.data_as(C.POINTER(C.c_double))))
+        mem_n = dm.bytes_allocated()
         # kernel times of one gradient
         dm.reset_timing()
         dm.enable_timing(True)
         grad()
         dm.enable_timing(False)
         st = dm.kernel_stats()
-        top = sorted(st.items(), key=lambda kv: -kv[1][1])[:8]
+        top = sorted(st.items(), key=lambda kv: -kv[1][1])[:None if all_kernels else 8]
         out = {"window": f"res 6 x 26 dry, Gaussian mountain, {n} steps at {dt:g} s, checkpoints every {k} steps",
                "forward_ms": round(t_f * 1e3, 3), "gradient_ms": round(t_g * 1e3, 3),
                "gradient_over_forward": round(t_g / t_f, 3),
                "reverse_over_forward": round((t_g - t_f) / t_f, 3), "cost": cost.value,
+               "gradient_phases_ms": {"forward_sweep": round(phases[0], 3), "recompute": round(phases[1], 3),
+                                      "reverse": round(phases[2], 3), "finish": round(phases[3], 3)},
+               "forward_step_ms": round(t_f * 1e3 / n, 4), "adjoint_step_ms": round(phases[2] / n, 4),
+               "adjoint_step_over_forward_step": round(phases[2] / (t_f * 1e3), 3),
+               "one_segment": {"checkpoint_every": n, "gradient_ms": round(t_gn * 1e3, 3),
+                               "reverse_over_forward": round((t_gn - t_f) / t_f, 3),
+                               "gradient_phases_ms": {"forward_sweep": round(phases_n[0], 3),
+                                                      "reverse": round(phases_n[2], 3)},
+                               "device_memory_GiB": round(mem_n / 2**30, 2)},
                "top_kernels_ms_per_gradient": {name: round(ms, 3) for name, (_, ms) in top}}
     finally:
         dm.close()
@@ -395,7 +418,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.window_only:
-        print(json.dumps(window_ratio(args.lloyd)), flush=True)
+        print(json.dumps(window_ratio(args.lloyd, all_kernels=True)), flush=True)
         return
 
     from game_archived_amd.grid import Grid
@@ -516,16 +539,20 @@ def main():
             + ", ".join(f"{k} {v['ratio']:.2f}" for k, v in ops.items()))
         win = window_ratio(args.lloyd)
         log(f"window: forward {win['forward_ms']:.2f} ms, gradient {win['gradient_ms']:.2f} ms, "
-            f"reverse/forward {win['reverse_over_forward']:.2f}")
+            f"(gradient - forward)/forward {win['reverse_over_forward']:.2f}, adjoint step / forward step "
+            f"{win['adjoint_step_over_forward_step']:.2f}")
         out["adjoint_forward_ratio"] = {
-            "full_step": win["reverse_over_forward"],
+            "full_step": win["adjoint_step_over_forward_step"],
+            "window_gradient_extra_over_forward": win["reverse_over_forward"],
             "window": win,
             "operators": ops,
             "max_operator": max(v["ratio"] for v in ops.values()),
-            "note": "full_step: wall time of the reverse sweep of the window (taped recomputation of every "
-                    "checkpointed segment plus the hand-written adjoint of every step) over the wall time of the "
-                    "window's forward run; gradient_over_forward adds the checkpointed forward sweep. operators: "
-                    "each operator adjoint kernel against its forward kernel(s) at the res 7 workload"}
+            "note": "full_step: device time of the reverse of one step from its tape (every hand-written adjoint "
+                    "kernel of the step) over the time of one forward step, both from the res 6 x 26 assimilation "
+                    "window; window_gradient_extra_over_forward: (gradient wall time - forward wall time) / forward "
+                    "wall time of the whole window, i.e. with the checkpointed sweep, the taped recomputation of "
+                    "every segment but the last and the reverse; operators: each operator adjoint kernel against "
+                    "its forward kernel(s) at the res 7 workload"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, state0, args.dt, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -227,5 +227,6 @@ def gpu_lib():
     lib.game_gpu_window_forward.argtypes = [C.c_int, C.c_double] + [VP] * 7
     lib.game_gpu_window_vjp.argtypes = [C.c_int, C.c_double, C.c_int] + [VP] * 10
     lib.game_gpu_window_gradient.argtypes = [C.c_int, C.c_double, C.c_int] + [VP] * 4 + [DP] + [VP] * 3
+    lib.game_gpu_window_last_timing.argtypes = [DP]
     lib.game_gpu_op_thomas_adjoint.argtypes = [C.c_int, C.c_int] + [VP] * 10
     return lib

@@ -14,7 +14,7 @@ struct AdjTablesHost {
     std::vector<int> rhp_e, trp_e, trp_m;
     std::vector<double> rhp_w, trp_w;
     /* reverse table of the rhombus PV read by the TRSK sums, padded to pvr_max */
-    std::vector<int> pvr_e, pvr_f;
+    std::vector<int> pvr_e, pvr_f, pvr_n;
     std::vector<double> pvr_w;
 };
 
@@ -135,9 +135,11 @@ inline int build_adj_tables_host(int nsh, int nvh, int nt, int n_pent, const int
     T.pvr_e.assign((size_t)pvr_max*NVH, 0);
     T.pvr_f.assign((size_t)pvr_max*NVH, 0);
     T.pvr_w.assign((size_t)pvr_max*NVH, 0.0);
+    T.pvr_n.assign(NVH, 0);
     for (size_t m = 0; m < NVH; ++m) {
         const size_t n = pe[m].size();
         if ((int)n > pvr_max) { snprintf(err, nerr, "rhombus %zu read by %zu TRSK terms", m, n); return 2; }
+        T.pvr_n[m] = (int)n;
         for (size_t k = 0; k < n; ++k) {
             T.pvr_e[k*NVH + m] = pe[m][k];
             T.pvr_f[k*NVH + m] = pf[m][k];

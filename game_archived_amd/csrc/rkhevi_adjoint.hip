@@ -37,45 +37,68 @@ __device__ __forceinline__ void adj_chunk(int n, int &l0, int &l1)
 /* ------------------------------------------------------------------ */
 /* grad^T: fbar(c, l) from gbar at the cell's edges and w points        */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ double grad_w_cot(const DevGrid &G, const AdjTables &A, const double *gb, int h, int lev,
-                                             int ne)
-{
-    /* cotangent of the vertical covariant gradient gw(h, lev): its own w
-       output plus the terrain corrections of the cell's edges at layers lev
-       (ipw slot 6) and lev - 1 (slot 7) (gradient_operators.c:14-33, averaging.c:14-39) */
-    const int nsh = G.nsh, vpl = G.vpl, L = G.L, no = G.n_oro;
-    double r = gb[h + (i64)lev*vpl];
-    if (!G.flat) {
-        if (lev >= L - no)
-            for (int j = 0; j < ne; ++j) {
-                i64 vi = nsh + (i64)lev*vpl + G.adj[6*h + j];
-                r += -G.slope[vi]*0.5*G.ipw[8*((i64)lev*nsh + h) + 6]*gb[vi];
-            }
-        if (lev - 1 >= L - no)
-            for (int j = 0; j < ne; ++j) {
-                i64 vi = nsh + (i64)(lev - 1)*vpl + G.adj[6*h + j];
-                r += -G.slope[vi]*0.5*G.ipw[8*((i64)(lev - 1)*nsh + h) + 7]*gb[vi];
-            }
-    }
-    return r;
-}
-
 __global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const double *gb, double *fb)
 {
-    i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
-    if (t >= G.s) return;
-    const int nsh = G.nsh, vpl = G.vpl, L = G.L;
-    const int l = (int)(t/nsh), h = (int)(t - (i64)l*nsh);
+    /* one thread per (cell, chunk of layers): layer l's edge cotangents feed
+       the horizontal term of layer l and the terrain terms of the w points
+       l (ipw slot 6) and l + 1 (slot 7); each w point's cotangent is formed
+       once and carried to the next layer */
+    const int h = blockIdx.x*ATPB + threadIdx.x;
+    if (h >= G.nsh) return;
+    const int nsh = G.nsh, vpl = G.vpl, L = G.L, top = G.flat ? L : L - G.n_oro;
     const int ne = h < GAME_N_PENT ? 5 : 6;
-    double r = 0.0;
-    for (int j = 0; j < ne; ++j) {
-        i64 vi = nsh + (i64)l*vpl + G.adj[6*h + j];
-        /* g_e = (f_to - f_from)/d_e: the from-cell (sign +1) gets -1/d_e */
-        r += -(double)G.sgn[6*h + j]*gb[vi]/G.nd[vi];
+    int ej[6];
+    double sg[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int jj = j < ne ? j : 0;
+        ej[j] = G.adj[6*h + jj];
+        sg[j] = j < ne ? (double)G.sgn[6*h + jj] : 0.0;
     }
-    if (l + 1 <= L - 1) r += grad_w_cot(G, A, gb, h, l + 1, ne)/G.nd[h + (i64)(l + 1)*vpl];
-    if (l >= 1) r -= grad_w_cot(G, A, gb, h, l, ne)/G.nd[h + (i64)l*vpl];
-    fb[t] = r;
+    int l0, l1;
+    adj_chunk(L, l0, l1);
+    /* layer l: the horizontal sum and the two terrain sums */
+    auto layer = [&](int l, double &hs, double &s6, double &s7) {
+        double g[6], sl[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const i64 vi = nsh + (i64)l*vpl + ej[j];
+            g[j] = gb[vi];
+            sl[j] = (l >= top) ? G.slope[vi] : 0.0;
+            /* g_e = (f_to - f_from)/d_e: the from-cell (sign +1) gets -1/d_e */
+            if (j == 0) hs = 0.0;
+            hs += -sg[j]*g[j]/G.nd[vi];
+        }
+        s6 = 0.0; s7 = 0.0;
+        if (l >= top) {
+            const double w6 = G.ipw[8*((i64)l*nsh + h) + 6], w7 = G.ipw[8*((i64)l*nsh + h) + 7];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                if (j < ne) {
+                    s6 += -sl[j]*0.5*w6*g[j];
+                    s7 += -sl[j]*0.5*w7*g[j];
+                }
+            }
+        }
+    };
+    /* the cotangent of the vertical gradient at w point lev over its normal distance */
+    auto wcot = [&](int lev, double s6, double s7_above) {
+        double r = gb[h + (i64)lev*vpl] + s6 + s7_above;
+        return r/G.nd[h + (i64)lev*vpl];
+    };
+    double hs, s6, s7, s7p = 0.0, unused_h, unused_s6;
+    if (l0 >= 1) layer(l0 - 1, unused_h, unused_s6, s7p);
+    layer(l0, hs, s6, s7);
+    double wc = l0 >= 1 ? wcot(l0, s6, s7p) : 0.0;
+    for (int l = l0; l < l1; ++l) {
+        double hn = 0.0, s6n = 0.0, s7n = 0.0, wn = 0.0;
+        if (l + 1 <= L - 1) {
+            layer(l + 1, hn, s6n, s7n);
+            wn = wcot(l + 1, s6n, s7);
+        }
+        fb[(i64)l*nsh + h] = hs + wn - wc;
+        hs = hn; s7 = s7n; wc = wn;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -432,7 +455,7 @@ __global__ void __launch_bounds__(ATPB) k_thomas_adj(int n, int ncol, const doub
 namespace adj {
 
 void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st)
-{ LAUNCH("k_grad_adj", k_grad_adj, AGRID(G.s), dim3(ATPB), st, G, A, gb, fb); }
+{ LAUNCH("k_grad_adj", k_grad_adj, AGRID2(G.nsh), dim3(ATPB), st, G, A, gb, fb); }
 
 void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st)
 { LAUNCH("k_divv", k_divv, AGRID(G.s), dim3(ATPB), st, G, in, out); }

@@ -486,6 +486,11 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape
         if (rk == 0) hor::pgrad(G, snr, W, first, M->cfg.impl_thermo_weight, st);
         else hor::pgc_v(G, snr, W, st);
         if (rk == 1 || first) hor::advection(G, snr.wind, snr.rho + (i64)G.n_cond*G.s, W, st);
+        if constexpr (std::is_same<WK, DevWork>::value)
+            if (tape && rk == 1) {
+                HIP_OK(hipMemcpyAsync(tape->flux, W.flux, (size_t)M->d.v*sizeof(double), hipMemcpyDeviceToDevice, st));
+                HIP_OK(hipMemcpyAsync(tape->pv, W.pv, (size_t)M->d.curl*sizeof(double), hipMemcpyDeviceToDevice, st));
+            }
         hor::vector_tend(G, W, td_wind, so.wind, sn.wind, rk == 1 ? 0.5 : 1.0, M->cfg.impl_thermo_weight, dt, st);
         hor::scalar_tend(G, sns, td, W, rk, st);
         if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
@@ -877,7 +882,7 @@ int rt::build_adjoint_tables(void)
         || (rc = upload_ints(&A.tr_off, T.tr_off)) || (rc = upload_ints(&A.tr_ent, T.tr_ent)))
         return rc;
     PvReverse &R = M->PR;
-    if ((rc = upload_ints(&R.e, T.pvr_e)) || (rc = upload_ints(&R.f, T.pvr_f))
+    if ((rc = upload_ints(&R.e, T.pvr_e)) || (rc = upload_ints(&R.f, T.pvr_f)) || (rc = upload_ints(&R.n, T.pvr_n))
         || (rc = upload(&R.w, T.pvr_w.data(), T.pvr_w.size())))
         return rc;
     HIP_OK(hipStreamSynchronize(M->stream));

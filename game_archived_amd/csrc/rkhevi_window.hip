@@ -119,9 +119,37 @@ struct Window {
     Cot bS, bSn, bS1;
     double *bt0_rho, *bt0_rt, *bt0_wind, *bt1_rho, *bt1_rt, *bt1_wind;
     double *bP, *bPold, *bA, *bA1, *bAprev, *bU;
-    double *Fb, *Fthb, *cvb, *vsqb, *tmpS, *tmpV1, *tmpV2, *fluxb, *pvb, *rvb, *dvb, *zb, *ws, *ones;
+    double *Fb, *Fthb, *cvb, *vsqb, *tmpS, *tmpV1, *tmpV2, *fluxb, *pvb, *rvb, *dvb, *zb, *ws, *ones, *vcb;
     DevWork AW;
+    /* phase clock of the last gradient: events and the phase each closes */
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_phase;
+    int n_ev = 0;
+    double phase_ms[4] = {0, 0, 0, 0};
+    void mark(int phase, hipStream_t st)
+    {
+        if (n_ev == (int)ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            ev.push_back(e);
+            ev_phase.push_back(0);
+        }
+        ev_phase[n_ev] = phase;
+        (void)hipEventRecord(ev[n_ev++], st);
+    }
+    /* after the stream is synchronised: the time between consecutive marks,
+       summed per phase */
+    void read_clock()
+    {
+        for (double &v : phase_ms) v = 0.0;
+        for (int i = 1; i < n_ev; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[i - 1], ev[i]) == hipSuccess) phase_ms[ev_phase[i]] += ms;
+        }
+        n_ev = 0;
+    }
 };
+enum { PH_FORWARD, PH_RECOMPUTE, PH_REVERSE, PH_FINISH };
 
 Window *Wn = nullptr;
 Window *Wbuild = nullptr;   /* the window whose buffers are being allocated */
@@ -139,6 +167,7 @@ void window_free(Window *w)
     if (!w) return;
     if (M && w->owner == M->generation)
         for (void *p : w->mem) dfree(p);
+    for (hipEvent_t e : w->ev) (void)hipEventDestroy(e);
     delete w;
 }
 
@@ -203,7 +232,8 @@ int window_alloc(Window &w, int k, int n_ckpt)
         StepTape &t = w.tape[j];
         if ((rc = alloc_state(t.s)) || (rc = alloc_state(t.s1)) || (rc = zeros(&t.td0_rho, S))
             || (rc = zeros(&t.td0_rt, S)) || (rc = zeros(&t.td0_wind, V)) || (rc = zeros(&t.td1_rho, S))
-            || (rc = zeros(&t.td1_rt, S)) || (rc = zeros(&t.td1_wind, V)))
+            || (rc = zeros(&t.td1_rt, S)) || (rc = zeros(&t.td1_wind, V)) || (rc = zeros(&t.flux, V))
+            || (rc = zeros(&t.pv, CURL)))
             return rc;
     }
     if ((rc = alloc_state(w.seg_end)) || (rc = alloc_cot(w.bS)) || (rc = alloc_cot(w.bSn)) || (rc = alloc_cot(w.bS1)))
@@ -217,7 +247,9 @@ int window_alloc(Window &w, int k, int n_ckpt)
     for (double **p : ev) if ((rc = zeros(p, L*NVH))) return rc;
     double **cv[] = {&w.pvb, &w.rvb, &w.dvb};
     for (double **p : cv) if ((rc = zeros(p, CURL))) return rc;
-    if ((rc = zeros(&w.zb, L*NT)) || (rc = zeros(&w.ws, (size_t)sadj::VW_N*L*NSH))) return rc;
+    if ((rc = zeros(&w.zb, L*NT)) || (rc = zeros(&w.ws, (size_t)sadj::VW_N*L*NSH))
+        || (rc = zeros(&w.vcb, 2*(size_t)M->G.n_oro*NSH + 1)))
+        return rc;
     DevWork &A = w.AW;
     if ((rc = zeros(&A.flux, V)) || (rc = zeros(&A.rvt, L*NT)) || (rc = zeros(&A.pv, CURL)) || (rc = zeros(&A.pvt, V))
         || (rc = zeros(&A.vsq, S)) || (rc = zeros(&A.vsq_grad, V)))
@@ -249,6 +281,36 @@ int zero(double *p, size_t n)
     HIP_OK(hipMemsetAsync(p, 0, n*sizeof(double), M->stream));
     return 0;
 }
+
+/* zeroes up to ZERO_MAX arrays in one launch */
+#define ZERO_MAX 20
+struct ZeroMany {
+    double *p[ZERO_MAX];
+    i64 n[ZERO_MAX];
+    int cnt;
+    GAME_HD void operator()(i64 t) const
+    {
+        for (int i = 0; i < cnt; ++i)
+            if (t < n[i]) p[i][t] = 0.0;
+    }
+};
+
+struct ZeroList {
+    ZeroMany z{};
+    i64 longest = 0;
+    void add(double *p, size_t n)
+    {
+        z.p[z.cnt] = p;
+        z.n[z.cnt++] = (i64)n;
+        if ((i64)n > longest) longest = (i64)n;
+    }
+    void add_cot(const Cot &c)
+    {
+        const size_t S = (size_t)M->d.s, V = (size_t)M->d.v;
+        add(c.rho, S); add(c.rt, S); add(c.tp, S); add(c.ep, S); add(c.wind, V);
+    }
+    void run(hipStream_t st) const { launch("k_zero_many", z, longest, st); }
+};
 
 int zero_cot(const Cot &c)
 {
@@ -319,23 +381,33 @@ int restore_checkpoint(int c)
 /* reverse of the momentum advection of (wind, rho) for the cotangent advb of
    pot_vort_tend - 0.5 v_squared_grad (vector_tendencies_expl.c:24-35),
    accumulated into b_wind and b_rho */
-int advection_adjoint(const double *wind, const double *rho, const double *advb, double *b_wind, double *b_rho)
+/* the reverse of the momentum advection of (wind, rho); flux and pv are the
+   forward's mass flux and potential vorticity if taped, else null (they are
+   then recomputed) */
+int advection_adjoint(const double *wind, const double *rho, const double *flux, const double *pv,
+                      const double *advb, double *b_wind, double *b_rho)
 {
     Window &w = *Wn;
     const DevGrid &G = M->G;
     const AdjTables &A = M->A;
     hipStream_t st = M->stream;
-    hor::advection(G, wind, rho, w.AW, st);                    /* the forward's mass flux and PV */
+    if (!flux || !pv) {
+        hor::advection(G, wind, rho, w.AW, st);
+        flux = w.AW.flux;
+        pv = w.AW.pv;
+    }
     /* kinetic energy: -0.5 grad(v.v) */
     launch("k_axpy", sadj::Axpy{G.v, -0.5, advb, w.tmpV1, 0}, G.v, st);
     adj::grad_adjoint(G, A, w.tmpV1, w.vsqb, st);
     adj::inner_product_adjoint(G, A, wind, w.vsqb, w.tmpV2, st);
     launch("k_axpy", sadj::Axpy{G.v, 2.0, w.tmpV2, b_wind, 1}, G.v, st);
     /* TRSK flux: in the mass flux and in the PV */
-    adj::vorticity_flux_adjoint(G, A, w.AW.pv, advb, w.fluxb, st);
-    launch("k_vflux_pv_adj", sadj::VFluxPvAdj{G, A, M->PR, w.AW.flux, advb, w.pvb}, G.curl, st);
+    adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, st);
+    const int vf_chunks = (G.L + VF_CH - 1)/VF_CH;
+    launch("k_vflux_pv_adj_rh", sadj::VFluxPvAdjRh{G, M->PR, flux, advb, w.pvb, vf_chunks}, (i64)vf_chunks*G.nvh, st);
+    launch("k_vflux_pv_adj_tan", sadj::VFluxPvAdjTan{G, A, flux, advb, w.pvb}, (i64)(G.L + 1)*G.nvh, st);
     /* PV = (rel_vort + f)/density */
-    launch("k_pv_div_adj", sadj::PvDivAdj{G, w.pvb, w.AW.pv, rho, w.rvb, w.dvb}, G.curl, st);
+    launch("k_pv_div_adj", sadj::PvDivAdj{G, w.pvb, pv, rho, w.rvb, w.dvb}, G.curl, st);
     launch("k_rhombus_dens_adj", sadj::RhombusDensAdj{G, A, w.dvb, b_rho}, G.s, st);
     launch("k_half_dens_adj", sadj::HalfDensAdj{G, w.dvb, b_rho}, G.s, st);
     launch("k_rhombus_vort_adj", sadj::RhombusVortAdj{G, w.rvb, w.zb}, (i64)G.L*G.nt, st);
@@ -353,6 +425,21 @@ int advection_adjoint(const double *wind, const double *rho, const double *advb,
    step produced (whose wind edges are next->wind), w.bP the cotangent of the
    step's pressure gradient nl + l, w.bA that of its second-substep advection;
    on return w.bS, w.bP, w.bA hold the same for the step before */
+/* the reverse of the gas's scalar tendencies (scalar_tendencies_expl.c:107-176)
+   of densities (rho, rt) with the updated wind: the wind's cotangent goes to
+   w.bU, the densities' are added to (b_rho, b_rt) */
+static void scalar_tend_adjoint(const double *rho, const double *rt, const double *wind, const double *b_trho,
+                                const double *b_trt, double *b_rho, double *b_rt)
+{
+    Window &w = *Wn;
+    const DevGrid &G = M->G;
+    hipStream_t st = M->stream;
+    if (!G.flat) launch("k_vcc_cot", sadj::VccCot{G, b_trho, b_trt, w.vcb}, (i64)G.n_oro*G.nsh, st);
+    launch("k_scalar_tend_adj_e", sadj::ScalarTendAdjE{G, M->A, rho, rt, wind, b_trho, b_trt, w.vcb, w.Fb, w.Fthb,
+                                                       w.bU}, (i64)G.L*G.nvh, st);
+    launch("k_scalar_tend_adj_c", sadj::ScalarTendAdjC{G, rho, rt, w.Fb, w.Fthb, b_rho, b_rt}, G.s, st);
+}
+
 /* the reverse of the sound-wave solver of one substep (column_solvers.c:19-383) */
 static void ver_waves_adjoint(const sadj::VwArgs &a, hipStream_t st)
 {
@@ -377,10 +464,15 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
     const double chw = 0.5 + iw, ohw = 1.0 - chw, cvw = 1.0 - iw;
     const size_t S = (size_t)G.s, V = (size_t)G.v;
     int rc;
-    if ((rc = zero_cot(w.bSn)) || (rc = zero_cot(w.bS1)) || (rc = zero(w.bt0_rho, S)) || (rc = zero(w.bt0_rt, S))
-        || (rc = zero(w.bt0_wind, V)) || (rc = zero(w.bt1_rho, S)) || (rc = zero(w.bt1_rt, S))
-        || (rc = zero(w.bt1_wind, V)) || (rc = zero(w.bPold, V)))
-        return rc;
+    {
+        ZeroList zl;
+        zl.add_cot(w.bSn);
+        zl.add_cot(w.bS1);
+        zl.add(w.bt0_rho, S); zl.add(w.bt0_rt, S); zl.add(w.bt0_wind, V);
+        zl.add(w.bt1_rho, S); zl.add(w.bt1_rt, S); zl.add(w.bt1_wind, V);
+        zl.add(w.bPold, V);
+        zl.run(st);
+    }
     /* ---- second substep ---- */
     {
         sadj::VwArgs a{G, 1, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s1.rho, T.s1.rt, T.s1.ep,
@@ -390,14 +482,11 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
         ver_waves_adjoint(a, st);
     }
     launch("k_pass_adj", PassThroughAdj{G, w.bS.wind, w.bU, w.bSn.wind}, G.v, st);
-    launch("k_scalar_tend_adj_e", sadj::ScalarTendAdjE{G, A, T.s1.rho, T.s1.rt, next.wind, w.bt1_rho, w.bt1_rt, w.Fb,
-                                                       w.Fthb, w.bU}, (i64)G.L*G.nvh, st);
-    launch("k_scalar_tend_adj_c", sadj::ScalarTendAdjC{G, T.s1.rho, T.s1.rt, next.wind, w.Fb, w.Fthb, w.bS1.rho,
-                                                       w.bS1.rt}, G.s, st);
+    scalar_tend_adjoint(T.s1.rho, T.s1.rt, next.wind, w.bt1_rho, w.bt1_rt, w.bS1.rho, w.bS1.rt);
     launch("k_wind_update_adj", sadj::WindUpdateAdj{G, dt, w.bU, w.bSn.wind, w.bt1_wind, 0}, G.v, st);
     launch("k_tend_adj_rk1", sadj::TendAdjRk1{G, ohw, chw, cvw, w.bt1_wind, w.bA, w.bt0_wind, w.bPold, w.bP, w.bA1},
            G.v, st);
-    if ((rc = advection_adjoint(T.s1.wind, T.s1.rho, w.bA1, w.bS1.wind, w.bS1.rho))) return rc;
+    if ((rc = advection_adjoint(T.s1.wind, T.s1.rho, T.flux, T.pv, w.bA1, w.bS1.wind, w.bS1.rho))) return rc;
     /* ---- first substep ---- */
     {
         sadj::VwArgs a{G, 0, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s.rho, T.s.rt, T.s.ep,
@@ -407,20 +496,18 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
         ver_waves_adjoint(a, st);
     }
     launch("k_pass_adj", PassThroughAdj{G, w.bS1.wind, w.bU, w.bSn.wind}, G.v, st);
-    launch("k_scalar_tend_adj_e", sadj::ScalarTendAdjE{G, A, T.s.rho, T.s.rt, T.s1.wind, w.bt0_rho, w.bt0_rt, w.Fb,
-                                                       w.Fthb, w.bU}, (i64)G.L*G.nvh, st);
-    launch("k_scalar_tend_adj_c", sadj::ScalarTendAdjC{G, T.s.rho, T.s.rt, T.s1.wind, w.Fb, w.Fthb, w.bSn.rho,
-                                                       w.bSn.rt}, G.s, st);
+    scalar_tend_adjoint(T.s.rho, T.s.rt, T.s1.wind, w.bt0_rho, w.bt0_rt, w.bSn.rho, w.bSn.rt);
     launch("k_wind_update_adj", sadj::WindUpdateAdj{G, dt, w.bU, w.bSn.wind, w.bt0_wind, 1}, G.v, st);
     launch("k_tend_adj_rk0", sadj::TendAdjRk0{G, ohw, chw, cvw, w.bt0_wind, w.bPold, w.bP, w.bAprev}, G.v, st);
     if (first) {
         /* pgrad_old = -(nl + l) of this step; the advection of the first
            substep computed from this step's state */
         launch("k_axpy", sadj::Axpy{G.v, -1.0, w.bPold, w.bP, 1}, G.v, st);
-        if ((rc = advection_adjoint(T.s.wind, T.s.rho, w.bAprev, w.bSn.wind, w.bSn.rho))) return rc;
+        if ((rc = advection_adjoint(T.s.wind, T.s.rho, nullptr, nullptr, w.bAprev, w.bSn.wind, w.bSn.rho)))
+            return rc;
     }
     /* ---- pressure gradient: P = mean(c_p (theta_bg + tp)) grad(ep) + mean(c_p tp) grad(exner_bg) ---- */
-    if ((rc = zero(w.tmpV1, V))) return rc;
+    if ((rc = zero(w.tmpV1, V))) return rc;     /* advection_adjoint used it */
     hor::grad(G, T.s.ep, w.tmpV1, st);
     launch("k_stv_adj_v", sadj::StvAdjV{G, G.thbg, T.s.tp, w.bP, GC_C_D_P, 1, 0, w.tmpV2}, G.v, st);
     adj::grad_adjoint(G, A, w.tmpV2, w.tmpS, st);
@@ -451,10 +538,17 @@ int window_run(int n_steps, double dt, int k, const double *rho0, const double *
     hipStream_t st = M->stream;
     /* forward sweep with checkpoints; the carried forcings of a checkpoint at
        step 0 are not read (the first step forms its own) */
+    /* the last segment is taped as the sweep runs it, so the reverse sweep
+       starts without recomputing it */
+    const int last0 = (n_ckpt - 1)*k;
+    w.n_ev = 0;
+    w.mark(PH_FORWARD, st);
     if ((rc = load_controls(rho0, rt0, wind0))) return rc;
     for (int n = 0; n < n_steps; ++n) {
-        if (n % k == 0 && (rc = save_checkpoint(n/k))) return rc;
-        if ((rc = game_gpu_run_steps(1, dt, n == 0))) return rc;   /* graph replays after the first */
+        if (n % k == 0 && n < last0 && (rc = save_checkpoint(n/k))) return rc;
+        if (n >= last0) rc = window_step(dt, n == 0, &w.tape[n - last0]);
+        else rc = game_gpu_run_steps(1, dt, n == 0);               /* graph replays after the first */
+        if (rc) return rc;
     }
     /* the terminal cotangent from the final state */
     if ((rc = zero_cot(w.bS)) || (rc = zero(w.bP, V)) || (rc = zero(w.bA, V))) return rc;
@@ -462,15 +556,20 @@ int window_run(int n_steps, double dt, int k, const double *rho0, const double *
     /* reverse sweep by segments */
     for (int c = n_ckpt - 1; c >= 0; --c) {
         const int n0 = c*k, n1 = (n0 + k < n_steps) ? n0 + k : n_steps;
-        if ((rc = restore_checkpoint(c))) return rc;
-        for (int n = n0; n < n1; ++n)
-            if ((rc = window_step(dt, n == 0, &w.tape[n - n0]))) return rc;
+        w.mark(c == n_ckpt - 1 ? PH_FORWARD : PH_REVERSE, st);
+        if (c < n_ckpt - 1) {
+            if ((rc = restore_checkpoint(c))) return rc;
+            for (int n = n0; n < n1; ++n)
+                if ((rc = window_step(dt, n == 0, &w.tape[n - n0]))) return rc;
+        }
         if ((rc = copy_state(w.seg_end, M->s[GAME_STATE_NEW]))) return rc;
+        w.mark(PH_RECOMPUTE, st);
         for (int n = n1 - 1; n >= n0; --n) {
             const DevState &next = (n == n1 - 1) ? w.seg_end : w.tape[n - n0 + 1].s;
             if ((rc = step_adjoint(w.tape[n - n0], next, dt, n == 0))) return rc;
         }
     }
+    w.mark(PH_REVERSE, st);
     /* the control variables: rho, rhotheta_v, wind */
     {
         const DevState &s0 = w.tape[0].s;
@@ -479,8 +578,10 @@ int window_run(int n_steps, double dt, int k, const double *rho0, const double *
     }
     if ((rc = copy(rho_bar, w.bS.rho, S)) || (rc = copy(rt_bar, w.bS.rt, S)) || (rc = copy(wind_bar, w.bS.wind, V)))
         return rc;
+    w.mark(PH_FINISH, st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
+    w.read_clock();
     return 0;
 }
 
@@ -556,6 +657,14 @@ int game_gpu_window_gradient(int n_steps, double dt, int checkpoint_every, const
         return 0;
     };
     return window_run(n_steps, dt, checkpoint_every, rho0, rt0, wind0, terminal, rho_bar, rt_bar, wind_bar);
+}
+
+int game_gpu_window_last_timing(double *ms)
+{
+    if (!M) return GAME_ERR_NOT_INIT;
+    if (!ms) return GAME_ERR_ARG;
+    for (int i = 0; i < 4; ++i) ms[i] = (Wn && Wn->owner == M->generation) ? Wn->phase_ms[i] : 0.0;
+    return 0;
 }
 
 }  /* extern "C" */

@@ -121,6 +121,8 @@ struct Model {
 struct StepTape {
     DevState s, s1;
     double *td0_rho, *td0_rt, *td0_wind, *td1_rho, *td1_rt, *td1_wind;
+    /* the second substep's mass flux and potential vorticity (FP64 form) */
+    double *flux, *pv;
 };
 
 extern Model *M;

@@ -27,6 +27,7 @@ struct PvReverse {
     const int *e;        /* PVR_MAX x NVH: output edge */
     const int *f;        /* PVR_MAX x NVH: flux edge */
     const double *w;     /* PVR_MAX x NVH: weight */
+    const int *n;        /* NVH: the number of entries of each rhombus */
 };
 
 namespace sadj {
@@ -34,6 +35,13 @@ namespace sadj {
 GAME_HD inline i64 eidx(const DevGrid &G, int l, int e) { return G.nsh + (i64)l*G.vpl + e; }
 GAME_HD inline i64 widx(const DevGrid &G, int lev, int h) { return (i64)lev*G.vpl + h; }
 GAME_HD inline i64 sidx(const DevGrid &G, int l, int h) { return (i64)l*G.nsh + h; }
+/* (q, r) = (t / n, t % n) for a flat index below 2^32 (every array here) */
+GAME_HD inline void split_index(i64 t, int n, int &q, int &r)
+{
+    const unsigned u = (unsigned)t, m = (unsigned)n;
+    q = (int)(u/m);
+    r = (int)(u - (unsigned)q*m);
+}
 
 /* ------------------------------------------------------------------ */
 /* three_band_solver_ver_waves, column_solvers.c:19-383: one column     */
@@ -503,65 +511,95 @@ GAME_HD inline double vcc_cot_h(const DevGrid &G, const double *db, int c, int l
     return db[s0]/G.vol[s0]*a - db[s1]/G.vol[s1]*a;
 }
 
-/* (divv_h^T db) at edge e of layer l (divergences.c:14-64, averaging.c:41-95) */
-GAME_HD inline double divv_adj_edge(const DevGrid &G, const AdjTables &A, const double *db, int l, int e)
+/* the cotangents of the vertical contravariant corrections of both fluxes
+   at (cell, terrain level L - n_oro + k): vc[k*NSH + c] for the rhotheta_v
+   flux's tendency cotangent, vc[n_oro*NSH + k*NSH + c] for the mass flux's */
+struct VccCot {
+    DevGrid G;
+    const double *b_trho, *b_trt;
+    double *vc;
+    GAME_HD void operator()(i64 t) const
+    {
+        const i64 n = (i64)G.n_oro*G.nsh;
+        if (t >= n) return;
+        const unsigned u = (unsigned)t, ns = (unsigned)G.nsh;
+        const int k = (int)(u/ns), c = (int)(u - (unsigned)k*ns);
+        const int lev = G.L - G.n_oro + k;
+        vc[t] = vcc_cot_h(G, b_trt, c, lev);
+        vc[n + t] = vcc_cot_h(G, b_trho, c, lev);
+    }
+};
+
+/* (divv_h^T db) at edge e of layer l (divergences.c:14-64, averaging.c:41-95),
+   with the terrain levels' correction cotangents vc of db (VccCot) */
+GAME_HD inline double divv_adj_edge(const DevGrid &G, const AdjTables &A, const double *db, const double *vc, int l,
+                                    int e, int fr, int to)
 {
     const i64 vi = eidx(G, l, e);
-    const int fr = G.from[e], to = G.to[e];
     const i64 sf = sidx(G, l, fr), st = sidx(G, l, to);
     double r = G.area[vi]*(db[sf]/G.vol[sf] - db[st]/G.vol[st]);
-    if (!G.flat && l >= G.L - G.n_oro) {
+    const int top = G.L - G.n_oro;
+    if (!G.flat && l >= top) {
         const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
         for (int p = 0; p < 2; ++p) {
             const int c = cell[p];
             const double k = -0.5*G.ipw[8*sidx(G, l, c) + slot[p]]*G.slope[vi];
-            double v = vcc_cot_h(G, db, c, l);
-            if (l + 1 <= G.L - 1) v += vcc_cot_h(G, db, c, l + 1);
+            double v = vc[(i64)(l - top)*G.nsh + c];
+            if (l + 1 <= G.L - 1) v += vc[(i64)(l + 1 - top)*G.nsh + c];
             r += k*v;
         }
     }
     return r;
 }
 
-/* per (layer, edge): the cotangents of F and of theta_v F; ub += mean(rho) Fb */
+/* per (layer, edge): the cotangents of F and of theta_v F; ub += mean(rho) Fb.
+   The cell sums of the densities' cotangents take, per edge,
+   P1 = 0.5 F Fthb (theta_v's) and P2 = 0.5 u Fb (rho's) */
 struct ScalarTendAdjE {
     DevGrid G;
     AdjTables A;
     const double *rho, *rt, *wind;          /* the substep's densities and updated wind */
     const double *b_trho, *b_trt;           /* cotangents of the two tendencies */
-    double *Fb, *Fthb;                      /* L x NVH, written */
+    const double *vc;                       /* VccCot of the two (terrain grids) */
+    double *P1, *P2;                        /* L x NVH, written */
     double *b_wind;                         /* accumulated at the edges */
     GAME_HD void operator()(i64 t) const
     {
         if (t >= (i64)G.L*G.nvh) return;
-        const int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
-        const i64 sf = sidx(G, l, G.from[e]), st = sidx(G, l, G.to[e]);
-        const double fthb = -divv_adj_edge(G, A, b_trt, l, e);
-        const double fb = -divv_adj_edge(G, A, b_trho, l, e) + 0.5*(rt[sf]/rho[sf] + rt[st]/rho[st])*fthb;
-        Fb[t] = fb;
-        Fthb[t] = fthb;
-        b_wind[eidx(G, l, e)] += 0.5*(rho[sf] + rho[st])*fb;
+        const unsigned u_ = (unsigned)t, nv = (unsigned)G.nvh;
+        const int l = (int)(u_/nv), e = (int)(u_ - (unsigned)l*nv);
+        const int fr = G.from[e], to = G.to[e];
+        const i64 sf = sidx(G, l, fr), st = sidx(G, l, to), vi = eidx(G, l, e);
+        const i64 nvc = (i64)G.n_oro*G.nsh;
+        const double fthb = -divv_adj_edge(G, A, b_trt, vc, l, e, fr, to);
+        const double rf = rho[sf], rt_ = rho[st];
+        const double fb = -divv_adj_edge(G, A, b_trho, vc + nvc, l, e, fr, to)
+                          + 0.5*(rt[sf]/rf + rt[st]/rt_)*fthb;
+        const double u = wind[vi];
+        const double F = 0.5*(rf + rt_)*u;
+        P1[t] = 0.5*F*fthb;
+        P2[t] = 0.5*u*fb;
+        b_wind[vi] += 0.5*(rf + rt_)*fb;
     }
 };
 
 /* per (layer, cell): the densities' cotangents from F and theta_v */
 struct ScalarTendAdjC {
     DevGrid G;
-    const double *rho, *rt, *wind;
-    const double *Fb, *Fthb;
+    const double *rho, *rt;
+    const double *P1, *P2;
     double *b_rho, *b_rt;                   /* accumulated */
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.s) return;
-        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        const unsigned u_ = (unsigned)t, ns = (unsigned)G.nsh;
+        const int l = (int)(u_/ns), c = (int)(u_ - (unsigned)l*ns);
         const int ne = c < GAME_N_PENT ? 5 : 6;
         double thvb = 0.0, rb = 0.0;
         for (int j = 0; j < ne; ++j) {
-            const int e = G.adj[6*c + j];
-            const i64 vi = eidx(G, l, e), k = (i64)l*G.nvh + e;
-            const double F = 0.5*(rho[sidx(G, l, G.from[e])] + rho[sidx(G, l, G.to[e])])*wind[vi];
-            thvb += 0.5*F*Fthb[k];
-            rb += 0.5*wind[vi]*Fb[k];
+            const i64 k = (i64)l*G.nvh + G.adj[6*c + j];
+            thvb += P1[k];
+            rb += P2[k];
         }
         const double r = rho[t];
         b_rho[t] += rb - thvb*rt[t]/(r*r);
@@ -585,7 +623,8 @@ struct StvAdjS {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.s) return;
-        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        int l, c;
+        split_index(t, G.nsh, l, c);
         const int ne = c < GAME_N_PENT ? 5 : 6;
         double r = 0.0;
         for (int j = 0; j < ne; ++j) {
@@ -611,7 +650,8 @@ struct StvAdjV {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.v) return;
-        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        int lev, h;
+        split_index(t, G.vpl, lev, h);
         double r = 0.0;
         if (h >= G.nsh) {
             const int e = h - G.nsh;
@@ -632,48 +672,73 @@ GAME_HD inline double vflux_vert_weight(const DevGrid &G, int lev)
     return (lev == 0 || lev == G.L) ? 1.0 : 0.5;
 }
 
-struct VFluxPvAdj {
+/* the rhombus PV rows: thread (chunk, m) walks the layers of its chunk,
+   loading each of m's reverse entries once; at most VF_CH layers per chunk */
+#define VF_CH 8
+struct VFluxPvAdjRh {
     DevGrid G;
-    AdjTables A;
     PvReverse R;
     const double *F, *ob;                   /* mass flux (V), cotangent of the output (V) */
-    double *pvb;                            /* curl, written */
+    double *pvb;                            /* curl, rhombus rows written */
+    int chunks;                             /* ceil(L / VF_CH) or more */
     GAME_HD void operator()(i64 t) const
     {
-        if (t >= G.curl) return;
         const int nvh = G.nvh, L = G.L;
-        const int lev = (int)(t/(2*nvh)), h = (int)(t - (i64)lev*2*nvh);
-        double r = 0.0;
-        if (h >= nvh) {
-            const int m = h - nvh;
-            for (int k = 0; k < PVR_MAX; ++k) {
-                const i64 km = (i64)k*nvh + m;
-                const double w = R.w[km];
-                if (w != 0.0) r += w*F[eidx(G, lev, R.f[km])]*ob[eidx(G, lev, R.e[km])];
-            }
-        } else {
-            const int e = h;
-            const int fr = G.from[e], to = G.to[e];
-            /* the non-standard term of the edge in layer lev (ipw slot 6, w at
-               level lev) and in layer lev - 1 (slot 7, w at level lev) */
-            if (lev <= L - 1)
-                r += -0.5*ob[eidx(G, lev, e)]*(G.ipw[8*sidx(G, lev, fr) + 6]*F[widx(G, lev, fr)]
-                                               + G.ipw[8*sidx(G, lev, to) + 6]*F[widx(G, lev, to)]);
-            if (lev >= 1)
-                r += -0.5*ob[eidx(G, lev - 1, e)]*(G.ipw[8*sidx(G, lev - 1, fr) + 7]*F[widx(G, lev, fr)]
-                                                   + G.ipw[8*sidx(G, lev - 1, to) + 7]*F[widx(G, lev, to)]);
-            /* the vertical outputs of the edge's two cells at level lev */
-            const double vw = vflux_vert_weight(G, lev);
-            const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
-            for (int p = 0; p < 2; ++p) {
-                const int c = cell[p];
-                double s = 0.0;
-                if (lev >= 1) s += G.ipw[8*sidx(G, lev - 1, c) + slot[p]]*F[eidx(G, lev - 1, e)];
-                if (lev <= L - 1) s += G.ipw[8*sidx(G, lev, c) + slot[p]]*F[eidx(G, lev, e)];
-                r += vw*s*ob[widx(G, lev, c)];
-            }
+        if (t >= (i64)chunks*nvh) return;
+        int ch, m;
+        split_index(t, nvh, ch, m);
+        const int l0 = (int)(((i64)L*ch)/chunks), l1 = (int)(((i64)L*(ch + 1))/chunks);
+        double acc[VF_CH];
+#pragma unroll
+        for (int q = 0; q < VF_CH; ++q) acc[q] = 0.0;
+        const int n = R.n[m];
+        for (int k = 0; k < n; ++k) {
+            const i64 km = (i64)k*nvh + m;
+            const double w = R.w[km];
+            const int fe = R.f[km], oe = R.e[km];
+#pragma unroll
+            for (int q = 0; q < VF_CH; ++q)
+                if (l0 + q < l1) acc[q] += w*F[eidx(G, l0 + q, fe)]*ob[eidx(G, l0 + q, oe)];
         }
-        pvb[t] = r;
+#pragma unroll
+        for (int q = 0; q < VF_CH; ++q)
+            if (l0 + q < l1) pvb[(i64)(l0 + q)*2*nvh + nvh + m] = acc[q];
+    }
+};
+
+/* the tangential (half-level) PV rows, per (level, edge) */
+struct VFluxPvAdjTan {
+    DevGrid G;
+    AdjTables A;
+    const double *F, *ob;                   /* mass flux (V), cotangent of the output (V) */
+    double *pvb;                            /* curl, tangential rows written */
+    GAME_HD void operator()(i64 t) const
+    {
+        const int nvh = G.nvh, L = G.L;
+        if (t >= (i64)(L + 1)*nvh) return;
+        int lev, e;
+        split_index(t, nvh, lev, e);
+        const int fr = G.from[e], to = G.to[e];
+        double r = 0.0;
+        /* the non-standard term of the edge in layer lev (ipw slot 6, w at
+           level lev) and in layer lev - 1 (slot 7, w at level lev) */
+        if (lev <= L - 1)
+            r += -0.5*ob[eidx(G, lev, e)]*(G.ipw[8*sidx(G, lev, fr) + 6]*F[widx(G, lev, fr)]
+                                           + G.ipw[8*sidx(G, lev, to) + 6]*F[widx(G, lev, to)]);
+        if (lev >= 1)
+            r += -0.5*ob[eidx(G, lev - 1, e)]*(G.ipw[8*sidx(G, lev - 1, fr) + 7]*F[widx(G, lev, fr)]
+                                               + G.ipw[8*sidx(G, lev - 1, to) + 7]*F[widx(G, lev, to)]);
+        /* the vertical outputs of the edge's two cells at level lev */
+        const double vw = vflux_vert_weight(G, lev);
+        const int cell[2] = {fr, to}, slot[2] = {A.slot_from[e], A.slot_to[e]};
+        for (int p = 0; p < 2; ++p) {
+            const int c = cell[p];
+            double s = 0.0;
+            if (lev >= 1) s += G.ipw[8*sidx(G, lev - 1, c) + slot[p]]*F[eidx(G, lev - 1, e)];
+            if (lev <= L - 1) s += G.ipw[8*sidx(G, lev, c) + slot[p]]*F[eidx(G, lev, e)];
+            r += vw*s*ob[widx(G, lev, c)];
+        }
+        pvb[(i64)lev*2*nvh + e] = r;
     }
 };
 
@@ -712,7 +777,8 @@ struct PvDivAdj {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.curl) return;
-        const int lev = (int)(t/(2*G.nvh)), h = (int)(t - (i64)lev*2*G.nvh);
+        int lev, h;
+        split_index(t, 2*G.nvh, lev, h);
         const double dv = pv_density(G, rho, lev, h);
         rvb[t] = pvb[t]/dv;
         dvb[t] = -pvb[t]*pv[t]/dv;
@@ -728,7 +794,8 @@ struct RhombusVortAdj {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= (i64)G.L*G.nt) return;
-        const int l = (int)(t/G.nt), tri = (int)(t - (i64)l*G.nt);
+        int l, tri;
+        split_index(t, G.nt, l, tri);
         const i64 base = G.nvh + (i64)l*G.dvpl;
         const double at = G.darea[base + tri];
         double r = 0.0;
@@ -757,7 +824,8 @@ struct TanVortAdjE {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= (i64)G.L*G.nvh) return;
-        const int l = (int)(t/G.nvh), e = (int)(t - (i64)l*G.nvh);
+        int l, e;
+        split_index(t, G.nvh, l, e);
         const i64 vi = eidx(G, l, e);
         double r = G.nd[vi]/G.darea[(i64)(l + 1)*G.dvpl + e]*tan_row_cot(G, rvb, l + 1, e);
         if (l >= 1) r += -G.nd[vi]/G.darea[(i64)l*G.dvpl + e]*tan_row_cot(G, rvb, l, e);
@@ -776,22 +844,26 @@ struct TanVortAdjC {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= (i64)(G.L + 1)*G.nsh) return;
-        const int lev = (int)(t/G.nsh), c = (int)(t - (i64)lev*G.nsh);
+        int lev, c;
+        split_index(t, G.nsh, lev, c);
         const int ne = c < GAME_N_PENT ? 5 : 6, L = G.L, o0 = G.L - G.n_oro;
         const i64 wi = widx(G, lev, c);
+        const bool rows = lev >= 1 && lev <= L - 1;
+        const bool t6 = !G.flat && lev >= o0 && lev <= L - 1, t7 = !G.flat && lev - 1 >= o0 && lev - 1 < L - 1;
+        const double ndw = rows ? G.nd[wi] : 0.0;
+        const double w6 = t6 ? G.ipw[8*sidx(G, lev, c) + 6] : 0.0, w7 = t7 ? G.ipw[8*sidx(G, lev - 1, c) + 7] : 0.0;
         double r = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const int e = G.adj[6*c + j];
-            if (lev >= 1 && lev <= L - 1) {
-                const double s = G.from[e] == c ? 1.0 : -1.0;
-                r += s*G.nd[wi]/G.darea[(i64)lev*G.dvpl + e]*tan_row_cot(G, rvb, lev, e);
-            }
-            if (!G.flat) {
-                if (lev >= o0 && lev <= L - 1)
-                    r += 0.5*G.slope[eidx(G, lev, e)]*G.ipw[8*sidx(G, lev, c) + 6]*cvb[(i64)lev*G.nvh + e];
-                if (lev - 1 >= o0 && lev - 1 < L - 1)
-                    r += 0.5*G.slope[eidx(G, lev - 1, e)]*G.ipw[8*sidx(G, lev - 1, c) + 7]
-                         *cvb[(i64)(lev - 1)*G.nvh + e];
+        /* the cell's edges; the sign is +1 where the cell is the edge's from-cell */
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (j < ne) {
+                const int e = G.adj[6*c + j];
+                if (rows) {
+                    const double s = (double)G.sgn[6*c + j];
+                    r += s*ndw/G.darea[(i64)lev*G.dvpl + e]*tan_row_cot(G, rvb, lev, e);
+                }
+                if (t6) r += 0.5*G.slope[eidx(G, lev, e)]*w6*cvb[(i64)lev*G.nvh + e];
+                if (t7) r += 0.5*G.slope[eidx(G, lev - 1, e)]*w7*cvb[(i64)(lev - 1)*G.nvh + e];
             }
         }
         b_wind[wi] += r;
@@ -806,7 +878,8 @@ struct HalfDensAdj {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.s) return;
-        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        int l, c;
+        split_index(t, G.nsh, l, c);
         const int ne = c < GAME_N_PENT ? 5 : 6, L = G.L;
         const i64 nv2 = 2*(i64)G.nvh;
         double r = 0.0;
@@ -814,11 +887,17 @@ struct HalfDensAdj {
             const int e = G.adj[6*c + j];
             if (l >= 1) r += 0.25*dvb[(i64)l*nv2 + e];
             if (l + 1 <= L - 1) r += 0.25*dvb[(i64)(l + 1)*nv2 + e];
-            const double kt = G.pv_top_dz2[e]/G.pv_top_dz1[e], ks = G.pv_sfc_dz2[e]/G.pv_sfc_dz1[e];
-            if (l == 0) r += 0.5*(1.0 + kt)*dvb[e];
-            if (l == 1) r += -0.5*kt*dvb[e];
-            if (l == L - 1) r += 0.5*(1.0 - ks)*dvb[(i64)L*nv2 + e];
-            if (l == L - 2) r += 0.5*ks*dvb[(i64)L*nv2 + e];
+            /* the extrapolations to the top and the surface */
+            if (l <= 1) {
+                const double kt = G.pv_top_dz2[e]/G.pv_top_dz1[e];
+                if (l == 0) r += 0.5*(1.0 + kt)*dvb[e];
+                if (l == 1) r += -0.5*kt*dvb[e];
+            }
+            if (l >= L - 2) {
+                const double ks = G.pv_sfc_dz2[e]/G.pv_sfc_dz1[e];
+                if (l == L - 1) r += 0.5*(1.0 - ks)*dvb[(i64)L*nv2 + e];
+                if (l == L - 2) r += 0.5*ks*dvb[(i64)L*nv2 + e];
+            }
         }
         b_rho[t] += r;
     }
@@ -833,7 +912,8 @@ struct RhombusDensAdj {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.s) return;
-        const int l = (int)(t/G.nsh), c = (int)(t - (i64)l*G.nsh);
+        int l, c;
+        split_index(t, G.nsh, l, c);
         double r = 0.0;
         for (int k = 0; k < ADJ_RH_MAX; ++k) {
             const double w = A.rhp_w[(i64)k*G.nsh + c];
@@ -860,7 +940,8 @@ struct TendAdjRk1 {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.v) return;
-        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        int lev, h;
+        split_index(t, G.vpl, lev, h);
         const bool edge = h >= G.nsh, inner_w = !edge && lev >= 1 && lev <= G.L - 1;
         if (!edge && !inner_w) { advb[t] = advb_next[t]; return; }
         const double b = tb1[t];
@@ -886,7 +967,8 @@ struct TendAdjRk0 {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.v) return;
-        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        int lev, h;
+        split_index(t, G.vpl, lev, h);
         const bool edge = h >= G.nsh, inner_w = !edge && lev >= 1 && lev <= G.L - 1;
         if (!edge && !inner_w) { advb_prev[t] = 0.0; return; }
         const double b = tb0[t];
@@ -927,7 +1009,8 @@ struct WindUpdateAdj {
     GAME_HD void operator()(i64 t) const
     {
         if (t >= G.v) return;
-        const int lev = (int)(t/G.vpl), h = (int)(t - (i64)lev*G.vpl);
+        int lev, h;
+        split_index(t, G.vpl, lev, h);
         if (h < G.nsh) return;
         (void)lev;
         b_wind[t] += ub[t];

@@ -375,6 +375,12 @@ int game_gpu_window_vjp(int n_steps, double dt, int checkpoint_every, const doub
 int game_gpu_window_gradient(int n_steps, double dt, int checkpoint_every, const double *rho0, const double *rt0,
                              const double *wind0, const double *rho_target, double *cost, double *rho_bar,
                              double *rt_bar, double *wind_bar);
+/* device time (ms, from HIP events on the library's stream) of the phases of
+   the last game_gpu_window_vjp/_gradient: [0] the forward sweep with its
+   checkpoints and the taped last segment, [1] the recomputation of the other
+   segments with their tapes, [2] the reverse of every step from its tape,
+   [3] the equation-of-state cotangent and the copies out */
+int game_gpu_window_last_timing(double *ms);
 
 #ifdef __cplusplus
 }
