@@ -299,6 +299,66 @@ def adjoint_ratios(grid, reps=20):
     return out
 
 
+def window_scale(reps=2):
+    """The scale window: res 8 x 55 dry, flat, 12 steps at the automatic time
+    step (45.04 s) checkpointed every 4 steps; forward and gradient wall
+    times, phases, and the device memory in use (hipMemGetInfo) after the
+    gradient, the library's allocations included"""
+    import ctypes as C
+    import torch
+    from game_archived_amd.grid import Grid
+    from game_archived_amd.rkhevi import DeviceModel
+    from game_archived_amd.state import initial_state
+    g = Grid(8, n_layers=55, n_oro_layers=49, moisture_on=0, n_lloyd=2000, rhombus_tolerance=1e-9,
+             trsk_tolerance=1e-9)
+    d = g.dims
+    dt, n, k = 45.04, 12, 4
+    dm = DeviceModel(g)
+    try:
+        sa = initial_state(g, dm.balance_terms(), moist=False, seed=20261202)
+        sb = initial_state(g, dm.balance_terms(), moist=False, seed=20261203)
+        lib = dm.lib
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+        p = lambda t: C.c_void_p(t.data_ptr())
+        xa = [dev(sa.rho), dev(sa.rhotheta_v), dev(sa.wind)]
+        xb = [dev(sb.rho), dev(sb.rhotheta_v), dev(sb.wind)]
+        outs = [torch.zeros(d.s, dtype=torch.float64, device="cuda") for _ in range(3)] + [
+            torch.zeros(d.v, dtype=torch.float64, device="cuda")]
+        grads = [torch.zeros(d.s, dtype=torch.float64, device="cuda"), torch.zeros(d.s, dtype=torch.float64, device="cuda"),
+                 torch.zeros(d.v, dtype=torch.float64, device="cuda")]
+        cost = C.c_double()
+        dm._check(lib.game_gpu_window_forward(n, dt, *map(p, xb), *map(p, outs)))
+        target = outs[0].clone()
+        fwd = lambda: dm._check(lib.game_gpu_window_forward(n, dt, *map(p, xa), *map(p, outs)))
+        grad = lambda: dm._check(lib.game_gpu_window_gradient(n, dt, k, *map(p, xa), p(target), C.byref(cost),
+                                                              *map(p, grads)))
+        fwd()
+        grad()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fwd()
+        t_f = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grad()
+        t_g = (time.perf_counter() - t0) / reps
+        phases = np.zeros(4)
+        dm._check(lib.game_gpu_window_last_timing(phases.ctypes.data_as(C.POINTER(C.c_double))))
+        free, total = torch.cuda.mem_get_info()
+        return {"window": f"res 8 x 55 dry, flat, {n} steps at {dt:g} s, checkpoints every {k} steps",
+                "cells": d.nsh, "forward_ms": round(t_f * 1e3, 2), "gradient_ms": round(t_g * 1e3, 2),
+                "reverse_over_forward": round((t_g - t_f) / t_f, 3),
+                "gradient_phases_ms": {"forward_sweep": round(phases[0], 2), "recompute": round(phases[1], 2),
+                                       "reverse": round(phases[2], 2)},
+                "adjoint_step_over_forward_step": round(phases[2] / (t_f * 1e3), 3),
+                "device_memory_in_use_GiB": round((total - free) / 2**30, 2),
+                "library_allocated_GiB": round(dm.bytes_allocated() / 2**30, 2),
+                "device_memory_total_GiB": round(total / 2**30, 1), "cost": cost.value}
+    finally:
+        dm.close()
+
+
 def window_ratio(lloyd, reps=3, all_kernels=False):
     """The assimilation window (res 6 x 26, dry, Gaussian mountain, 24 steps at
     120 s, checkpoints every 4 steps, J = 1/2 sum (rho_N - rho_target)^2 with the
@@ -404,6 +464,8 @@ def main():
     ap.add_argument("--lloyd", type=int, default=2000)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precision forms to time; the JSON value is the mixed form's when it runs")
+    ap.add_argument("--scale-window", action="store_true",
+                    help="only time the res 8 x 55 window's forward run and gradient, print their JSON and exit")
     ap.add_argument("--window-only", action="store_true",
                     help="only time the assimilation window's forward run and gradient, print their JSON and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -416,6 +478,9 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.scale_window:
+        print(json.dumps(window_scale()), flush=True)
         return
     if args.window_only:
         print(json.dumps(window_ratio(args.lloyd, all_kernels=True)), flush=True)

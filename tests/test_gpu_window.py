@@ -11,7 +11,10 @@ game_gpu_window_*), against:
   a second seeded initial perturbation (seed 20261203);
 * the tangent-linear dot-product test: <M'(x) dx, y> from central
   differences of the forward against <dx, M'(x)^T y> from the adjoint, for
-  random cotangents y of every output field.
+  random cotangents y of every output field;
+* the scale window: res 8 x 55 dry and flat, 12 steps at the automatic time
+  step (45.04 s) checkpointed every 4, against central differences in 4
+  directions, with the device memory in use reported.
 
 Control variables: rho, rhotheta_v and wind; theta_v_pert and exner_pert are
 diagnosed from rho and rhotheta_v by the equation of state.
@@ -209,6 +212,44 @@ def test_tangent_linear_dot_product():
             ad = sum(float(torch.sum(a * b)) for a, b in zip(xb, dxd))
             err = abs(tl - ad) / abs(ad)
             print(f"direction {i}: <M' dx, y> {tl:.12e} <dx, M'^T y> {ad:.12e} rel {err:.2e}", flush=True)
+            assert err <= 1e-6, (i, err)
+    finally:
+        W.close()
+
+
+@pytest.mark.timeout(900)
+def test_scale_window_res8x55():
+    """res 8 x 55 (655362 columns), dry, flat: the gradient of J over 12 steps
+    at 45.04 s checkpointed every 4 steps against central differences
+    (eps = 1e-6) in 4 seeded directions, each within 1e-6 relative."""
+    import torch
+    from game_archived_amd.grid import Grid
+    print("case: scale window, res 8 x 55 dry flat, 12 steps, checkpoint every 4", flush=True)
+    g = Grid(8, n_layers=55, n_oro_layers=49, moisture_on=0, n_lloyd=2000, rhombus_tolerance=1e-9,
+             trsk_tolerance=1e-9)
+    print("grid built", flush=True)
+    s0 = oracle_state(g, seed=SEED_A)
+    sb = oracle_state(g, seed=SEED_B)
+    dt, n, k = 45.04, 12, 4
+    W = Window(g)
+    try:
+        xa = [W.dev(a) for a in controls(s0)]
+        target = W.forward(n, dt, [W.dev(a) for a in controls(sb)])[0].clone()
+        cost, grad = W.gradient(n, dt, k, xa, target)
+        free, total = torch.cuda.mem_get_info()
+        print(f"J {cost}; device memory in use {(total - free) / 2**30:.1f} GiB of {total / 2**30:.0f}; "
+              f"library {W.dm.bytes_allocated() / 2**30:.1f} GiB", flush=True)
+
+        def J(x):
+            r = W.forward(n, dt, x)[0]
+            return 0.5 * float(torch.sum((r - target) ** 2))
+        eps = 1e-6
+        for i, dx in enumerate(directions(g, controls(s0), 4, 31)):
+            dxd = [W.dev(a) for a in dx]
+            gd = sum(float(torch.sum(a * b)) for a, b in zip(grad, dxd))
+            fd = (J([a + eps * b for a, b in zip(xa, dxd)]) - J([a - eps * b for a, b in zip(xa, dxd)])) / (2 * eps)
+            err = abs(fd - gd) / abs(gd)
+            print(f"direction {i}: adjoint {gd:.12e} fd {fd:.12e} rel {err:.2e}", flush=True)
             assert err <= 1e-6, (i, err)
     finally:
         W.close()
