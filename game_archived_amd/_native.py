@@ -125,6 +125,14 @@ class GridExtras(C.Structure):
                 ("triangle_face_unit_sphere", DP), ("oro", DP), ("vorticity_indices_rhombi", IP)]
 
 
+class GridFileExtras(C.Structure):
+    """game_grid_file_extras_t (include/game_files.h)"""
+    _fields_ = [("latitude_vector", DP), ("longitude_vector", DP), ("latitude_scalar_dual", DP),
+                ("longitude_scalar_dual", DP), ("interpol_indices", IP), ("interpol_weights", DP),
+                ("sfc_albedo", DP), ("sfc_rho_c", DP), ("t_conductivity", DP), ("roughness_length", DP),
+                ("n_lloyd_iterations", C.c_int)]
+
+
 class InitParams(C.Structure):
     _fields_ = [("t_sfc", C.c_double), ("lapse_rate", C.c_double), ("z_tropopause", C.c_double),
                 ("p_sfc", C.c_double), ("u_max", C.c_double), ("z_jet_top", C.c_double), ("sigma_t", C.c_double),
@@ -191,6 +199,16 @@ def host_lib():
     lib.game_init_phase2.restype = C.c_int
     lib.game_gauss.argtypes = [C.c_uint64, C.c_longlong, C.c_int]
     lib.game_gauss.restype = C.c_double
+    P = C.POINTER
+    lib.game_grid_file_write.argtypes = [C.c_char_p, P(Dims), P(GridView), P(DualgridView), P(GridFileExtras)]
+    lib.game_grid_file_read.argtypes = [C.c_char_p, P(Dims), P(GridView), P(DualgridView), P(GridFileExtras)]
+    lib.game_latlon_interpolation.argtypes = [C.c_int, DP, DP, IP, IP, IP, IP, DP]
+    lib.game_sfc_properties.argtypes = [C.c_int, IP, DP, DP, DP, DP]
+    lib.game_state_file_write.argtypes = [C.c_char_p, P(Dims), DP, DP, DP, DP, DP, DP, DP, DP, C.c_int]
+    lib.game_state_file_read.argtypes = [C.c_char_p, P(Dims), DP, DP, DP, DP, IP]
+    lib.game_nc_last_error.restype = C.c_char_p
+    lib.game_state_from_temperature.argtypes = [P(Dims), P(GridView), DP, DP, P(StateView)]
+    lib.game_state_from_temperature.restype = C.c_int
     return lib
 
 

@@ -43,7 +43,8 @@ def build_host():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libgame_host.so")
     run(["gcc", *HOST_FLAGS, "-shared", "-o", out + ".tmp",
-         os.path.join(CSRC, "gridgen.c"), os.path.join(CSRC, "init_state.c"), "-lm"])
+         os.path.join(CSRC, "gridgen.c"), os.path.join(CSRC, "init_state.c"), os.path.join(CSRC, "ncio.c"),
+         os.path.join(CSRC, "gridfile.c"), "-lm"])
     os.replace(out + ".tmp", out)
     return out
 
@@ -73,7 +74,7 @@ def build_dropins():
         run(["gcc", "-O2", "-fPIC", "-std=gnu99", "-Wall", "-shared",
              f"-DGAME_RES_ID={r}", f"-DGAME_NO_OF_LAYERS={l}", f"-DGAME_MOISTURE_ON={m}",
              "-o", out + ".tmp", os.path.join(CSRC, "dropin_shim.c"),
-             "-L" + LIB, "-lgame_rkhevi", "-Wl,-rpath,$ORIGIN"])
+             "-L" + LIB, "-lgame_rkhevi", "-lgame_host", "-lm", "-Wl,-rpath,$ORIGIN"])
         os.replace(out + ".tmp", out)
         outs.append(out)
     return outs

@@ -16,8 +16,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #include "../../include/game_abi.h"
 #include "../../include/game_rkhevi.h"
+#include "../../include/game_files.h"
 
 static const Grid *g_grid = NULL;
 static game_grid_view_t g_gv;
@@ -57,16 +59,21 @@ unsigned long long game_abi_fingerprint(void)
     return h;
 }
 
-int game_dropin_init(Grid *grid, Dualgrid *dualgrid, Config *config)
+static game_dims_t layout_dims(int n_oro_layers)
 {
     game_dims_t d;
     memset(&d, 0, sizeof d);
     d.res_id = GAME_RES_ID;
     d.n_layers = GAME_NO_OF_LAYERS;
     d.moisture_on = GAME_MOISTURE_ON;
-    d.n_oro_layers = grid->no_of_oro_layers;
+    d.n_oro_layers = n_oro_layers;
     game_dims_derive(&d);
-    game_grid_view_t *g = &g_gv;
+    return d;
+}
+
+/* views of the members of struct grid / dualgrid the library reads or fills */
+static void struct_views(Grid *grid, Dualgrid *dualgrid, game_grid_view_t *g, game_dualgrid_view_t *dg)
+{
     memset(g, 0, sizeof *g);
     g->no_of_oro_layers = grid->no_of_oro_layers;
     g->normal_distance = grid->normal_distance; g->volume = grid->volume; g->area = grid->area;
@@ -82,15 +89,71 @@ int game_dropin_init(Grid *grid, Dualgrid *dualgrid, Config *config)
     g->density_to_rhombi_weights = grid->density_to_rhombi_weights; g->trsk_weights = grid->trsk_weights;
     g->is_land = grid->is_land; g->toa = grid->toa; g->radius = grid->radius;
     g->stretching_parameter = grid->stretching_parameter; g->oro_id = grid->oro_id;
-    game_dualgrid_view_t *dg = &g_dgv;
     dg->area = dualgrid->area; dg->z_vector = dualgrid->z_vector; dg->normal_distance = dualgrid->normal_distance;
     dg->from_index = dualgrid->from_index; dg->to_index = dualgrid->to_index;
     dg->vorticity_indices_triangles = dualgrid->vorticity_indices_triangles;
     dg->vorticity_signs_triangles = dualgrid->vorticity_signs_triangles; dg->f_vec = dualgrid->f_vec;
+}
+
+int game_dropin_init(Grid *grid, Dualgrid *dualgrid, Config *config)
+{
+    game_dims_t d = layout_dims(grid->no_of_oro_layers);
+    struct_views(grid, dualgrid, &g_gv, &g_dgv);
     game_config_t c = config_copy(config);
-    int rc = game_gpu_init(&d, g, dg, &c, GAME_FP64);
+    int rc = game_gpu_init(&d, &g_gv, &g_dgv, &c, GAME_FP64);
     if (rc) { fail(rc, "game_gpu_init"); return rc; }
     g_grid = grid;
+    return 0;
+}
+
+/* src/io/set_grid_properties.c:21-253 without the netCDF library: the grid
+   file's variables into the structs, the derived metrics, and the soil
+   grid the reference sets up at the end */
+int set_grid_properties(Grid *grid, Dualgrid *dualgrid, char grid_file_name[])
+{
+    int n_oro = -1;
+    game_nc_file_t *f = game_nc_open(grid_file_name);
+    if (!f || game_nc_get(f, "no_of_oro_layers", GAME_NC_INT, &n_oro, 1) != 0) {
+        fprintf(stderr, "game drop-in: reading %s failed: %s\n", grid_file_name, game_nc_last_error());
+        if (f) game_nc_close(f);
+        if (!getenv("GAME_GPU_NOABORT")) exit(2);
+        return GAME_NC_EIO;
+    }
+    game_nc_close(f);
+    game_dims_t d = layout_dims(n_oro);
+    game_grid_view_t g;
+    game_dualgrid_view_t dg;
+    struct_views(grid, dualgrid, &g, &dg);
+    game_grid_file_extras_t x;
+    memset(&x, 0, sizeof x);
+    x.interpol_indices = grid->latlon_interpol_indices;
+    x.interpol_weights = grid->latlon_interpol_weights;
+    x.sfc_albedo = grid->sfc_albedo;
+    x.sfc_rho_c = grid->sfc_rho_c;
+    x.t_conductivity = grid->t_conduc_soil;
+    x.roughness_length = grid->roughness_length;
+    int rc = game_grid_file_read(grid_file_name, &d, &g, &dg, &x);
+    if (rc) {
+        fprintf(stderr, "game drop-in: reading %s failed: %s\n", grid_file_name, game_nc_last_error());
+        if (!getenv("GAME_GPU_NOABORT")) exit(2);
+        return rc;
+    }
+    grid->no_of_oro_layers = g.no_of_oro_layers;
+    grid->toa = g.toa;
+    grid->radius = g.radius;
+    grid->stretching_parameter = g.stretching_parameter;
+    /* set_grid_properties.c:223-253: the soil's constant-temperature depth
+       and temperature, and the soil layers (sigma 0.352, rescaled to the depth) */
+    const double t_0 = 273.15, sigma_soil = 0.352;
+    grid->z_t_const = -10.0;
+    for (int i = 0; i < GA_NO_OF_SCALARS_H; ++i) grid->t_const_soil[i] = t_0 + 25.0*cos(2.0*grid->latitude_scalar[i]);
+    grid->z_soil_interface[0] = 0;
+    for (int i = 1; i < GA_NO_OF_SOIL_LAYERS + 1; ++i)
+        grid->z_soil_interface[i] = grid->z_soil_interface[i - 1] + pow(sigma_soil, GA_NO_OF_SOIL_LAYERS - i);
+    const double rescale_factor = grid->z_t_const/grid->z_soil_interface[GA_NO_OF_SOIL_LAYERS];
+    for (int i = 1; i < GA_NO_OF_SOIL_LAYERS + 1; ++i) grid->z_soil_interface[i] = rescale_factor*grid->z_soil_interface[i];
+    for (int i = 0; i < GA_NO_OF_SOIL_LAYERS; ++i)
+        grid->z_soil_center[i] = 0.5*(grid->z_soil_interface[i] + grid->z_soil_interface[i + 1]);
     return 0;
 }
 

@@ -1123,6 +1123,128 @@ static int rhombus_averaging(const game_dims_t *dm, game_grid_view_t *g, game_du
     return 0;
 }
 
+/* the metrics the model derives from a grid file, src/io/set_grid_properties.c:196-221 */
+void game_grid_derived_metrics(const game_dims_t *dm, game_grid_view_t *g)
+{
+    const int nsh = dm->nsh, vpl = dm->vpl;
+    for (int i = 0; i < 6*nsh; ++i) if (g->adjacent_vector_indices_h[i] == -1) g->adjacent_vector_indices_h[i] = 0;
+    for (long long i = 0; i < dm->s; ++i) {
+        int l = (int)(i/nsh), h = (int)(i - (long long)l*nsh);
+        g->layer_thickness[i] = g->z_vector[h + (long long)l*vpl] - g->z_vector[h + (long long)(l + 1)*vpl];
+    }
+    host_grad_hor_cov(dm, g, g->z_scalar, g->slope);
+    host_grad(dm, g, g->gravity_potential, g->gravity_m);
+    host_grad(dm, g, g->exner_bg, g->exner_bg_grad);
+}
+
+/* ------------------------------------------------------------------ */
+/* lat-lon output interpolation, grid_generator/src/interpolate_ll.c     */
+/* ------------------------------------------------------------------ */
+/* The reference scans every cell for each point and picks five times
+   (index_helpers.c:49-67): the pick starts from cell 0 with its distance as
+   the running minimum, without testing whether cell 0 is excluded, and
+   replaces it by each later non-excluded cell with a strictly smaller
+   distance.  The nearest non-excluded cells lie within two rings of the
+   point's nearest cell, which a greedy walk over the cell adjacency finds
+   (the adjacency is the Delaunay graph of the generators), so the pick is
+   made over those candidates and cell 0. */
+int game_latlon_interpolation(int res_id, const double *lat, const double *lon, const int *adj, const int *from,
+                              const int *to, int *out_idx, double *out_w)
+{
+    if (res_id < 0 || res_id > 12 || !lat || !lon || !adj || !from || !to || !out_idx || !out_w) {
+        snprintf(g_err, sizeof g_err, "bad arguments");
+        return 1;
+    }
+    const int nsh = 12 + 10*((1 << (2*res_id)) - 1);
+    const int n_lon = 4 << res_id, n_lat = 2 << res_id;
+    const double delta_latitude = M_PI/n_lat, delta_longitude = 2*M_PI/n_lon;
+    int err = 0;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int la = 0; la < n_lat; ++la) {
+        int cur = 0;
+        for (int lo = 0; lo < n_lon; ++lo) {
+            const long long i = (long long)la*n_lon + lo;
+            const double lat_value = M_PI/2 - 0.5*delta_latitude - la*delta_latitude;
+            const double lon_value = lo*delta_longitude;
+            /* greedy walk to the nearest cell, from the previous point's */
+            double dc = dist_h(lat_value, lon_value, lat[cur], lon[cur], 1);
+            for (int moved = 1; moved;) {
+                moved = 0;
+                const int ne = cur < N_PENT ? 5 : 6;
+                for (int j = 0; j < ne; ++j) {
+                    const int e = adj[6*cur + j], nb = from[e] == cur ? to[e] : from[e];
+                    const double d = dist_h(lat_value, lon_value, lat[nb], lon[nb], 1);
+                    if (d < dc) { dc = d; cur = nb; moved = 1; }
+                }
+            }
+            /* candidates: the nearest cell and two rings around it */
+            int cand[1 + 6 + 36];
+            int nc = 0;
+            cand[nc++] = cur;
+            for (int ring = 0, lo_i = 0; ring < 2; ++ring) {
+                const int hi_i = nc;
+                for (int k = lo_i; k < hi_i; ++k) {
+                    const int c = cand[k], ne = c < N_PENT ? 5 : 6;
+                    for (int j = 0; j < ne; ++j) {
+                        const int e = adj[6*c + j], nb = from[e] == c ? to[e] : from[e];
+                        int seen = 0;
+                        for (int q = 0; q < nc; ++q) if (cand[q] == nb) { seen = 1; break; }
+                        if (!seen && nc < 43) cand[nc++] = nb;
+                    }
+                }
+                lo_i = hi_i;
+            }
+            double cd[43];
+            for (int q = 0; q < nc; ++q) cd[q] = dist_h(lat_value, lon_value, lat[cand[q]], lon[cand[q]], 1);
+            const double d0 = dist_h(lat_value, lon_value, lat[0], lon[0], 1);
+            int picked[5] = {-1, -1, -1, -1, -1};
+            double w[5], wsum = 0;
+            for (int p = 0; p < 5; ++p) {
+                int best = 0;
+                double bmin = d0;
+                /* the smallest index among equal distances: the reference's
+                   scan keeps the first of equal strict minima */
+                for (int q = 0; q < nc; ++q) {
+                    const int c = cand[q];
+                    if (c == 0) continue;
+                    int excl = 0;
+                    for (int r = 0; r < 5; ++r) if (picked[r] == c) excl = 1;
+                    if (excl) continue;
+                    if (cd[q] < bmin || (cd[q] == bmin && best != 0 && c < best)) { bmin = cd[q]; best = c; }
+                }
+                picked[p] = best;
+                const double d = best == 0 ? d0 : bmin;
+                w[p] = 1/(pow(d, 2 + EPS_SECURITY) + EPS_SECURITY);
+                wsum += w[p];
+            }
+            for (int p = 0; p < 5; ++p) {
+                if (picked[p] < 0 || picked[p] >= nsh) err = 1;
+                out_idx[5*i + p] = picked[p];
+                out_w[5*i + p] = w[p]/wsum;
+            }
+        }
+    }
+    if (err) { snprintf(g_err, sizeof g_err, "interpolation index out of range"); return 2; }
+    return 0;
+}
+
+/* grid_generator/src/phys_sfc_properties.c:150-198 for cells that are not land */
+int game_sfc_properties(int nsh, const int *is_land, double *albedo, double *rho_c, double *t_cond, double *rough)
+{
+    const double c_p_water = 4184.0, albedo_water = 0.06, t_conductivity_water = 1.4e-7, density_water = 1024.0;
+    for (int i = 0; i < nsh; ++i) {
+        if (is_land && is_land[i]) {
+            snprintf(g_err, sizeof g_err, "cell %d is land: no land surface properties are generated here", i);
+            return 1;
+        }
+        albedo[i] = albedo_water;
+        rho_c[i] = density_water*c_p_water;
+        rough[i] = fmax(0.0001, 0.08);
+        t_cond[i] = t_conductivity_water;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* the full generator, grid_generator/src/coordinator.c:178-352          */
 /* ------------------------------------------------------------------ */
@@ -1423,15 +1545,7 @@ int game_grid_generate(const game_grid_params_t *P, const game_dims_t *dm,
     rc = coriolis(dm, g, dg, X->latitude_vector, X->longitude_vector, lat_d, lon_d,
                   P->trsk_tolerance > 0 ? P->trsk_tolerance : EPS_SECURITY);
     if (rc) goto done;
-    /* derived metrics, src/io/set_grid_properties.c:196-221 */
-    for (int i = 0; i < 6*nsh; ++i) if (g->adjacent_vector_indices_h[i] == -1) g->adjacent_vector_indices_h[i] = 0;
-    for (long long i = 0; i < dm->s; ++i) {
-        int l = (int)(i/nsh), h = (int)(i - (long long)l*nsh);
-        g->layer_thickness[i] = g->z_vector[h + (long long)l*vpl] - g->z_vector[h + (long long)(l + 1)*vpl];
-    }
-    host_grad_hor_cov(dm, g, g->z_scalar, g->slope);
-    host_grad(dm, g, g->gravity_potential, g->gravity_m);
-    host_grad(dm, g, g->exner_bg, g->exner_bg_grad);
+    game_grid_derived_metrics(dm, g);
 done:
     free(hg.tri_pts);
     return rc;

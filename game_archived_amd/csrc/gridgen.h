@@ -74,6 +74,11 @@ int game_grid_generators(int res_id, int n_lloyd_iterations, double *latlon_out)
 int game_grid_lloyd_state(int res_id, int n_lloyd_iterations, double *latlon_out, double *latlon_dual_out,
                           int *adj_out, int *from_dual_out, int *to_dual_out);
 
+/* src/io/set_grid_properties.c:196-221 on a grid read from a file or just
+   generated: -1 adjacency slots to 0, layer thickness, slope, gravity_m,
+   exner_bg_grad */
+void game_grid_derived_metrics(const game_dims_t *dims, game_grid_view_t *grid);
+
 const char *game_grid_last_error(void);
 
 #ifdef __cplusplus

@@ -48,6 +48,30 @@ class Grid:
                  oro_lat_deg=30.0, oro_lon_deg=90.0, radius_rescale=1.0, cache=True, rhombus_tolerance=0.0,
                  trsk_tolerance=0.0):
         t0 = time.time()
+        self._alloc(res_id, n_layers, n_oro_layers, moisture_on)
+        d = self.dims
+        gen = generators(res_id, n_lloyd, cache=cache)
+        p = N.GridParams()
+        p.res_id, p.n_layers, p.n_oro_layers, p.n_lloyd_iterations = res_id, n_layers, n_oro_layers, n_lloyd
+        p.stretching_parameter, p.toa, p.radius_rescale = stretching, toa, radius_rescale
+        p.oro_id = {"flat": 0, "gaussian": 2}[orography]
+        p.oro_h0, p.oro_half_width = oro_h0, oro_half_width
+        p.oro_lat, p.oro_lon = math.radians(oro_lat_deg), math.radians(oro_lon_deg)
+        p.generator_latlon = N.ptr(gen)
+        # 0: the reference's bound on the rhombus weight sums (1e-10)
+        p.rhombus_tolerance = rhombus_tolerance
+        # 0: the reference's bound on the TRSK kite-area and antisymmetry checks (1e-10)
+        p.trsk_tolerance = trsk_tolerance
+        self._gen = gen
+        self.orography = orography
+        self.n_lloyd = n_lloyd
+        self.gv.no_of_oro_layers = n_oro_layers
+        rc = N.host_lib().game_grid_generate(p, d, self.gv, self.dgv, self.extras_view())
+        if rc != 0:
+            raise RuntimeError(f"grid generation failed ({rc}): {N.host_lib().game_grid_last_error().decode()}")
+        self.build_seconds = time.time() - t0
+
+    def _alloc(self, res_id, n_layers, n_oro_layers, moisture_on):
         self.dims = N.make_dims(res_id, n_layers, moisture_on, n_oro_layers)
         d = self.dims
         f8 = lambda n: np.zeros(int(n), dtype=np.float64)
@@ -73,28 +97,9 @@ class Grid:
             "longitude_scalar_dual": f8(d.nt), "direction_dual": f8(d.nvh), "pent_hex_face_unity_sphere": f8(d.nsh),
             "triangle_face_unit_sphere": f8(d.nt), "oro": f8(d.nsh), "vorticity_indices_rhombi": i4(4 * d.nvh),
         }
-        gen = generators(res_id, n_lloyd, cache=cache)
-        p = N.GridParams()
-        p.res_id, p.n_layers, p.n_oro_layers, p.n_lloyd_iterations = res_id, n_layers, n_oro_layers, n_lloyd
-        p.stretching_parameter, p.toa, p.radius_rescale = stretching, toa, radius_rescale
-        p.oro_id = {"flat": 0, "gaussian": 2}[orography]
-        p.oro_h0, p.oro_half_width = oro_h0, oro_half_width
-        p.oro_lat, p.oro_lon = math.radians(oro_lat_deg), math.radians(oro_lon_deg)
-        p.generator_latlon = N.ptr(gen)
-        # 0: the reference's bound on the rhombus weight sums (1e-10)
-        p.rhombus_tolerance = rhombus_tolerance
-        # 0: the reference's bound on the TRSK kite-area and antisymmetry checks (1e-10)
-        p.trsk_tolerance = trsk_tolerance
-        self._gen = gen
-        self.orography = orography
-        self.n_lloyd = n_lloyd
         self.gv = N.fill_view(N.GridView(), self.g)
         self.gv.no_of_oro_layers = n_oro_layers
         self.dgv = N.fill_view(N.DualgridView(), self.dg)
-        rc = N.host_lib().game_grid_generate(p, d, self.gv, self.dgv, self.extras_view())
-        if rc != 0:
-            raise RuntimeError(f"grid generation failed ({rc}): {N.host_lib().game_grid_last_error().decode()}")
-        self.build_seconds = time.time() - t0
 
     def grid_view(self):
         return self.gv
@@ -104,3 +109,55 @@ class Grid:
 
     def extras_view(self):
         return N.fill_view(N.GridExtras(), self.x)
+
+    def file_arrays(self):
+        """the grid file's arrays that the generator does not keep: the
+        lat-lon output interpolation (grid_generator/src/interpolate_ll.c)
+        and the surface properties of cells that are not land"""
+        d = self.dims
+        lib = N.host_lib()
+        nll = 8 << (2 * d.res_id)
+        a = {"interpol_indices": np.zeros(5 * nll, dtype=np.int32), "interpol_weights": np.zeros(5 * nll)}
+        for k in ("sfc_albedo", "sfc_rho_c", "t_conductivity", "roughness_length"):
+            a[k] = np.zeros(d.nsh)
+        g = self.g
+        if lib.game_latlon_interpolation(d.res_id, N.ptr(g["latitude_scalar"]), N.ptr(g["longitude_scalar"]),
+                                         N.ptr(g["adjacent_vector_indices_h"]), N.ptr(g["from_index"]),
+                                         N.ptr(g["to_index"]), N.ptr(a["interpol_indices"]),
+                                         N.ptr(a["interpol_weights"])) != 0:
+            raise RuntimeError(lib.game_grid_last_error().decode())
+        if lib.game_sfc_properties(d.nsh, N.ptr(g["is_land"]), N.ptr(a["sfc_albedo"]), N.ptr(a["sfc_rho_c"]),
+                                   N.ptr(a["t_conductivity"]), N.ptr(a["roughness_length"])) != 0:
+            raise RuntimeError(lib.game_grid_last_error().decode())
+        return a
+
+    def write_file(self, path):
+        """the reference's grid file (grid_generator/src/coordinator.c:354-611)"""
+        a = self.file_arrays()
+        x = N.fill_view(N.GridFileExtras(), {**self.x, **a})
+        x.n_lloyd_iterations = self.n_lloyd
+        rc = N.host_lib().game_grid_file_write(os.fsencode(path), self.dims, self.gv, self.dgv, x)
+        if rc != 0:
+            raise RuntimeError(f"grid file write failed ({rc}): {N.host_lib().game_nc_last_error().decode()}")
+
+    @classmethod
+    def from_file(cls, path, res_id, n_layers=26, n_oro_layers=23, moisture_on=0):
+        """a grid read as src/io/set_grid_properties.c:21-221 reads it; the
+        file's extra arrays are in .file (interpolation, surface properties)"""
+        self = cls.__new__(cls)
+        t0 = time.time()
+        self._alloc(res_id, n_layers, n_oro_layers, moisture_on)
+        d = self.dims
+        nll = 8 << (2 * d.res_id)
+        self.file = {"interpol_indices": np.zeros(5 * nll, dtype=np.int32), "interpol_weights": np.zeros(5 * nll)}
+        for k in ("sfc_albedo", "sfc_rho_c", "t_conductivity", "roughness_length"):
+            self.file[k] = np.zeros(d.nsh)
+        x = N.fill_view(N.GridFileExtras(), {**self.x, **self.file})
+        rc = N.host_lib().game_grid_file_read(os.fsencode(path), d, self.gv, self.dgv, x)
+        if rc != 0:
+            raise RuntimeError(f"grid file read failed ({rc}): {N.host_lib().game_nc_last_error().decode()}")
+        self.n_lloyd = x.n_lloyd_iterations
+        self.orography = "file"
+        self._gen = None
+        self.build_seconds = time.time() - t0
+        return self

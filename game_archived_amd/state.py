@@ -92,3 +92,41 @@ def initial_state(grid, balance_terms, moist=False, seed=20261202, sigma_t=0.1):
         raise RuntimeError(f"initial state phase 2 failed ({rc})")
     st.init_temperature = work["temperature"]
     return st
+
+
+def write_state_file(path, grid, state, temperature, tke=None, n_soil_layers=5):
+    """the model's state file (src/io/write_output.c:1428-1503): densities,
+    temperature and wind of `state`; rh, rel_vort, divv_h_all_layers and
+    t_soil are written as zeros (output diagnostics), tke if given"""
+    import os
+    d = grid.dims
+    lib = N.host_lib()
+    tk = N.ptr(np.ascontiguousarray(tke, dtype=np.float64)) if tke is not None else None
+    rc = lib.game_state_file_write(os.fsencode(path), d, N.ptr(state.rho), N.ptr(np.ascontiguousarray(temperature)),
+                                   N.ptr(state.wind), None, None, None, tk, None, n_soil_layers)
+    if rc != 0:
+        raise RuntimeError(f"state file write failed ({rc}): {lib.game_nc_last_error().decode()}")
+
+
+def read_state_file(path, grid):
+    """read_init_data (src/io/set_initial_state.c:244-336): densities,
+    temperature and wind from the file, vapour limited to saturation, then
+    rhotheta_v, theta_v_pert and exner_pert from the equation of state
+    (game_state_from_temperature).  Returns (state, temperature, tke or None)."""
+    import ctypes as C
+    import os
+    d = grid.dims
+    lib = N.host_lib()
+    st = State(d)
+    temperature = np.zeros(d.s)
+    tke = np.zeros(d.s)
+    found = C.c_int(0)
+    rc = lib.game_state_file_read(os.fsencode(path), d, N.ptr(st.rho), N.ptr(temperature), N.ptr(st.wind),
+                                  N.ptr(tke), C.byref(found))
+    if rc != 0:
+        raise RuntimeError(f"state file read failed ({rc}): {lib.game_nc_last_error().decode()}")
+    rho = st.rho.copy()
+    rc = lib.game_state_from_temperature(d, grid.grid_view(), N.ptr(rho), N.ptr(temperature), st.view())
+    if rc != 0:
+        raise RuntimeError(f"state from temperature failed ({rc})")
+    return st, temperature, (tke if found.value else None)

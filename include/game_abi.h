@@ -241,6 +241,14 @@ int temperature_diagnostics(State *state, Grid *grid, Diagnostics *diagnostics);
    (src/spatial_operators/spatial_operators.h:26; called at src/coordinator.c:251) */
 int inner_product(Vector_field in_field_0, Vector_field in_field_1, Scalar_field out_field, Grid *grid);
 
+/* replaces src/io/set_grid_properties.c:21-253 (src/io/io.h:6; called at
+   src/coordinator.c:~240): reads the grid file written by the reference's
+   grid generator or by game_grid_file_write, without the netCDF library,
+   derives layer thickness, slope, gravity_m and exner_bg_grad and sets up
+   the soil grid.  Exits with status 2 on a read error, as the reference
+   does, unless GAME_GPU_NOABORT is set. */
+int set_grid_properties(Grid *grid, Dualgrid *dualgrid, char grid_file_name[]);
+
 /* the integrals of write_out_integral (src/io/write_output.c:1515-1596, io.h:12)
    without the text files: out[0..NO_OF_CONSTITUENTS) masses, then rhotheta_v,
    kinetic, potential and internal energy; compensated sums on the device */

@@ -133,6 +133,10 @@ static const struct { const char *s, *m; long off; } members[] = {
     MEMBER(Grid, longitude_scalar), MEMBER(Grid, inner_product_weights), MEMBER(Grid, direction),
     MEMBER(Grid, density_to_rhombi_weights), MEMBER(Grid, trsk_weights), MEMBER(Grid, is_land),
     MEMBER(Grid, toa), MEMBER(Grid, radius), MEMBER(Grid, oro_id), MEMBER(Grid, stretching_parameter),
+    MEMBER(Grid, sfc_albedo), MEMBER(Grid, sfc_rho_c), MEMBER(Grid, t_conduc_soil), MEMBER(Grid, roughness_length),
+    MEMBER(Grid, latlon_interpol_indices), MEMBER(Grid, latlon_interpol_weights), MEMBER(Grid, z_soil_interface),
+    MEMBER(Grid, z_soil_center), MEMBER(Grid, mean_velocity_area), MEMBER(Grid, t_const_soil),
+    MEMBER(Grid, z_t_const), MEMBER(Grid, eff_hor_res),
     MEMBER(Dualgrid, area), MEMBER(Dualgrid, z_vector), MEMBER(Dualgrid, normal_distance),
     MEMBER(Dualgrid, from_index), MEMBER(Dualgrid, to_index), MEMBER(Dualgrid, vorticity_indices_triangles),
     MEMBER(Dualgrid, vorticity_signs_triangles), MEMBER(Dualgrid, f_vec),

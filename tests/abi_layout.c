@@ -17,6 +17,10 @@ int main(void)
     P(Grid, adjacent_signs_h); P(Grid, density_to_rhombi_indices); P(Grid, latitude_scalar); P(Grid, longitude_scalar);
     P(Grid, inner_product_weights); P(Grid, direction); P(Grid, density_to_rhombi_weights); P(Grid, trsk_weights);
     P(Grid, is_land); P(Grid, toa); P(Grid, radius); P(Grid, oro_id); P(Grid, stretching_parameter);
+    P(Grid, sfc_albedo); P(Grid, sfc_rho_c); P(Grid, t_conduc_soil); P(Grid, roughness_length);
+    P(Grid, latlon_interpol_indices); P(Grid, latlon_interpol_weights); P(Grid, z_soil_interface);
+    P(Grid, z_soil_center); P(Grid, mean_velocity_area); P(Grid, t_const_soil); P(Grid, z_t_const);
+    P(Grid, eff_hor_res);
     P(Dualgrid, area); P(Dualgrid, z_vector); P(Dualgrid, normal_distance); P(Dualgrid, from_index);
     P(Dualgrid, to_index); P(Dualgrid, vorticity_indices_triangles); P(Dualgrid, vorticity_signs_triangles);
     P(Dualgrid, f_vec);

@@ -82,3 +82,48 @@ def test_failed_init_leaves_no_model():
     assert lib.game_gpu_integrals(1, N.ptr(out)) == 3
     assert lib.game_gpu_bytes_allocated() == 0
     assert lib.game_gpu_finalize() == 0
+
+
+def test_host_library_exports_every_declared_file_function():
+    lib = C.CDLL(os.path.join(LIB, "libgame_host.so"))
+    names = declared("game_files.h") + ["game_nc_open"]
+    assert "game_grid_file_read" in names and "game_state_file_write" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_dropin_set_grid_properties_reads_a_grid_file(tmp_path):
+    """the drop-in's set_grid_properties (src/io/set_grid_properties.c:21-253)
+    fills the reference's own struct grid and dualgrid (res 5 x 26, the
+    reference's compiled layout) from a grid file: every member equals the
+    generated grid's, the lat-lon interpolation and surface arrays included"""
+    import numpy as np
+    from conftest import get_grid
+    g = get_grid(5, n_lloyd=2000)
+    d = g.dims
+    p = str(tmp_path / "grid.nc")
+    g.write_file(p)
+    extra = g.file_arrays()
+    C.CDLL(os.path.join(LIB, "libgame_host.so"), mode=C.RTLD_GLOBAL)
+    C.CDLL(os.path.join(LIB, "libgame_rkhevi.so"), mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(os.path.join(LIB, "libgame_dropin_r5l26m1.so"))
+    lib.set_grid_properties.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p]
+    G, DG = R.RefStruct("Grid"), R.RefStruct("Dualgrid")
+    assert lib.set_grid_properties(G.ptr, DG.ptr, p.encode()) == 0
+    for m in R.GRID_MEMBERS:
+        a = g.g[m]
+        assert np.array_equal(G.member(m, a.dtype, a.size), a), m
+    for m in R.DUAL_MEMBERS:
+        a = g.dg[m][:d.dual_v] if m == "area" else g.dg[m]
+        assert np.array_equal(DG.member(m, a.dtype, a.size), a), m
+    assert G.member("no_of_oro_layers", np.int32, 1)[0] == 23
+    assert G.member("toa", np.float64, 1)[0] == 41152.0
+    assert np.array_equal(G.member("latlon_interpol_indices", np.int32, extra["interpol_indices"].size),
+                          extra["interpol_indices"])
+    assert np.array_equal(G.member("latlon_interpol_weights", np.float64, extra["interpol_weights"].size),
+                          extra["interpol_weights"])
+    assert np.array_equal(G.member("t_conduc_soil", np.float64, d.nsh), extra["t_conductivity"])
+    z = G.member("z_soil_interface", np.float64, 6)
+    assert z[0] == 0.0 and abs(z[5] + 10.0) < 1e-12 and (np.diff(z) < 0).all()
+    assert G.member("z_t_const", np.float64, 1)[0] == -10.0

@@ -1,0 +1,288 @@
+"""The reference's on-disk formats (include/game_files.h): netCDF classic
+files without the netCDF library, the grid file and the state file.
+
+* The codec against scipy.io.netcdf_file, an independent implementation of
+  the same published format: files written here are read by scipy with the
+  same names, dimensions, types, units and values; files scipy writes (CDF-1
+  and the 64-bit-offset CDF-2) are read here.
+* The grid file: every variable, dimension and unit of
+  grid_generator/src/coordinator.c:354-611; a generated grid written and
+  read back (src/io/set_grid_properties.c:21-221) is bit-identical,
+  derived metrics included.
+* The lat-lon interpolation (grid_generator/src/interpolate_ll.c) against a
+  brute-force restatement of the reference's scan at res 3 and 4 and
+  against the reference's own compiled function at res 5.
+* The state file (src/io/write_output.c:1428-1503) and its reading by
+  read_init_data (src/io/set_initial_state.c:244-336): written, read back
+  and re-derived through the equation of state.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+from conftest import get_grid, oracle_state
+from game_archived_amd import _native as N
+from game_archived_amd.grid import Grid
+from game_archived_amd.state import read_state_file, write_state_file
+
+# grid_generator/src/coordinator.c:354-611: (name, dimension, type, units)
+GRID_SCHEMA = [
+    ("no_of_lloyd_iterations", "single_int_dimid_index", "i", None),
+    ("no_of_oro_layers", "single_int_dimid_index", "i", None),
+    ("stretching_parameter", "single_double_dimid_index", "d", None),
+    ("toa", "single_double_dimid_index", "d", "m"), ("radius", "single_double_dimid_index", "d", "m"),
+    ("latitude_scalar", "scalar_h_index", "d", None), ("longitude_scalar", "scalar_h_index", "d", None),
+    ("latitude_scalar_dual", "scalar_dual_h_index", "d", None),
+    ("longitude_scalar_dual", "scalar_dual_h_index", "d", None),
+    ("z_scalar", "scalar_index", "d", "m"), ("theta_v_bg", "scalar_index", "d", "K"),
+    ("exner_bg", "scalar_index", "d", None), ("gravity_potential", "scalar_index", "d", "m^2/s^2"),
+    ("z_vector", "vector_index", "d", "m"), ("normal_distance", "vector_index", "d", "m"),
+    ("volume", "scalar_index", "d", "m^3"), ("area", "vector_index", "d", "m^2"),
+    ("trsk_weights", "vector_h_10_index", "d", None), ("z_vector_dual", "vector_index_dual", "d", "m"),
+    ("normal_distance_dual", "vector_index_dual", "d", "m"), ("area_dual", "vector_index_dual", "d", "m^2"),
+    ("f_vec", "f_vec_index", "d", "1/s"), ("direction", "vector_h_index", "d", None),
+    ("latitude_vector", "vector_h_index", "d", None), ("longitude_vector", "vector_h_index", "d", None),
+    ("inner_product_weights", "scalar_8_index", "d", None),
+    ("density_to_rhombi_weights", "vector_h_4_index", "d", None),
+    ("interpol_weights", "latlon_3_index", "d", None),
+    ("from_index", "vector_h_index", "i", None), ("to_index", "vector_h_index", "i", None),
+    ("from_index_dual", "vector_h_index", "i", None), ("to_index_dual", "vector_h_index", "i", None),
+    ("adjacent_vector_indices_h", "scalar_h_6_index", "i", None), ("interpol_indices", "latlon_3_index", "i", None),
+    ("trsk_indices", "vector_h_10_index", "i", None), ("trsk_modified_curl_indices", "vector_h_10_index", "i", None),
+    ("adjacent_signs_h", "scalar_h_6_index", "i", None),
+    ("vorticity_signs_triangles", "scalar_dual_h_3_index", "i", None),
+    ("vorticity_indices_triangles", "scalar_dual_h_3_index", "i", None),
+    ("density_to_rhombi_indices", "vector_h_4_index", "i", None),
+    ("sfc_albedo", "scalar_h_index", "d", None), ("sfc_rho_c", "scalar_h_index", "d", "J/(K*m**3)"),
+    ("is_land", "scalar_h_index", "i", None), ("t_conductivity", "scalar_h_index", "d", "m^2/2"),
+    ("roughness_length", "scalar_h_index", "d", "m"),
+]
+
+
+def codec():
+    lib = N.host_lib()
+    lib.game_nc_write.argtypes = [C.c_char_p, C.c_void_p, C.c_int]
+    lib.game_nc_open.argtypes = [C.c_char_p]
+    lib.game_nc_open.restype = C.c_void_p
+    lib.game_nc_inq.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    lib.game_nc_get.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_longlong]
+    lib.game_nc_close.argtypes = [C.c_void_p]
+    return lib
+
+
+class NcVar(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("dim", C.c_char_p), ("len", C.c_longlong), ("type", C.c_int),
+                ("data", C.c_void_p), ("units", C.c_char_p)]
+
+
+def c_write(path, variables):
+    """variables: (name, dim, numpy array, units)"""
+    arr = (NcVar * len(variables))()
+    keep = []
+    for i, (name, dim, a, units) in enumerate(variables):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        arr[i].name, arr[i].dim, arr[i].len = name.encode(), dim.encode(), a.size
+        arr[i].type = 6 if a.dtype == np.float64 else 4
+        arr[i].data = a.ctypes.data
+        arr[i].units = units.encode() if units else None
+    return codec().game_nc_write(path.encode(), C.cast(arr, C.c_void_p), len(variables))
+
+
+def c_read(path, name, dtype, n):
+    lib = codec()
+    h = lib.game_nc_open(path.encode())
+    assert h, lib.game_nc_last_error()
+    try:
+        t, ln = C.c_int(), C.c_longlong()
+        if lib.game_nc_inq(h, name.encode(), C.byref(t), C.byref(ln)) != 0:
+            return None
+        assert ln.value == n
+        out = np.zeros(n, dtype=dtype)
+        rc = lib.game_nc_get(h, name.encode(), 6 if dtype == np.float64 else 4, out.ctypes.data, n)
+        assert rc == 0, lib.game_nc_last_error()
+        return out
+    finally:
+        lib.game_nc_close(h)
+
+
+def test_codec_files_read_by_scipy(tmp_path):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(1001)
+    b = rng.integers(-2**31, 2**31 - 1, 37, dtype=np.int64).astype(np.int32)
+    c = np.array([np.pi])
+    p = str(tmp_path / "x.nc")
+    assert c_write(p, [("alpha", "n_1001", a, "m^2/s^2"), ("beta", "n_37", b, None), ("gamma", "one", c, "K"),
+                       ("delta", "n_1001", -a, None)]) == 0
+    f = netcdf_file(p, "r", mmap=False)
+    try:
+        assert f.version_byte == 1
+        assert list(f.variables) == ["alpha", "beta", "gamma", "delta"]
+        assert f.dimensions == {"n_1001": 1001, "n_37": 37, "one": 1}
+        assert f.variables["alpha"].units == b"m^2/s^2" and f.variables["gamma"].units == b"K"
+        assert not hasattr(f.variables["beta"], "units")
+        assert f.variables["alpha"].typecode() == "d" and f.variables["beta"].typecode() == "i"
+        assert np.array_equal(f.variables["alpha"].data, a) and np.array_equal(f.variables["beta"].data, b)
+        assert np.array_equal(f.variables["delta"].data, -a) and f.variables["gamma"].data[0] == np.pi
+    finally:
+        f.close()
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_codec_reads_scipy_files(tmp_path, version):
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal(999)
+    b = np.arange(-5, 8, dtype=np.int32)
+    p = str(tmp_path / "s.nc")
+    f = netcdf_file(p, "w", version=version)
+    f.title = "global attribute"
+    f.createDimension("n", a.size)
+    f.createDimension("m", b.size)
+    v = f.createVariable("a", "d", ("n",))
+    v[:] = a
+    v.units = "m/s"
+    v.long_name = "a second attribute"
+    w = f.createVariable("b", "i", ("m",))
+    w[:] = b
+    f.close()
+    assert np.array_equal(c_read(p, "a", np.float64, a.size), a)
+    assert np.array_equal(c_read(p, "b", np.int32, b.size), b)
+    assert c_read(p, "absent", np.float64, 1) is None
+
+
+def test_codec_rejects_a_wrong_length_or_type(tmp_path):
+    p = str(tmp_path / "y.nc")
+    assert c_write(p, [("a", "n", np.zeros(10), None)]) == 0
+    lib = codec()
+    h = lib.game_nc_open(p.encode())
+    try:
+        out = np.zeros(11)
+        assert lib.game_nc_get(h, b"a", 6, out.ctypes.data, 11) != 0
+        assert lib.game_nc_get(h, b"a", 4, out.ctypes.data, 10) != 0
+    finally:
+        lib.game_nc_close(h)
+    assert lib.game_nc_open(str(tmp_path / "missing.nc").encode()) is None
+
+
+@pytest.mark.parametrize("res,oro", [(3, "gaussian"), (4, "flat")])
+def test_grid_file_schema_and_round_trip(tmp_path, res, oro):
+    g = get_grid(res, oro)
+    d = g.dims
+    p = str(tmp_path / "grid.nc")
+    g.write_file(p)
+    f = netcdf_file(p, "r", mmap=False)
+    try:
+        assert [v for v in f.variables] == [s[0] for s in GRID_SCHEMA]
+        for name, dim, tc, units in GRID_SCHEMA:
+            v = f.variables[name]
+            assert v.dimensions == (dim,) and v.typecode() == tc, name
+            assert getattr(v, "units", None) == (units.encode() if units else None), name
+        adj = f.variables["adjacent_vector_indices_h"].data.reshape(-1, 6)
+        assert (adj[:12, 5] == -1).all() and (adj[12:] >= 0).all()       # the generator's -1 slots
+        assert f.variables["no_of_oro_layers"].data[0] == d.n_oro_layers
+        assert f.variables["toa"].data[0] == 41152.0
+        assert np.array_equal(f.variables["inner_product_weights"].data, g.g["inner_product_weights"])
+        assert f.dimensions["vector_index_dual"] == d.dual_v
+        assert f.dimensions["latlon_3_index"] == 5 * 8 * 4 ** res
+        assert (f.variables["sfc_albedo"].data == 0.06).all()
+        assert (f.variables["sfc_rho_c"].data == 1024.0 * 4184.0).all()
+    finally:
+        f.close()
+    h = Grid.from_file(p, res, d.n_layers, d.n_oro_layers, d.moisture_on)
+    for k in g.g:
+        assert np.array_equal(g.g[k], h.g[k]), k          # derived metrics recomputed bit for bit
+    for k in g.dg:
+        n = d.dual_v if k == "area" else g.dg[k].size     # area_dual holds the dual vectors
+        assert np.array_equal(g.dg[k][:n], h.dg[k][:n]), k
+    for k in ("latitude_vector", "longitude_vector", "latitude_scalar_dual", "longitude_scalar_dual"):
+        assert np.array_equal(g.x[k], h.x[k]), k
+    assert (h.gv.toa, h.gv.radius, h.gv.stretching_parameter) == (g.gv.toa, g.gv.radius, g.gv.stretching_parameter)
+    with pytest.raises(RuntimeError):
+        Grid.from_file(p, res, d.n_layers, d.n_oro_layers - 1, d.moisture_on)
+
+
+def interpolation_brute_force(res, lat, lon):
+    """interpolate_ll.c:16-74 with index_helpers.c:49-67 as the reference
+    scans: the running minimum starts at cell 0 whether or not cell 0 is
+    excluded, and only a strictly smaller non-excluded distance replaces it"""
+    n_lon, n_lat = 4 * 2 ** res, 2 * 2 ** res
+    dla, dlo = np.pi / n_lat, 2 * np.pi / n_lon
+    idx = np.zeros((n_lat * n_lon, 5), dtype=np.int32)
+    w = np.zeros((n_lat * n_lon, 5))
+    for i in range(n_lat * n_lon):
+        la, lo = divmod(i, n_lon)
+        lat_v = np.pi / 2 - 0.5 * dla - la * dla
+        lon_v = lo * dlo
+        dist = 2.0 * np.arcsin(np.sqrt(0.5 - 0.5 * (np.cos(lat_v) * np.cos(lat) * np.cos(lon - lon_v)
+                                                     + np.sin(lat_v) * np.sin(lat))))
+        picked = []
+        for _ in range(5):
+            excl = np.zeros(dist.size, bool)
+            excl[[p for p in picked]] = True
+            cand = np.where(~excl[1:] & (dist[1:] < dist[0]))[0] + 1
+            best = int(cand[np.argmin(dist[cand])]) if cand.size else 0
+            picked.append(best)
+        idx[i] = picked
+        ww = 1 / (dist[picked] ** (2 + 1e-10) + 1e-10)
+        w[i] = ww / ww.sum()
+    return idx.ravel(), w.ravel()
+
+
+@pytest.mark.parametrize("res", [3, 4])
+def test_latlon_interpolation_against_the_full_scan(res):
+    g = get_grid(res)
+    a = g.file_arrays()
+    idx, w = interpolation_brute_force(res, g.g["latitude_scalar"], g.g["longitude_scalar"])
+    assert np.array_equal(a["interpol_indices"], idx)
+    # numpy's transcendental functions are not libm's: the weights agree to rounding
+    assert np.abs(a["interpol_weights"] - w).max() < 1e-12
+    # the reference's rule repeats cell 0 once it is among the nearest
+    assert (a["interpol_indices"].reshape(-1, 5)[:, 1:] == 0).any()
+
+
+def test_latlon_interpolation_matches_the_reference_at_res5():
+    import refharness as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    g = get_grid(5, n_lloyd=2000)
+    a = g.file_arrays()
+    n = 5 * 8 * 4 ** 5
+    idx, w = np.zeros(n, dtype=np.int32), np.zeros(n)
+    fn = R.ref_fn("interpolate_ll", N.DP, N.DP, N.IP, N.DP)
+    fn(N.ptr(g.g["latitude_scalar"]), N.ptr(g.g["longitude_scalar"]), N.ptr(idx), N.ptr(w))
+    assert np.array_equal(a["interpol_indices"], idx)
+    assert np.array_equal(a["interpol_weights"], w)
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_state_file_round_trip_and_restart(tmp_path, moist):
+    """write_output.c's state file, then read_init_data: densities,
+    temperature and wind come back bit for bit, and the equation of state
+    re-derives the state the temperature came from"""
+    import pyoracle
+    g = get_grid(3, "gaussian", moist=1 if moist else 0)
+    d = g.dims
+    s0 = oracle_state(g, moist=moist)
+    T = np.zeros(d.s)
+    pyoracle.lib().or_temperature_diagnostics(pyoracle.ctx(g), s0.view(), N.ptr(T))
+    p = str(tmp_path / "state.nc")
+    tke = np.linspace(0.0, 1.0, d.s)
+    write_state_file(p, g, s0, T, tke=tke)
+    f = netcdf_file(p, "r", mmap=False)
+    try:
+        assert list(f.variables) == ["densities", "temperature", "wind", "rh", "rel_vort", "divv_h_all_layers",
+                                     "tke", "t_soil"]
+        assert f.variables["densities"].dimensions == ("densities_index",)
+        assert f.variables["densities"].units == b"kg/m^3" and f.variables["wind"].units == b"m/s"
+        assert f.dimensions["soil_index"] == 5 * d.nsh
+        assert np.array_equal(f.variables["wind"].data, s0.wind)
+    finally:
+        f.close()
+    st, T2, tke2 = read_state_file(p, g)
+    assert np.array_equal(T2, T) and np.array_equal(st.wind, s0.wind) and np.array_equal(tke2, tke)
+    assert np.allclose(st.rho, s0.rho, rtol=0, atol=0) or moist     # vapour may be limited to saturation
+    assert np.allclose(st.rhotheta_v, s0.rhotheta_v, rtol=1e-13, atol=0)
+    assert np.allclose(st.exner_pert + g.g["exner_bg"], s0.exner_pert + g.g["exner_bg"], rtol=1e-13, atol=0)
