@@ -464,6 +464,9 @@ def main():
     ap.add_argument("--lloyd", type=int, default=2000)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precision forms to time; the JSON value is the mixed form's when it runs")
+    ap.add_argument("--round-tendencies", choices=("rho", "rhotheta_v", "both"), default=None,
+                    help="precision probe: the mixed form rounds these tendencies to FP32 after the scalar "
+                         "tendencies (sets GAME_GPU_ROUND_TENDENCIES); the FP64 form is unchanged")
     ap.add_argument("--scale-window", action="store_true",
                     help="only time the res 8 x 55 window's forward run and gradient, print their JSON and exit")
     ap.add_argument("--window-only", action="store_true",
@@ -479,6 +482,8 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.round_tendencies:
+        os.environ["GAME_GPU_ROUND_TENDENCIES"] = args.round_tendencies
     if args.scale_window:
         print(json.dumps(window_scale()), flush=True)
         return
@@ -590,6 +595,7 @@ def main():
             "lloyd_iterations": args.lloyd, "value_form": main_name, "forms": form_cfg,
             "rhombus_weight_tolerance": rh_tol or 1e-10, "trsk_check_tolerance": rh_tol or 1e-10,
             "finite": finite,
+            "mixed_tendencies_rounded_to_fp32": args.round_tendencies,
         },
         "roofline": {"bound": "hbm", "kernel": dom_name, "form": main_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

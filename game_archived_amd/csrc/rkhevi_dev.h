@@ -58,6 +58,10 @@ struct DevGrid {
     int split_edge;
     /* 1: tracer solver with one (column, tracer) per lane in registers; 0: LDS-staged */
     int gd_streaming;
+    /* mixed form, a precision probe: bit 0 rounds the density tendencies,
+       bit 1 the rhotheta_v tendency, to FP32 after the scalar tendencies
+       (the precision of FP32 storage, not its bytes) */
+    int round_tendencies;
     /* 1: sound-wave solver with one column per lane; 0: three phases over LDS */
     int vw_streaming;
     /* 1 (26 layers): the row-parallel three-phase sound-wave kernel instead of the layer-run one */

@@ -2219,6 +2219,26 @@ static void vector_tend_t(const DevGrid &G, const DevWorkT<TW> &W, TW *tend_w, c
            cvw, dt);
 }
 
+/* the precision of FP32 storage on an FP64 array (a probe of the mixed form) */
+__global__ void __launch_bounds__(TPB) k_round_f32(double *x, i64 n)
+{
+    const i64 t = (i64)blockIdx.x*TPB + threadIdx.x;
+    if (t < n) x[t] = (double)(float)x[t];
+}
+
+template <typename TW>
+static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkT<TW> &W, int rk,
+                          hipStream_t st);
+
+template <typename TW>
+static void scalar_tend_probe(const DevGrid &G, const DevState &td, hipStream_t st)
+{
+    if (sizeof(TW) != sizeof(float) || !G.round_tendencies) return;
+    if (G.round_tendencies & 1)
+        LAUNCH("k_round_f32", k_round_f32, GRID1((i64)G.n_const*G.s), dim3(TPB), st, td.rho, (i64)G.n_const*G.s);
+    if (G.round_tendencies & 2) LAUNCH("k_round_f32", k_round_f32, GRID1(G.s), dim3(TPB), st, td.rt, G.s);
+}
+
 template <typename TW>
 static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &td, const DevWorkT<TW> &W, int rk,
                           hipStream_t st)
@@ -2241,6 +2261,7 @@ static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &t
         else LAUNCH("k_scalar_tend", (k_scalar_tend<false, TW, 0>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
                     W, td.rho, td.rt, rk);
     }
+    scalar_tend_probe<TW>(G, td, st);
 }
 
 void pgrad(const DevGrid &G, const DevState &s, const DevWork &W, int first, double itw, hipStream_t st)

@@ -209,6 +209,12 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
         if (v >= 1 && v <= d.n_layers) G.split_cell = v;
     }
     G.gd_streaming = getenv("GAME_GPU_GD_STREAMING") ? 1 : 0;
+    G.round_tendencies = 0;
+    if (const char *e = getenv("GAME_GPU_ROUND_TENDENCIES")) {
+        if (!strcmp(e, "rho")) G.round_tendencies = 1;
+        else if (!strcmp(e, "rhotheta_v")) G.round_tendencies = 2;
+        else if (!strcmp(e, "both")) G.round_tendencies = 3;
+    }
     G.vw_streaming = getenv("GAME_GPU_VW_STREAMING") ? 1 : 0;
     G.vw_rows = getenv("GAME_GPU_VW_ROWS") ? 1 : 0;
     G.sh_per_box = getenv("GAME_GPU_SH_PER_BOX") ? 1 : 0;
