@@ -1106,6 +1106,52 @@ __device__ __forceinline__ void thomas_lds(const double *cc, const double *dd, d
     }
 }
 
+/* thomas_lds for systems whose every row has one off-diagonal entry at
+   most: interface j (between rows j and j + 1) stores that entry in ce[j]
+   and in fl[j] whether it is the upper one (e_j, fl 1) or the lower one
+   (c_j, row j + 1's, fl 0); the other is 0.0, as the reference leaves it.
+   The arithmetic is thomas_lds's; the multipliers overwrite ce. */
+template <int NCOL, int NMAX>
+__device__ __forceinline__ void thomas_lds_ce(double *ce, const unsigned char *fl, const double *dd, double *rr,
+                                              int n, int col)
+{
+    const double d0 = dd[col], q0 = ce[col];
+    const bool f0 = fl[col];
+    double ep = 0.0, x;
+    if (n > 1) ep = (f0 ? q0 : 0.0)/d0;
+    x = rr[col]/d0;
+    ce[col] = ep; rr[col] = x;
+    double cp = f0 ? 0.0 : q0, dn = 0.0, rn = 0.0, qn = 0.0;
+    bool fn = false;
+    if (n > 1) { dn = dd[NCOL + col]; rn = rr[NCOL + col]; qn = ce[NCOL + col]; fn = fl[NCOL + col]; }
+#pragma unroll
+    for (int j = 1; j < NMAX; ++j) {
+        if (j >= n) break;
+        const int o = j*NCOL + col, on = (j + 1 < n ? j + 1 : j)*NCOL + col;
+        const double dnn = dd[on], rnn = rr[on], qnn = ce[on];
+        const bool fnn = fl[on];
+        const double en = fn ? qn : 0.0, cpn = fn ? 0.0 : qn;
+        const double den = dn - ep*cp;
+        if (j < n - 1) ep = en/den;
+        x = (rn - x*cp)/den;
+        ce[o] = ep; rr[o] = x;
+        cp = cpn; dn = dnn; rn = rnn; qn = qnn; fn = fnn;
+    }
+    /* back substitution, upwards */
+    double eb = 0.0, rb = 0.0;
+    if (n > 1) { eb = ce[(n - 2)*NCOL + col]; rb = rr[(n - 2)*NCOL + col]; }
+#pragma unroll
+    for (int t = 0; t < NMAX - 1; ++t) {
+        const int j = n - 2 - t;
+        if (j < 0) break;
+        const int o = j*NCOL + col, om = (j > 0 ? j - 1 : 0)*NCOL + col;
+        const double ebn = ce[om], rbn = rr[om];
+        x = rb - eb*x;
+        rr[o] = x;
+        eb = ebn; rb = rbn;
+    }
+}
+
 #define VW_NCOL 32
 #define VW_THREADS 256
 
@@ -1479,14 +1525,17 @@ __global__ void __launch_bounds__(VR_THREADS) k_ver_waves_run(DevGrid G, DevStat
    sediment heat, which k_sed_heat forms separately, so one block solves NCOL
    columns of one tracer in three phases:
      1. every thread builds rows (column, layer) of the tridiagonal system
-        into LDS -- all loads independent, fully coalesced;
+        into LDS -- all loads independent, fully coalesced; an interface has
+        one nonzero off-diagonal (the upstream side of its flux), stored
+        once with a flag, so a column takes three doubles and a byte per row;
      2. one lane per column runs the Thomas elimination on LDS;
      3. every thread writes the clipped solution back, coalesced.
    Each coefficient is formed with the reference's arithmetic. */
 #define GD_THREADS 256
-/* columns per block: 64 at 26 layers (53 KiB of LDS, three blocks per CU);
-   32 for deeper columns, so that two blocks still fit in LDS at 55 or 64
-   layers (56 / 64 KiB) -- the lanes of a 32-column group are half a wave */
+/* columns per block: 64 at 26 layers (41 KiB of LDS, three blocks per CU;
+   60 columns and four blocks measured no faster); 32 for deeper columns
+   (43 KiB at 55 layers: three blocks per CU where four-array rows allowed
+   two, 50 KiB at 64) -- the lanes of a 32-column group are half a wave */
 __host__ __device__ constexpr int gd_ncol(int lt) { return lt == 26 ? 64 : 32; }
 
 /* vertical fluxes of tracer k through interface jj of column i
@@ -1533,7 +1582,10 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     if (k < ncc/4) vsink = K.snow_v;
     else if (k < ncc/2) vsink = K.rain_v;
     else if (k < ncc) vsink = K.cloud_v;
-    double *cc = lds, *dd = lds + L*GD_NCOL, *ee = lds + 2*L*GD_NCOL, *rr = lds + 3*L*GD_NCOL;
+    /* one off-diagonal per interface (the flux's sign picks c or e), so three
+       arrays of doubles and one of flags */
+    double *ce = lds, *dd = lds + L*GD_NCOL, *rr = lds + 2*L*GD_NCOL;
+    unsigned char *fl = (unsigned char *)(lds + 3*L*GD_NCOL);
 
     /* phase 1: rows (column_solvers.c:466-615).  Each wave takes a run of
        layers of the block's columns (lane = column, so every load is
@@ -1628,13 +1680,12 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
                     if (k < ncc) r += -ew*vsink*dt*r_low*a_sfc/v;
                 } else r += ew*dt*(-fr_m + fr)/v;
                 /* off-diagonals of interface j (column_solvers.c:475-488);
-                   -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly */
-                double c = 0.0, e = 0.0;
-                if (j < L - 1) {
-                    if (fi >= 0.0) e = (-q)*fi;
-                    else c = cur.qlo*fi;
-                }
-                if (valid) { cc[idx] = c; dd[idx] = d; ee[idx] = e; rr[idx] = r; }
+                   -iw*dt/vol*fi is (-(iw*dt/vol))*fi exactly; the other one
+                   stays 0.0 */
+                double o = 0.0;
+                const bool up = fi >= 0.0;
+                if (j < L - 1) o = up ? (-q)*fi : cur.qlo*fi;
+                if (valid) { ce[idx] = o; fl[idx] = up; dd[idx] = d; rr[idx] = r; }
                 fi_m = fi;
                 fr_m = fr;
                 cur = nxt;
@@ -1645,7 +1696,7 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     /* phase 2: Thomas algorithm (column_solvers.c:641-667), one lane per column;
        the multipliers overwrite e, the solution overwrites r */
     if (threadIdx.x < GD_NCOL && i0 + (int)threadIdx.x < nsh)
-        thomas_lds<GD_NCOL, (LT > 0 ? LT : GAME_MAX_LAYERS)>(cc, dd, ee, rr, L, threadIdx.x);
+        thomas_lds_ce<GD_NCOL, (LT > 0 ? LT : GAME_MAX_LAYERS)>(ce, fl, dd, rr, L, threadIdx.x);
     __syncthreads();
     /* phase 3: clipped solution (column_solvers.c:616-637) */
     double *rho_n = sn.rho + (i64)k*G.s;
@@ -2381,7 +2432,8 @@ static void gen_densities_only(const DevGrid &G, const DevState &so, const DevSt
     }
     const int ncol = gd_ncol(G.L == 26 ? 26 : (G.L == 55 ? 55 : 0));
     dim3 g(((G.nsh + ncol - 1)/ncol)*(G.n_const - 1)), b(GD_THREADS);
-    size_t lds = (size_t)4*G.L*ncol*sizeof(double);   /* 53 KiB at 26 layers, 56 KiB at 55, 64 KiB at 64 */
+    /* 41 KiB at 26 layers, 43 KiB at 55, 50 KiB at 64 */
+    size_t lds = (size_t)3*G.L*ncol*sizeof(double) + (((size_t)G.L*ncol + 7) & ~(size_t)7);
     if (G.L == 26) LAUNCH_SHM("k_gen_densities", k_gen_densities<26>, g, b, lds, st, G, so, sn, td, K);
     else if (G.L == 55) LAUNCH_SHM("k_gen_densities", k_gen_densities<55>, g, b, lds, st, G, so, sn, td, K);
     else LAUNCH_SHM("k_gen_densities", k_gen_densities<0>, g, b, lds, st, G, so, sn, td, K);
