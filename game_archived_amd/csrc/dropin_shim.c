@@ -174,7 +174,7 @@ int manage_rkhevi(State *state_old, State *state_new, Grid *grid, Dualgrid *dual
                   Diagnostics *diagnostics, Forcings *forcings, Irreversible_quantities *irrev, Config *config,
                   double delta_t, double time_coordinate)
 {
-    (void)state_tendency; (void)diagnostics; (void)forcings; (void)irrev; (void)time_coordinate;
+    (void)time_coordinate;
     int rc = ensure_init(grid, dualgrid, config);
     if (rc) return rc;
     game_config_t c = config_copy(config);
@@ -184,6 +184,16 @@ int manage_rkhevi(State *state_old, State *state_new, Grid *grid, Dualgrid *dual
     if ((rc = game_gpu_set_state(GAME_STATE_NEW, &vn))) { fail(rc, "upload state_new"); return rc; }
     if ((rc = game_gpu_manage_rkhevi(delta_t, config->totally_first_step_bool))) { fail(rc, "manage_rkhevi"); return rc; }
     if ((rc = game_gpu_get_state(GAME_STATE_NEW, &vn))) { fail(rc, "download state_new"); return rc; }
+    /* the reference's step writes the tendency state, diagnostics, forcings
+       and irreversible quantities in place, and a driver may read them at
+       output time: they are copied back after every step unless the driver
+       asks to fetch them itself (GAME_DROPIN_DEFER_AUX, then
+       game_dropin_sync_aux) */
+    if (!getenv("GAME_DROPIN_DEFER_AUX")
+        && (rc = game_dropin_sync_aux(state_tendency, diagnostics, forcings, irrev))) {
+        fail(rc, "download of the carried fields");
+        return rc;
+    }
     return 0;
 }
 

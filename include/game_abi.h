@@ -222,8 +222,11 @@ typedef struct irreversible_quantities {
    (declared at src/time_stepping/time_stepping.h:6, called at src/coordinator.c:322).
    Reads state_old, updates state_new in place; forcings, the tendency state,
    diagnostics and irreversible quantities are carried on the device between
-   calls (host copies: game_dropin_sync_aux).  time_coordinate is unused
-   (radiation is outside this path).  Returns 0, or a game_status code. */
+   calls and copied into the caller's structs after every call, as the
+   reference writes them in place (with GAME_DROPIN_DEFER_AUX set they are
+   copied only by game_dropin_sync_aux).  The device copies are the ones the
+   next call reads.  time_coordinate is unused (radiation is outside this
+   path).  Returns 0, or a game_status code. */
 int manage_rkhevi(State *state_old, State *state_new, Grid *grid, Dualgrid *dualgrid, State *state_tendency,
                   Diagnostics *diagnostics, Forcings *forcings, Irreversible_quantities *irrev, Config *config,
                   double delta_t, double time_coordinate);

@@ -28,9 +28,14 @@ LIB = os.path.join(ROOT, "game_archived_amd", "lib")
 VP = C.c_void_p
 
 
-def test_dropin_driver_protocol_matches_reference():
+@pytest.mark.parametrize("aux", ["per_step", "deferred"])
+def test_dropin_driver_protocol_matches_reference(aux):
+    """aux per_step: the carried fields reach the host structs after every
+    step, as the reference writes them in place; deferred
+    (GAME_DROPIN_DEFER_AUX): they stay on the device until
+    game_dropin_sync_aux"""
     from game_archived_amd import _native as N
-    print("case: drop-in r5l26m1, res 5 moist gaussian, 3 steps dt 300", flush=True)
+    print(f"case: drop-in r5l26m1, res 5 moist gaussian, 3 steps dt 300, carried fields {aux}", flush=True)
     C.CDLL(os.path.join(LIB, "libgame_rkhevi.so"), mode=C.RTLD_GLOBAL)
     lib = C.CDLL(os.path.join(LIB, "libgame_dropin_r5l26m1.so"))
     lib.linear_combine_two_states.argtypes = [VP, VP, VP, C.c_double, C.c_double, VP]
@@ -48,6 +53,8 @@ def test_dropin_driver_protocol_matches_reference():
     cfg = R.config_struct(N.default_config())
     first = cfg.member("totally_first_step_bool", np.int32, 1)
     os.environ["GAME_GPU_NOABORT"] = "1"
+    if aux == "deferred":
+        os.environ["GAME_DROPIN_DEFER_AUX"] = "1"
     try:
         # the maintainer's added call after the grid is read (before coordinator.c:250)
         assert lib.game_dropin_init(g.ptr, dg.ptr, cfg.ptr) == 0
@@ -58,10 +65,13 @@ def test_dropin_driver_protocol_matches_reference():
             assert lib.manage_rkhevi(so.ptr, sn.ptr, g.ptr, dg.ptr, td.ptr, diag.ptr, forc.ptr, irrev.ptr, cfg.ptr,
                                      P.DT, step * P.DT) == 0
             first[0] = 0
-        assert lib.game_dropin_sync_aux(td.ptr, diag.ptr, forc.ptr, irrev.ptr) == 0
+        if aux == "deferred":
+            assert not np.array(forc.member("pot_vort_tend", np.float64, d.v)).any()   # not yet on the host
+            assert lib.game_dropin_sync_aux(td.ptr, diag.ptr, forc.ptr, irrev.ptr) == 0
     finally:
         lib.game_dropin_finalize()
         os.environ.pop("GAME_GPU_NOABORT", None)
+        os.environ.pop("GAME_DROPIN_DEFER_AUX", None)
 
     got = {k: np.array(v) for k, v in R.state_fields(sn, d).items()}
     for f in ("pgrad_acc_old", "pressure_gradient_acc_neg_nl", "pressure_gradient_acc_neg_l", "v_squared_grad",
