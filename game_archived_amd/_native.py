@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(HERE, "lib")
+# GAME_LIB_DIR: another directory of the same libraries (an instrumented build)
+LIBDIR = os.environ.get("GAME_LIB_DIR") or os.path.join(HERE, "lib")
 
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)

@@ -22,7 +22,7 @@ class Ctx(C.Structure):
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        path = os.environ.get("GAME_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built (make -C oracle)")
         _lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
