@@ -448,6 +448,7 @@ static void ver_waves_adjoint(const sadj::VwArgs &a, hipStream_t st)
     launch("k_vw_layer_fwd", sadj::VwLayerFwd{a}, G.s, st);
     launch("k_vw_iface_fwd", sadj::VwIfaceFwd{a}, ni, st);
     launch("k_vw_solve", sadj::VwSolve{a, 0}, G.nsh, st);
+    launch("k_vw_layer_new", sadj::VwLayerNew{a}, G.s, st);
     launch("k_vw_iface_sol_bar", sadj::VwIfaceSolBar{a}, ni, st);
     launch("k_vw_solve_t", sadj::VwSolve{a, 1}, G.nsh, st);
     launch("k_vw_iface_bar", sadj::VwIfaceBar{a}, ni, st);

@@ -65,7 +65,8 @@ GAME_HD inline void split_index(i64 t, int n, int &q, int &r)
 /* writers.  The workspace holds VW_N arrays of L doubles per column,   */
 /* array k of column c element j at (k*L + j)*ncol + c.                 */
 /* ------------------------------------------------------------------ */
-enum { VW_AL, VW_BE, VW_GA, VW_TIN, VW_C, VW_D, VW_E, VW_R, VW_X, VW_EP, VW_BX, VW_BR, VW_BTIN, VW_BRIO, VW_N };
+enum { VW_AL, VW_BE, VW_GA, VW_TIN, VW_C, VW_D, VW_E, VW_R, VW_X, VW_EP, VW_BX, VW_BR, VW_BTIN, VW_BRIO,
+       VW_RN, VW_RTN, VW_BRE, VW_BRTE, VW_N };
 
 struct VwArgs {
     DevGrid G;
@@ -217,6 +218,25 @@ struct VwIfaceFwd {
     }
 };
 
+/* the new state's rho and rt in layer j and their cotangents from the new
+   state's rho, rt, tp, ep and w, formed once for the kernels after the solve */
+struct VwLayerNew {
+    VwArgs a;
+    GAME_HD void operator()(i64 t) const
+    {
+        if (t >= a.G.s) return;
+        int j, c;
+        a.split(t, j, c);
+        double rn, rtn, bre, brte;
+        a.rnew(j, c, rn, rtn);
+        a.bnew(j, c, rn, rtn, bre, brte);
+        a.W(VW_RN, j, c) = rn;
+        a.W(VW_RTN, j, c) = rtn;
+        a.W(VW_BRE, j, c) = bre;
+        a.W(VW_BRTE, j, c) = brte;
+    }
+};
+
 /* Thomas algorithm of column_solvers.c:641-667: A x = r (bands c below,
    d, e above) or, with transpose, A^T x = r (e below, c above); the
    quotients go to VW_EP */
@@ -261,11 +281,8 @@ struct VwIfaceSolBar {
         a.split(t, j, c);
         const double dt = a.dt;
         const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c), wi = widx(G, j + 1, c);
-        double rn0, rtn0, rn1, rtn1, bre0, brte0, bre1, brte1;
-        a.rnew(j, c, rn0, rtn0);
-        a.rnew(j + 1, c, rn1, rtn1);
-        a.bnew(j, c, rn0, rtn0, bre0, brte0);
-        a.bnew(j + 1, c, rn1, rtn1, bre1, brte1);
+        const double bre0 = a.W(VW_BRE, j, c), brte0 = a.W(VW_BRTE, j, c);
+        const double bre1 = a.W(VW_BRE, j + 1, c), brte1 = a.W(VW_BRTE, j + 1, c);
         const double grt = brte0/G.vol[b] - brte1/G.vol[lo];
         const double grho = bre0/G.vol[b] - bre1/G.vol[lo];
         double bx = 2.0/(G.area[wi]*a.rio(j, c))*a.b_wind[wi];
@@ -316,11 +333,8 @@ struct VwIfaceBar {
         const double dt = a.dt, iw = a.iw;
         const i64 b = sidx(G, j, c), lo = sidx(G, j + 1, c), wi = widx(G, j + 1, c);
         /* the new state: theta_i sol in the rt flux, w_new */
-        double rn0, rtn0, rn1, rtn1, bre0, brte0, bre1, brte1;
-        a.rnew(j, c, rn0, rtn0);
-        a.rnew(j + 1, c, rn1, rtn1);
-        a.bnew(j, c, rn0, rtn0, bre0, brte0);
-        a.bnew(j + 1, c, rn1, rtn1, bre1, brte1);
+        const double rn0 = a.W(VW_RN, j, c), rn1 = a.W(VW_RN, j + 1, c);
+        const double brte0 = a.W(VW_BRTE, j, c), brte1 = a.W(VW_BRTE, j + 1, c);
         const double sol = a.sol(j, c), rio = a.rio(j, c), wo = a.wind_o[wi], bw = a.b_wind[wi];
         const double grt = brte0/G.vol[b] - brte1/G.vol[lo];
         const double din = 0.5*(rn0 + rn1);
@@ -386,9 +400,8 @@ struct VwLayerBar {
         const double ro = a.rho_o[b], rto = a.rt_o[b], vol = G.vol[b], exbg = G.exbg[b];
         const double tr = a.t_rho[b], tt = a.t_rt[b];
         const double al = a.W(VW_AL, j, c), be = a.W(VW_BE, j, c), ga = a.W(VW_GA, j, c);
-        double rn, rtn, bre, brte;
-        a.rnew(j, c, rn, rtn);
-        a.bnew(j, c, rn, rtn, bre, brte);
+        const double rtn = a.W(VW_RTN, j, c), brte = a.W(VW_BRTE, j, c);
+        double bre = a.W(VW_BRE, j, c);
         double bal = 0.0, bbe = 0.0, bga = vol*(rtn - rto)*a.b_ep[b], btpe = 0.0, bepe = 0.0;
         /* row j (the layer above interface j) */
         if (j < n) {
@@ -692,6 +705,8 @@ struct VFluxPvAdjRh {
 #pragma unroll
         for (int q = 0; q < VF_CH; ++q) acc[q] = 0.0;
         const int n = R.n[m];
+        /* four entries' gathers in flight at once; the sums keep entry order */
+#pragma unroll 4
         for (int k = 0; k < n; ++k) {
             const i64 km = (i64)k*nvh + m;
             const double w = R.w[km];
