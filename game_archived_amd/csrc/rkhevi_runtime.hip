@@ -891,6 +891,8 @@ int rt::build_adjoint_tables(void)
     if ((rc = upload_ints(&R.e, T.pvr_e)) || (rc = upload_ints(&R.f, T.pvr_f)) || (rc = upload_ints(&R.n, T.pvr_n))
         || (rc = upload(&R.w, T.pvr_w.data(), T.pvr_w.size())))
         return rc;
+    R.n_max = 0;
+    for (int c : T.pvr_n) R.n_max = c > R.n_max ? c : R.n_max;
     HIP_OK(hipStreamSynchronize(M->stream));
     M->adj_ready = 1;
     return 0;

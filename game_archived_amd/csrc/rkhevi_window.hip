@@ -404,7 +404,11 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
     /* TRSK flux: in the mass flux and in the PV */
     adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, st);
     const int vf_chunks = (G.L + VF_CH - 1)/VF_CH;
-    launch("k_vflux_pv_adj_rh", sadj::VFluxPvAdjRh{G, M->PR, flux, advb, w.pvb, vf_chunks}, (i64)vf_chunks*G.nvh, st);
+    if (M->PR.n_max <= 12)
+        launch("k_vflux_pv_adj_rh", sadj::VFluxPvAdjRhN<12>{G, M->PR, flux, advb, w.pvb, vf_chunks},
+               (i64)vf_chunks*G.nvh, st);
+    else
+        launch("k_vflux_pv_adj_rh", sadj::VFluxPvAdjRh{G, M->PR, flux, advb, w.pvb, vf_chunks}, (i64)vf_chunks*G.nvh, st);
     launch("k_vflux_pv_adj_tan", sadj::VFluxPvAdjTan{G, A, flux, advb, w.pvb}, (i64)(G.L + 1)*G.nvh, st);
     /* PV = (rel_vort + f)/density */
     launch("k_pv_div_adj", sadj::PvDivAdj{G, w.pvb, pv, rho, w.rvb, w.dvb}, G.curl, st);

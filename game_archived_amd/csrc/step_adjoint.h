@@ -28,6 +28,7 @@ struct PvReverse {
     const int *f;        /* PVR_MAX x NVH: flux edge */
     const double *w;     /* PVR_MAX x NVH: weight */
     const int *n;        /* NVH: the number of entries of each rhombus */
+    int n_max;           /* the largest of them (12 on the icosahedral grids) */
 };
 
 namespace sadj {
@@ -718,6 +719,49 @@ struct VFluxPvAdjRh {
 #pragma unroll
         for (int q = 0; q < VF_CH; ++q)
             if (l0 + q < l1) pvb[(i64)(l0 + q)*2*nvh + nvh + m] = acc[q];
+    }
+};
+
+/* the same rows with a fixed NE entries per rhombus (NE >= every count;
+   the padding entries have weight 0 and read edge 0, valid indices whose
+   products add a zero after the real terms): the entries stay in registers
+   while the thread walks its chunk, and every gather of a layer is issued
+   before the layer's sum, which keeps entry order */
+template <int NE>
+struct VFluxPvAdjRhN {
+    DevGrid G;
+    PvReverse R;
+    const double *F, *ob;                   /* mass flux (V), cotangent of the output (V) */
+    double *pvb;                            /* curl, rhombus rows written */
+    int chunks;                             /* layer chunks per rhombus */
+    GAME_HD void operator()(i64 t) const
+    {
+        const int nvh = G.nvh, L = G.L;
+        if (t >= (i64)chunks*nvh) return;
+        int ch, m;
+        split_index(t, nvh, ch, m);
+        const int l0 = (int)(((i64)L*ch)/chunks), l1 = (int)(((i64)L*(ch + 1))/chunks);
+        double w[NE];
+        int fe[NE], oe[NE];
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const i64 km = (i64)k*nvh + m;
+            w[k] = R.w[km];
+            fe[k] = R.f[km];
+            oe[k] = R.e[km];
+        }
+        for (int l = l0; l < l1; ++l) {
+            double fv[NE], ov[NE];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                fv[k] = F[eidx(G, l, fe[k])];
+                ov[k] = ob[eidx(G, l, oe[k])];
+            }
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < NE; ++k) acc += w[k]*fv[k]*ov[k];
+            pvb[(i64)l*2*nvh + nvh + m] = acc;
+        }
     }
 };
 
