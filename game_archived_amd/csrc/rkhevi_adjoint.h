@@ -27,16 +27,21 @@ struct AdjTables {
 #define ADJ_TR_MAX 10
 
 namespace adj {
-void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st);
+/* grad^T of gs*gb into fb (acc: added to fb) */
+void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st,
+                  double gs = 1.0, int acc = 0);
 void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st);
 void divv_h_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *Fb, hipStream_t st);
-void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st);
+/* acc: the edge cotangents are added to wb, its w points left as they are */
+void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st,
+                            int acc = 0);
 void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream_t st);
 void rhombus_density_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *rb, hipStream_t st);
 void vorticity_flux_adjoint(const DevGrid &G, const AdjTables &A, const double *pv, const double *ob, double *Fb,
                             hipStream_t st);
+/* acc: ab += scale*(the adjoint); otherwise ab = the adjoint */
 void inner_product_adjoint(const DevGrid &G, const AdjTables &A, const double *b, const double *ib, double *ab,
-                           hipStream_t st);
+                           hipStream_t st, double scale = 1.0, int acc = 0);
 void thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r, double *x,
             double *work, hipStream_t st);
 void thomas_adjoint(int n, int ncol, const double *c, const double *d, const double *e, const double *x,

@@ -37,8 +37,10 @@ __device__ __forceinline__ void adj_chunk(int n, int &l0, int &l1)
 /* ------------------------------------------------------------------ */
 /* grad^T: fbar(c, l) from gbar at the cell's edges and w points        */
 /* ------------------------------------------------------------------ */
-__global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const double *gb, double *fb)
+__global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const double *gb, double *fb, double gs,
+                                                   int acc)
 {
+    /* the input is gs*gb, formed as it is loaded; acc adds the result to fb */
     /* one thread per (cell, chunk of layers): layer l's edge cotangents feed
        the horizontal term of layer l and the terrain terms of the w points
        l (ipw slot 6) and l + 1 (slot 7); each w point's cotangent is formed
@@ -63,7 +65,7 @@ __global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             const i64 vi = nsh + (i64)l*vpl + ej[j];
-            g[j] = gb[vi];
+            g[j] = gs*gb[vi];
             sl[j] = (l >= top) ? G.slope[vi] : 0.0;
             /* g_e = (f_to - f_from)/d_e: the from-cell (sign +1) gets -1/d_e */
             if (j == 0) hs = 0.0;
@@ -83,7 +85,7 @@ __global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const
     };
     /* the cotangent of the vertical gradient at w point lev over its normal distance */
     auto wcot = [&](int lev, double s6, double s7_above) {
-        double r = gb[h + (i64)lev*vpl] + s6 + s7_above;
+        double r = gs*gb[h + (i64)lev*vpl] + s6 + s7_above;
         return r/G.nd[h + (i64)lev*vpl];
     };
     double hs, s6, s7, s7p = 0.0, unused_h, unused_s6;
@@ -96,7 +98,9 @@ __global__ void __launch_bounds__(ATPB) k_grad_adj(DevGrid G, AdjTables A, const
             layer(l + 1, hn, s6n, s7n);
             wn = wcot(l + 1, s6n, s7);
         }
-        fb[(i64)l*nsh + h] = hs + wn - wc;
+        const double v = hs + wn - wc;
+        const i64 o = (i64)l*nsh + h;
+        fb[o] = acc ? fb[o] + v : v;
         hs = hn; s7 = s7n; wc = wn;
     }
 }
@@ -211,13 +215,15 @@ __device__ __forceinline__ CurlAdjTerm curl_adj_term(const DevGrid &G, const dou
     return c;
 }
 
-__global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const double *zb, double *wb)
+__global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const double *zb, double *wb, int acc)
 {
+    /* acc adds the edge cotangents to wb and leaves its w points; otherwise
+       the w points are set to zero */
     const int t = blockIdx.x*ATPB + threadIdx.x;
     const int nsh = G.nsh, vpl = G.vpl, L = G.L;
     int l0, l1;
     adj_chunk(L, l0, l1);
-    if (t < nsh) {
+    if (!acc && t < nsh) {
         for (int l = l0; l < l1; ++l) wb[(i64)l*vpl + t] = 0.0;
         if (l1 == L) wb[(i64)L*vpl + t] = 0.0;
     }
@@ -241,7 +247,8 @@ __global__ void __launch_bounds__(ATPB) k_curl_adj(DevGrid G, AdjTables A, const
             r += cur.s[p];
             if (lp + 1 <= L - 1 && nxt.d[p] == -1) r += nxt.o[p];
         }
-        wb[nsh + (i64)lp*vpl + e] = r;
+        const i64 o = nsh + (i64)lp*vpl + e;
+        wb[o] = acc ? wb[o] + r : r;
         prev = cur;
         cur = nxt;
         if (lp + 2 <= L - 1 && lp + 1 < l1) nxt = curl_adj_term(G, zb, e, tri, sg, lp + 2);
@@ -368,8 +375,9 @@ __global__ void __launch_bounds__(ATPB) k_vflux_adj_w(DevGrid G, const double *p
 /* inner product^T in its first argument                                 */
 /* ------------------------------------------------------------------ */
 __global__ void __launch_bounds__(ATPB) k_ip_adj(DevGrid G, AdjTables A, const double *b, const double *ib,
-                                                 double *ab)
+                                                 double *ab, double scale, int acc)
 {
+    /* acc: ab += scale*result; otherwise ab = result */
     i64 t = (i64)blockIdx.x*ATPB + threadIdx.x;
     if (t >= G.v) return;
     const int nsh = G.nsh, vpl = G.vpl, L = G.L;
@@ -386,7 +394,7 @@ __global__ void __launch_bounds__(ATPB) k_ip_adj(DevGrid G, AdjTables A, const d
         if (l <= L - 1) s += G.ipw[8*((i64)l*nsh + h) + 6]*ib[(i64)l*nsh + h];
         r = b[t]*s;
     }
-    ab[t] = r;
+    ab[t] = acc ? ab[t] + scale*r : r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -454,8 +462,9 @@ __global__ void __launch_bounds__(ATPB) k_thomas_adj(int n, int ncol, const doub
 /* ------------------------------------------------------------------ */
 namespace adj {
 
-void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st)
-{ LAUNCH("k_grad_adj", k_grad_adj, AGRID2(G.nsh), dim3(ATPB), st, G, A, gb, fb); }
+void grad_adjoint(const DevGrid &G, const AdjTables &A, const double *gb, double *fb, hipStream_t st, double gs,
+                  int acc)
+{ LAUNCH("k_grad_adj", k_grad_adj, AGRID2(G.nsh), dim3(ATPB), st, G, A, gb, fb, gs, acc); }
 
 void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st)
 { LAUNCH("k_divv", k_divv, AGRID(G.s), dim3(ATPB), st, G, in, out); }
@@ -463,8 +472,9 @@ void divv_h(const DevGrid &G, const double *in, double *out, hipStream_t st)
 void divv_h_adjoint(const DevGrid &G, const AdjTables &A, const double *db, double *Fb, hipStream_t st)
 { LAUNCH("k_divv_adj", k_divv_adj, AGRID(G.v), dim3(ATPB), st, G, A, db, Fb); }
 
-void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st)
-{ LAUNCH("k_curl_adj", k_curl_adj, AGRID2(G.nvh > G.nsh ? G.nvh : G.nsh), dim3(ATPB), st, G, A, zb, wb); }
+void curl_triangles_adjoint(const DevGrid &G, const AdjTables &A, const double *zb, double *wb, hipStream_t st,
+                            int acc)
+{ LAUNCH("k_curl_adj", k_curl_adj, AGRID2(G.nvh > G.nsh ? G.nvh : G.nsh), dim3(ATPB), st, G, A, zb, wb, acc); }
 
 void rhombus_density(const DevGrid &G, const double *rho, double *out, hipStream_t st)
 { LAUNCH("k_rhombus", k_rhombus, AGRID((i64)G.L*G.nvh), dim3(ATPB), st, G, rho, out); }
@@ -480,8 +490,8 @@ void vorticity_flux_adjoint(const DevGrid &G, const AdjTables &A, const double *
 }
 
 void inner_product_adjoint(const DevGrid &G, const AdjTables &A, const double *b, const double *ib, double *ab,
-                           hipStream_t st)
-{ LAUNCH("k_ip_adj", k_ip_adj, AGRID(G.v), dim3(ATPB), st, G, A, b, ib, ab); }
+                           hipStream_t st, double scale, int acc)
+{ LAUNCH("k_ip_adj", k_ip_adj, AGRID(G.v), dim3(ATPB), st, G, A, b, ib, ab, scale, acc); }
 
 void thomas(int n, int ncol, const double *c, const double *d, const double *e, const double *r, double *x,
             double *work, hipStream_t st)

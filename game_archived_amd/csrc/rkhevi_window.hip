@@ -119,7 +119,7 @@ struct Window {
     Cot bS, bSn, bS1;
     double *bt0_rho, *bt0_rt, *bt0_wind, *bt1_rho, *bt1_rt, *bt1_wind;
     double *bP, *bPold, *bA, *bA1, *bAprev, *bU;
-    double *Fb, *Fthb, *cvb, *vsqb, *tmpS, *tmpV1, *tmpV2, *fluxb, *pvb, *rvb, *dvb, *zb, *ws, *ones, *vcb;
+    double *Fb, *Fthb, *cvb, *vsqb, *tmpV1, *tmpV2, *fluxb, *pvb, *rvb, *dvb, *zb, *ws, *ones, *vcb;
     DevWork AW;
     /* phase clock of the last gradient: events and the phase each closes */
     std::vector<hipEvent_t> ev;
@@ -238,7 +238,7 @@ int window_alloc(Window &w, int k, int n_ckpt)
     }
     if ((rc = alloc_state(w.seg_end)) || (rc = alloc_cot(w.bS)) || (rc = alloc_cot(w.bSn)) || (rc = alloc_cot(w.bS1)))
         return rc;
-    double **sv[] = {&w.bt0_rho, &w.bt0_rt, &w.bt1_rho, &w.bt1_rt, &w.vsqb, &w.tmpS, &w.ones};
+    double **sv[] = {&w.bt0_rho, &w.bt0_rt, &w.bt1_rho, &w.bt1_rt, &w.vsqb, &w.ones};
     for (double **p : sv) if ((rc = zeros(p, S))) return rc;
     double **vv[] = {&w.bt0_wind, &w.bt1_wind, &w.bP, &w.bPold, &w.bA, &w.bA1, &w.bAprev, &w.bU, &w.tmpV1, &w.tmpV2,
                      &w.fluxb};
@@ -397,10 +397,8 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
         pv = w.AW.pv;
     }
     /* kinetic energy: -0.5 grad(v.v) */
-    launch("k_axpy", sadj::Axpy{G.v, -0.5, advb, w.tmpV1, 0}, G.v, st);
-    adj::grad_adjoint(G, A, w.tmpV1, w.vsqb, st);
-    adj::inner_product_adjoint(G, A, wind, w.vsqb, w.tmpV2, st);
-    launch("k_axpy", sadj::Axpy{G.v, 2.0, w.tmpV2, b_wind, 1}, G.v, st);
+    adj::grad_adjoint(G, A, advb, w.vsqb, st, -0.5);
+    adj::inner_product_adjoint(G, A, wind, w.vsqb, b_wind, st, 2.0, 1);
     /* TRSK flux: in the mass flux and in the PV */
     adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, st);
     const int vf_chunks = (G.L + VF_CH - 1)/VF_CH;
@@ -415,8 +413,7 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
     launch("k_rhombus_dens_adj", sadj::RhombusDensAdj{G, A, w.dvb, b_rho}, G.s, st);
     launch("k_half_dens_adj", sadj::HalfDensAdj{G, w.dvb, b_rho}, G.s, st);
     launch("k_rhombus_vort_adj", sadj::RhombusVortAdj{G, w.rvb, w.zb}, (i64)G.L*G.nt, st);
-    adj::curl_triangles_adjoint(G, A, w.zb, w.tmpV2, st);
-    launch("k_axpy", sadj::Axpy{G.v, 1.0, w.tmpV2, b_wind, 1}, G.v, st);
+    adj::curl_triangles_adjoint(G, A, w.zb, b_wind, st, 1);
     launch("k_tan_vort_adj_e", sadj::TanVortAdjE{G, w.rvb, w.cvb, b_wind}, (i64)G.L*G.nvh, st);
     launch("k_tan_vort_adj_c", sadj::TanVortAdjC{G, A, w.rvb, w.cvb, b_wind}, (i64)(G.L + 1)*G.nsh, st);
     /* mass flux = rho * wind */
@@ -512,11 +509,11 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
             return rc;
     }
     /* ---- pressure gradient: P = mean(c_p (theta_bg + tp)) grad(ep) + mean(c_p tp) grad(exner_bg) ---- */
-    if ((rc = zero(w.tmpV1, V))) return rc;     /* advection_adjoint used it */
+    /* grad writes the edges and interior w points of tmpV1, which nothing
+       else writes: its top and surface w points keep the allocation's zero */
     hor::grad(G, T.s.ep, w.tmpV1, st);
     launch("k_stv_adj_v", sadj::StvAdjV{G, G.thbg, T.s.tp, w.bP, GC_C_D_P, 1, 0, w.tmpV2}, G.v, st);
-    adj::grad_adjoint(G, A, w.tmpV2, w.tmpS, st);
-    launch("k_axpy", sadj::Axpy{G.s, 1.0, w.tmpS, w.bSn.ep, 1}, G.s, st);
+    adj::grad_adjoint(G, A, w.tmpV2, w.bSn.ep, st, 1.0, 1);
     launch("k_stv_adj_s", sadj::StvAdjS{G, w.tmpV1, G.exbg_grad, w.bP, GC_C_D_P, 1, w.bSn.tp}, G.s, st);
     /* theta_v_pert is re-derived from rho and rhotheta_v by the driver's copy */
     launch("k_theta_adj", sadj::ThetaBoundaryAdj{G, T.s.rho, T.s.rt, w.bSn.rho, w.bSn.rt, w.bSn.tp}, G.s, st);
