@@ -2250,8 +2250,8 @@ static void pgc_v_t(const DevGrid &G, const DevState &s, const DevWorkT<TW> &W, 
 }
 
 template <typename TW>
-static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
-                        hipStream_t st)
+static void advection_pv_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
+                           hipStream_t st)
 {
     LAUNCH("k_mass_flux", k_mass_flux<TW>, GRID2(G.nvh > G.nsh ? G.nvh : G.nsh, G.split_edge), dim3(TPB), st, G,
            rho_gas, wind, W.flux);
@@ -2261,9 +2261,22 @@ static void advection_t(const DevGrid &G, const double *wind, const double *rho_
            (const TW *)W.pv, W.pvt);
     LAUNCH("k_vorticity_flux_w", k_vorticity_flux_w<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G,
            (const TW *)W.flux, (const TW *)W.pv, W.pvt);
+}
+
+template <typename TW>
+static void advection_kinetic_t(const DevGrid &G, const double *wind, const DevWorkT<TW> &W, hipStream_t st)
+{
     LAUNCH("k_inner_product", k_inner_product<TW>, GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, wind, wind, W.vsq);
     FLAT_DISPATCH("k_grad_e", k_grad_e, GRID2(G.nvh, G.split_edge), G, (const TW *)W.vsq, W.vsq_grad);
     LAUNCH("k_grad_w", k_grad_w<TW>, GRID1((i64)(G.L - 1)*G.nsh), dim3(TPB), st, G, (const TW *)W.vsq, W.vsq_grad);
+}
+
+template <typename TW>
+static void advection_t(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkT<TW> &W,
+                        hipStream_t st)
+{
+    advection_pv_t<TW>(G, wind, rho_gas, W, st);
+    advection_kinetic_t<TW>(G, wind, W, st);
 }
 
 template <typename TW>
@@ -2332,6 +2345,14 @@ void advection(const DevGrid &G, const double *wind, const double *rho_gas, cons
 { advection_t<double>(G, wind, rho_gas, W, st); }
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st)
 { advection_t<float>(G, wind, rho_gas, W, st); }
+void advection_pv(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st)
+{ advection_pv_t<double>(G, wind, rho_gas, W, st); }
+void advection_pv(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st)
+{ advection_pv_t<float>(G, wind, rho_gas, W, st); }
+void advection_kinetic(const DevGrid &G, const double *wind, const DevWork &W, hipStream_t st)
+{ advection_kinetic_t<double>(G, wind, W, st); }
+void advection_kinetic(const DevGrid &G, const double *wind, const DevWorkF &W, hipStream_t st)
+{ advection_kinetic_t<float>(G, wind, W, st); }
 void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
                  double new_w, double itw, double dt, hipStream_t st)
 { vector_tend_t<double>(G, W, tend_w, w_old, w_new, new_w, itw, dt, st); }

@@ -19,6 +19,12 @@ void pgc_v(const DevGrid &G, const DevState &s, const DevWork &W, hipStream_t st
 void pgc_v(const DevGrid &G, const DevState &s, const DevWorkF &W, hipStream_t st);
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st);
 void advection(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st);
+/* the two independent parts of advection: mass flux, PV and the TRSK flux
+   (pot_vort_tend); kinetic energy and its gradient (v_squared_grad) */
+void advection_pv(const DevGrid &G, const double *wind, const double *rho_gas, const DevWork &W, hipStream_t st);
+void advection_pv(const DevGrid &G, const double *wind, const double *rho_gas, const DevWorkF &W, hipStream_t st);
+void advection_kinetic(const DevGrid &G, const double *wind, const DevWork &W, hipStream_t st);
+void advection_kinetic(const DevGrid &G, const double *wind, const DevWorkF &W, hipStream_t st);
 void vector_tend(const DevGrid &G, const DevWork &W, double *tend_w, const double *w_old, double *w_new,
                  double new_w, double itw, double dt, hipStream_t st);
 void vector_tend(const DevGrid &G, const DevWorkF &W, float *tend_w, const double *w_old, double *w_new,

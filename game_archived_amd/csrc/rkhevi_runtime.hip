@@ -138,6 +138,10 @@ int game_gpu_finalize(void)
     for (void *p : M->allocs) (void)hipFree(p);
     if (M->h_partials) (void)hipHostFree(M->h_partials);
     for (auto e : M->timer.pool) (void)hipEventDestroy(e);
+    if (M->side) (void)hipStreamSynchronize(M->side);
+    if (M->ev_fork) (void)hipEventDestroy(M->ev_fork);
+    if (M->ev_join) (void)hipEventDestroy(M->ev_join);
+    if (M->side) (void)hipStreamDestroy(M->side);
     if (M->stream) (void)hipStreamDestroy(M->stream);
     g_kernel_timer = nullptr;
     delete M;
@@ -188,6 +192,10 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
     M->cfg = *config;
     M->precision = precision;
     HIP_OK(hipStreamCreate(&M->stream));
+    HIP_OK(hipStreamCreate(&M->side));
+    HIP_OK(hipEventCreateWithFlags(&M->ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&M->ev_join, hipEventDisableTiming));
+    M->concurrent = getenv("GAME_GPU_ONE_STREAM") ? 0 : 1;
     DevGrid &G = M->G;
     G.nsh = d.nsh; G.nvh = d.nvh; G.nt = d.nt; G.vpl = d.vpl; G.dvpl = d.dvpl; G.L = d.n_layers;
     G.n_oro = g->no_of_oro_layers; G.n_const = d.n_const; G.n_cond = d.n_cond; G.moist = d.moisture_on;
@@ -489,9 +497,24 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
         DevState sns = snr;                                 /* ... with the horizontal wind just updated */
         sns.wind = sn.wind;
-        if (rk == 0) hor::pgrad(G, snr, W, first, M->cfg.impl_thermo_weight, st);
-        else hor::pgc_v(G, snr, W, st);
-        if (rk == 1 || first) hor::advection(G, snr.wind, snr.rho + (i64)G.n_cond*G.s, W, st);
+        const double *rho_gas = snr.rho + (i64)G.n_cond*G.s;
+        if ((rk == 1 || first) && M->concurrent) {
+            /* the pressure gradient and the kinetic energy on the side
+               stream, the PV and TRSK flux on this one: they read the same
+               state and write different arrays */
+            HIP_OK(hipEventRecord(M->ev_fork, st));
+            HIP_OK(hipStreamWaitEvent(M->side, M->ev_fork, 0));
+            if (rk == 0) hor::pgrad(G, snr, W, first, M->cfg.impl_thermo_weight, M->side);
+            else hor::pgc_v(G, snr, W, M->side);
+            hor::advection_kinetic(G, snr.wind, W, M->side);
+            hor::advection_pv(G, snr.wind, rho_gas, W, st);
+            HIP_OK(hipEventRecord(M->ev_join, M->side));
+            HIP_OK(hipStreamWaitEvent(st, M->ev_join, 0));
+        } else {
+            if (rk == 0) hor::pgrad(G, snr, W, first, M->cfg.impl_thermo_weight, st);
+            else hor::pgc_v(G, snr, W, st);
+            if (rk == 1 || first) hor::advection(G, snr.wind, rho_gas, W, st);
+        }
         if constexpr (std::is_same<WK, DevWork>::value)
             if (tape && rk == 1) {
                 HIP_OK(hipMemcpyAsync(tape->flux, W.flux, (size_t)M->d.v*sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -501,7 +524,18 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape
         hor::scalar_tend(G, sns, td, W, rk, st);
         if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
         else col::ver_waves(G, so, snr, sn, td, (const float *)td_wind, M->K, rk, st);
-        if (G.n_const > 1) col::gen_densities(G, so, sn, td, W.temp, W.cpc, W.sed_heat, M->K, rk, st);
+        if (G.n_const > 1) {
+            if (M->concurrent && rk == 0 && !tape) {
+                /* the first substep's tracer solve (and sediment heat) on the
+                   side stream: it writes only the tracer densities and the
+                   sediment heat, which the second substep's PV chain does not
+                   read; its pressure-gradient kernel, which reads all the
+                   densities, follows on the side stream */
+                HIP_OK(hipEventRecord(M->ev_fork, st));
+                HIP_OK(hipStreamWaitEvent(M->side, M->ev_fork, 0));
+                col::gen_densities(G, so, sn, td, W.temp, W.cpc, W.sed_heat, M->K, rk, M->side);
+            } else col::gen_densities(G, so, sn, td, W.temp, W.cpc, W.sed_heat, M->K, rk, st);
+        }
         if (tape && rk == 0) {
             if ((rc = copy_state_d2d(tape->s1, sn, st))) return rc;
             if ((rc = copy_tendency(tape->td0_rho, tape->td0_rt, tape->td0_wind, td, st))) return rc;

@@ -98,6 +98,12 @@ struct Model {
     StepConsts K;
     double idv_dt = -1.0;   /* the dt G.idv holds quotients for */
     hipStream_t stream = nullptr;
+    /* a second stream (default priority) for the branches of a step that do
+       not depend on each other, forked from and joined back into `stream`
+       with the two events; concurrent = 0 runs everything on `stream` */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int concurrent = 1;
     std::vector<void *> allocs;
     long long bytes = 0;
     double *partials = nullptr, *h_partials = nullptr;
