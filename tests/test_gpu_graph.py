@@ -1,7 +1,9 @@
 """Steady steps replayed from HIP graphs against the same steps launched
 eagerly (GAME_GPU_NO_GRAPH): bit-identical states after every replay, with
 the program's own work (state downloads, integrals, a changed dt that
-rebuilds the graphs) and synchronisations between the replays."""
+rebuilds the graphs) and synchronisations between the replays.  The same for
+the step's independent branches on two streams against one stream
+(GAME_GPU_ONE_STREAM)."""
 import os
 
 import numpy as np
@@ -17,16 +19,19 @@ FIELDS = ("rho", "rhotheta_v", "theta_v_pert", "exner_pert", "wind")
 CALLS = [(3, 300.0), (1, 300.0), (2, 300.0), (2, 250.0), (3, 300.0)]
 
 
-def trajectory(grid, s0, precision, graphs):
+def trajectory(grid, s0, precision, graphs, one_stream=False):
     from game_archived_amd.rkhevi import DeviceModel
-    if graphs:
-        os.environ.pop("GAME_GPU_NO_GRAPH", None)
-    else:
-        os.environ["GAME_GPU_NO_GRAPH"] = "1"
+    env = {"GAME_GPU_NO_GRAPH": None if graphs else "1", "GAME_GPU_ONE_STREAM": "1" if one_stream else None}
+    for k, v in env.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
     try:
         dm = DeviceModel(grid, precision=precision)
     finally:
-        os.environ.pop("GAME_GPU_NO_GRAPH", None)
+        for k in env:
+            os.environ.pop(k, None)
     out = []
     try:
         dm.start(s0)
@@ -54,3 +59,21 @@ def test_graph_replays_match_eager_steps(precision_name):
             assert np.array_equal(fe[f], fg[f]), (k, f)
             assert np.isfinite(fg[f]).all(), (k, f)
         assert np.array_equal(ie, ig), k
+
+
+@pytest.mark.parametrize("precision_name", ["fp64", "mixed"])
+def test_two_streams_match_one_stream(precision_name):
+    """the branches on the second stream (kinetic energy, pressure gradient,
+    the first substep's tracer solve) change no bit against one stream, the
+    two-stream steps replayed from graphs and the one-stream steps eager"""
+    from game_archived_amd.rkhevi import FP64, MIXED
+    prec = {"fp64": FP64, "mixed": MIXED}[precision_name]
+    print(f"case: two streams (graphs) vs one stream (eager), res 4 moist gaussian, {precision_name}", flush=True)
+    g = get_grid(4, "gaussian", moist=1)
+    s0 = oracle_state(g, moist=True)
+    one = trajectory(g, s0, prec, graphs=False, one_stream=True)
+    two = trajectory(g, s0, prec, graphs=True)
+    for k, ((fo, io), (ft, it)) in enumerate(zip(one, two)):
+        for f in FIELDS:
+            assert np.array_equal(fo[f], ft[f]), (k, f)
+        assert np.array_equal(io, it), k
