@@ -493,12 +493,14 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape
         col::temperature(G, so, W.temp, G.n_cond > 0 ? W.cpc : nullptr, W.gas_frac, st);
         if (G.moist) col::phase_trans(G, so, W.temp, W.cpc, W.ptr, W.phr, dt, st);
     }
+    /* per-kernel events time each kernel alone: with them on, one stream */
+    const bool conc = M->concurrent && !g_kernel_timer;
     for (int rk = 0; rk < 2; ++rk) {
         const DevState &snr = rk == 0 ? so : sn;           /* state_new before this substep writes it */
         DevState sns = snr;                                 /* ... with the horizontal wind just updated */
         sns.wind = sn.wind;
         const double *rho_gas = snr.rho + (i64)G.n_cond*G.s;
-        if ((rk == 1 || first) && M->concurrent) {
+        if ((rk == 1 || first) && conc) {
             /* the pressure gradient and the kinetic energy on the side
                stream, the PV and TRSK flux on this one: they read the same
                state and write different arrays */
@@ -525,7 +527,7 @@ static int step_t(const WK &W, TW *td_wind, double dt, int first, const StepTape
         if (sizeof(TW) == sizeof(double)) col::ver_waves(G, so, snr, sn, td, M->K, rk, st);
         else col::ver_waves(G, so, snr, sn, td, (const float *)td_wind, M->K, rk, st);
         if (G.n_const > 1) {
-            if (M->concurrent && rk == 0 && !tape) {
+            if (conc && rk == 0 && !tape) {
                 /* the first substep's tracer solve (and sediment heat) on the
                    side stream: it writes only the tracer densities and the
                    sediment heat, which the second substep's PV chain does not
