@@ -396,9 +396,16 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
         flux = w.AW.flux;
         pv = w.AW.pv;
     }
-    /* kinetic energy: -0.5 grad(v.v) */
-    adj::grad_adjoint(G, A, advb, w.vsqb, st, -0.5);
-    adj::inner_product_adjoint(G, A, wind, w.vsqb, b_wind, st, 2.0, 1);
+    /* kinetic energy: -0.5 grad(v.v).  With two streams it runs on the side
+       stream beside the TRSK and PV reverse below, which does not touch
+       b_wind until the curl's adjoint, where the two join */
+    const hipStream_t side = (M->concurrent && !g_kernel_timer) ? M->side : nullptr;
+    if (side) {
+        HIP_OK(hipEventRecord(M->ev_fork, st));
+        HIP_OK(hipStreamWaitEvent(side, M->ev_fork, 0));
+    }
+    adj::grad_adjoint(G, A, advb, w.vsqb, side ? side : st, -0.5);
+    adj::inner_product_adjoint(G, A, wind, w.vsqb, b_wind, side ? side : st, 2.0, 1);
     /* TRSK flux: in the mass flux and in the PV */
     adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, st);
     const int vf_chunks = (G.L + VF_CH - 1)/VF_CH;
@@ -413,6 +420,10 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
     launch("k_rhombus_dens_adj", sadj::RhombusDensAdj{G, A, w.dvb, b_rho}, G.s, st);
     launch("k_half_dens_adj", sadj::HalfDensAdj{G, w.dvb, b_rho}, G.s, st);
     launch("k_rhombus_vort_adj", sadj::RhombusVortAdj{G, w.rvb, w.zb}, (i64)G.L*G.nt, st);
+    if (side) {
+        HIP_OK(hipEventRecord(M->ev_join, side));
+        HIP_OK(hipStreamWaitEvent(st, M->ev_join, 0));
+    }
     adj::curl_triangles_adjoint(G, A, w.zb, b_wind, st, 1);
     launch("k_tan_vort_adj_e", sadj::TanVortAdjE{G, w.rvb, w.cvb, b_wind}, (i64)G.L*G.nvh, st);
     launch("k_tan_vort_adj_c", sadj::TanVortAdjC{G, A, w.rvb, w.cvb, b_wind}, (i64)(G.L + 1)*G.nsh, st);
@@ -475,6 +486,18 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
         zl.add(w.bPold, V);
         zl.run(st);
     }
+    /* the gradient of the step's exner_pert, which the pressure gradient's
+       reverse below needs, depends on the tape alone: with two streams it is
+       formed on the side stream now (joined by the advection reverse's join
+       or at the end).  It writes the edges and interior w points of tmpV1,
+       which nothing else writes: its top and surface w points keep the
+       allocation's zero */
+    const hipStream_t side = (M->concurrent && !g_kernel_timer) ? M->side : nullptr;
+    if (side) {
+        HIP_OK(hipEventRecord(M->ev_fork, st));
+        HIP_OK(hipStreamWaitEvent(side, M->ev_fork, 0));
+        hor::grad(G, T.s.ep, w.tmpV1, side);
+    }
     /* ---- second substep ---- */
     {
         sadj::VwArgs a{G, 1, dt, iw, T.s.rho, T.s.rt, T.s.tp, T.s.ep, T.s.wind, T.s1.rho, T.s1.rt, T.s1.ep,
@@ -509,9 +532,10 @@ int step_adjoint(const StepTape &T, const DevState &next, double dt, int first)
             return rc;
     }
     /* ---- pressure gradient: P = mean(c_p (theta_bg + tp)) grad(ep) + mean(c_p tp) grad(exner_bg) ---- */
-    /* grad writes the edges and interior w points of tmpV1, which nothing
-       else writes: its top and surface w points keep the allocation's zero */
-    hor::grad(G, T.s.ep, w.tmpV1, st);
+    if (side) {
+        HIP_OK(hipEventRecord(M->ev_join, side));
+        HIP_OK(hipStreamWaitEvent(st, M->ev_join, 0));
+    } else hor::grad(G, T.s.ep, w.tmpV1, st);
     launch("k_stv_adj_v", sadj::StvAdjV{G, G.thbg, T.s.tp, w.bP, GC_C_D_P, 1, 0, w.tmpV2}, G.v, st);
     adj::grad_adjoint(G, A, w.tmpV2, w.bSn.ep, st, 1.0, 1);
     launch("k_stv_adj_s", sadj::StvAdjS{G, w.tmpV1, G.exbg_grad, w.bP, GC_C_D_P, 1, w.bSn.tp}, G.s, st);
