@@ -3,7 +3,9 @@ hand-written adjoints (verification harness only: the library itself has
 no torch types; these pass device pointers through the C ABI).
 
 Each Function's forward calls game_gpu_op_<name>, its backward calls
-game_gpu_op_<name>_adjoint.  The library runs on its own stream and returns
+game_gpu_op_<name>_adjoint.  make_window wraps a whole window of steps the
+same way: forward game_gpu_window_forward, backward game_gpu_window_vjp
+(the hand-written adjoint of every kernel of the step, with checkpoints).  The library runs on its own stream and returns
 after it has finished, so the wrappers synchronise torch's stream before
 handing it device memory.
 """
@@ -107,3 +109,28 @@ def make_thomas(ops, n, ncol):
             return cb, db, eb, rb
 
     return Thomas.apply
+
+
+def make_window(win, n_steps, dt, checkpoint_every):
+    """(rho0, rhotheta_v0, wind0) -> (rho, rhotheta_v, exner_pert, wind)
+    after n_steps dry steps; `win` has forward(n, dt, x) and
+    vjp(n, dt, k, x, bars) over the C ABI (tests/test_gpu_window.py)"""
+
+    class Window(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rho0, rt0, wind0):
+            x = [rho0.contiguous(), rt0.contiguous(), wind0.contiguous()]
+            torch.cuda.synchronize()
+            out = win.forward(n_steps, dt, x)
+            ctx.save_for_backward(*x)
+            return tuple(out)
+
+        @staticmethod
+        def backward(ctx, b_rho, b_rt, b_ex, b_wind):
+            x = list(ctx.saved_tensors)
+            bars = [b.contiguous() if b is not None else torch.zeros_like(o)
+                    for b, o in zip((b_rho, b_rt, b_ex, b_wind), (x[0], x[1], x[0], x[2]))]
+            torch.cuda.synchronize()
+            return tuple(win.vjp(n_steps, dt, checkpoint_every, x, bars))
+
+    return Window.apply

@@ -12,6 +12,8 @@ game_gpu_window_*), against:
 * the tangent-linear dot-product test: <M'(x) dx, y> from central
   differences of the forward against <dx, M'(x)^T y> from the adjoint, for
   random cotangents y of every output field;
+* the window wrapped in a torch.autograd.Function (tests/adjoint_harness.py):
+  gradients from torch.autograd.grad through it against central differences;
 * the scale window: res 8 x 55 dry and flat, 12 steps at the automatic time
   step (45.04 s) checkpointed every 4, against central differences in 4
   directions, with the device memory in use reported.
@@ -212,6 +214,52 @@ def test_tangent_linear_dot_product():
             ad = sum(float(torch.sum(a * b)) for a, b in zip(xb, dxd))
             err = abs(tl - ad) / abs(ad)
             print(f"direction {i}: <M' dx, y> {tl:.12e} <dx, M'^T y> {ad:.12e} rel {err:.2e}", flush=True)
+            assert err <= 1e-6, (i, err)
+    finally:
+        W.close()
+
+
+def test_window_autograd_function_matches_finite_differences():
+    """the window as a torch.autograd.Function: a scalar of all four output
+    fields (weighted sums and a square), differentiated by torch.autograd.grad
+    through the Function's backward (game_gpu_window_vjp), against central
+    differences (eps = 1e-6) of the same scalar in 4 seeded directions,
+    after 6 steps at res 4 over the mountain checkpointed every 4"""
+    import torch
+    from adjoint_harness import make_window
+    print("case: window autograd.Function vs central differences, res 4 x 26 gaussian, 6 steps, checkpoint 4",
+          flush=True)
+    g = get_grid(4, "gaussian")
+    d = g.dims
+    s0 = oracle_state(g)
+    dt, n, k = 300.0, 6, 4
+    W = Window(g)
+    try:
+        x0 = [W.dev(a) for a in controls(s0)]
+        rng = np.random.default_rng(23)
+        outs0 = W.forward(n, dt, x0)
+        scale = [float(torch.sqrt(torch.mean(o ** 2))) for o in outs0]
+        ws = [W.dev(rng.standard_normal(o.numel()) / max(sc, 1e-30)) for o, sc in zip(outs0, scale)]
+        wl = (np.arange(d.n_levels)[:, None] * d.vpl + np.arange(d.nsh)[None, :])
+        ws[3][torch.as_tensor(np.concatenate([wl[0], wl[-1]])).cuda()] = 0.0   # top and surface w: pass-through
+        target = outs0[0] * 1.001
+
+        def scalar(outs):
+            return sum(torch.sum(o * w) for o, w in zip(outs, ws)) + 0.5 * torch.sum((outs[0] - target) ** 2)
+
+        window = make_window(W, n, dt, k)
+        xs = [a.clone().requires_grad_(True) for a in x0]
+        J = scalar(window(*xs))
+        grads = torch.autograd.grad(J, xs)
+        eps = 1e-6
+        for i, dx in enumerate(directions(g, controls(s0), 4, 29)):
+            dxd = [W.dev(a) for a in dx]
+            jp = float(scalar(W.forward(n, dt, [a + eps * b for a, b in zip(x0, dxd)])))
+            jm = float(scalar(W.forward(n, dt, [a - eps * b for a, b in zip(x0, dxd)])))
+            fd = (jp - jm) / (2 * eps)
+            ad = sum(float(torch.sum(a * b)) for a, b in zip(grads, dxd))
+            err = abs(fd - ad) / abs(ad)
+            print(f"direction {i}: fd {fd:.12e} autograd {ad:.12e} rel {err:.2e}", flush=True)
             assert err <= 1e-6, (i, err)
     finally:
         W.close()
