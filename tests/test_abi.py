@@ -80,6 +80,27 @@ def test_failed_init_leaves_no_model():
     assert lib.game_gpu_run_steps(1, 60.0, 1) == 3                                    # GAME_ERR_NOT_INIT
     assert lib.game_gpu_manage_rkhevi(60.0, 1) == 3
     assert lib.game_gpu_integrals(1, N.ptr(out)) == 3
+    # every other entry point that needs a model, with null buffers: none may
+    # reach them without one
+    dt = C.c_double(60.0)
+    calls = {
+        "game_gpu_set_state": (1, None), "game_gpu_get_state": (1, None), "game_gpu_get_forcings": (None,),
+        "game_gpu_get_diagnostics": (None,), "game_gpu_get_irrev": (None,), "game_gpu_set_config": (None,),
+        "game_gpu_linear_combine_two_states": (0, 1, 1, C.c_double(0.5), C.c_double(0.5)),
+        "game_gpu_inner_product": (None, None, None), "game_gpu_temperature_diagnostics": (1,),
+        "game_gpu_write_integrals": (1, C.c_double(0.0), b"."),
+        "game_gpu_run_steps_write_integrals": (1, dt, 1, C.c_double(0.0), b"."),
+        "game_gpu_init_balance_terms": (None, None, None, None), "game_gpu_enable_timing": (1,),
+        "game_gpu_reset_timing": (), "game_gpu_synchronize": (), "game_gpu_op_grad": (None, None),
+        "game_gpu_op_grad_adjoint": (None, None), "game_gpu_op_thomas": (2, 2, None, None, None, None, None, None),
+        "game_gpu_window_forward": (1, dt, None, None, None, None, None, None, None),
+        "game_gpu_window_vjp": (1, dt, 1, None, None, None, None, None, None, None, None, None, None),
+        "game_gpu_window_gradient": (1, dt, 1, None, None, None, None, None, None, None, None),
+        "game_gpu_window_last_timing": (None,),
+    }
+    for name, args in calls.items():
+        assert getattr(lib, name)(*args) == 3, name                                  # GAME_ERR_NOT_INIT
+    assert lib.game_gpu_kernel_count() == 0
     assert lib.game_gpu_bytes_allocated() == 0
     assert lib.game_gpu_finalize() == 0
 
