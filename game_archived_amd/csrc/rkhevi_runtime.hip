@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 #include <type_traits>
+#include <initializer_list>
 
 #include "../../include/game_rkhevi.h"
 #include "rkhevi_dev.h"
@@ -941,6 +942,20 @@ static int op_done(void)
     return 0;
 }
 
+/* every buffer argument of an operator entry point must be given: a null one
+   is an argument error, not a launch on address zero */
+static bool all_given(std::initializer_list<const void *> ps)
+{
+    for (const void *p : ps)
+        if (!p) return false;
+    return true;
+}
+#define NEED(...)                                                                \
+    if (!all_given({__VA_ARGS__})) {                                             \
+        snprintf(g_err, sizeof g_err, "%s: a buffer argument is null", __func__); \
+        return GAME_ERR_ARG;                                                     \
+    }
+
 #define OP_PROLOGUE(NEED_TABLES)                                               \
     if (!M) return GAME_ERR_NOT_INIT;                                          \
     if (M->precision != GAME_FP64) {                                           \
@@ -954,6 +969,7 @@ extern "C" {
 int game_gpu_op_grad(const double *f, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(f, out)
     HIP_OK(hipMemsetAsync(out, 0, (size_t)M->d.v*sizeof(double), M->stream));
     hor::grad(M->G, f, out, M->stream);
     return op_done();
@@ -962,6 +978,7 @@ int game_gpu_op_grad(const double *f, double *out)
 int game_gpu_op_grad_adjoint(const double *gbar, double *fbar)
 {
     OP_PROLOGUE(1)
+    NEED(gbar, fbar)
     adj::grad_adjoint(M->G, M->A, gbar, fbar, M->stream);
     return op_done();
 }
@@ -969,6 +986,7 @@ int game_gpu_op_grad_adjoint(const double *gbar, double *fbar)
 int game_gpu_op_divv_h(const double *in, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(in, out)
     adj::divv_h(M->G, in, out, M->stream);
     return op_done();
 }
@@ -976,6 +994,7 @@ int game_gpu_op_divv_h(const double *in, double *out)
 int game_gpu_op_divv_h_adjoint(const double *dbar, double *inbar)
 {
     OP_PROLOGUE(1)
+    NEED(dbar, inbar)
     adj::divv_h_adjoint(M->G, M->A, dbar, inbar, M->stream);
     return op_done();
 }
@@ -983,6 +1002,7 @@ int game_gpu_op_divv_h_adjoint(const double *dbar, double *inbar)
 int game_gpu_op_curl_triangles(const double *wind, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(wind, out)
     hor::curl_triangles(M->G, wind, out, M->stream);
     return op_done();
 }
@@ -990,6 +1010,7 @@ int game_gpu_op_curl_triangles(const double *wind, double *out)
 int game_gpu_op_curl_triangles_adjoint(const double *zbar, double *windbar)
 {
     OP_PROLOGUE(1)
+    NEED(zbar, windbar)
     adj::curl_triangles_adjoint(M->G, M->A, zbar, windbar, M->stream);
     return op_done();
 }
@@ -997,6 +1018,7 @@ int game_gpu_op_curl_triangles_adjoint(const double *zbar, double *windbar)
 int game_gpu_op_rhombus_density(const double *rho, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(rho, out)
     adj::rhombus_density(M->G, rho, out, M->stream);
     return op_done();
 }
@@ -1004,6 +1026,7 @@ int game_gpu_op_rhombus_density(const double *rho, double *out)
 int game_gpu_op_rhombus_density_adjoint(const double *dbar, double *rhobar)
 {
     OP_PROLOGUE(1)
+    NEED(dbar, rhobar)
     adj::rhombus_density_adjoint(M->G, M->A, dbar, rhobar, M->stream);
     return op_done();
 }
@@ -1011,6 +1034,7 @@ int game_gpu_op_rhombus_density_adjoint(const double *dbar, double *rhobar)
 int game_gpu_op_vorticity_flux(const double *flux, const double *pot_vort, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(flux, pot_vort, out)
     hor::vorticity_flux(M->G, flux, pot_vort, out, M->stream);
     return op_done();
 }
@@ -1018,6 +1042,7 @@ int game_gpu_op_vorticity_flux(const double *flux, const double *pot_vort, doubl
 int game_gpu_op_vorticity_flux_adjoint(const double *pot_vort, const double *outbar, double *fluxbar)
 {
     OP_PROLOGUE(1)
+    NEED(pot_vort, outbar, fluxbar)
     adj::vorticity_flux_adjoint(M->G, M->A, pot_vort, outbar, fluxbar, M->stream);
     return op_done();
 }
@@ -1025,6 +1050,7 @@ int game_gpu_op_vorticity_flux_adjoint(const double *pot_vort, const double *out
 int game_gpu_op_inner_product(const double *a, const double *b, double *out)
 {
     OP_PROLOGUE(0)
+    NEED(a, b, out)
     hor::inner_product(M->G, a, b, out, M->stream);
     return op_done();
 }
@@ -1032,6 +1058,7 @@ int game_gpu_op_inner_product(const double *a, const double *b, double *out)
 int game_gpu_op_inner_product_adjoint(const double *b, const double *outbar, double *abar)
 {
     OP_PROLOGUE(1)
+    NEED(b, outbar, abar)
     adj::inner_product_adjoint(M->G, M->A, b, outbar, abar, M->stream);
     return op_done();
 }
@@ -1041,6 +1068,7 @@ int game_gpu_op_thomas(int n, int ncol, const double *c, const double *d, const 
 {
     if (!M) return GAME_ERR_NOT_INIT;
     if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    NEED(c, d, e, r, x, work)
     adj::thomas(n, ncol, c, d, e, r, x, work, M->stream);
     return op_done();
 }
@@ -1050,6 +1078,7 @@ int game_gpu_op_thomas_noalloc(int n, int ncol, const double *c, const double *d
 {
     if (!M) return GAME_ERR_NOT_INIT;
     if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    NEED(c, d, e, r, x)
     if ((long long)n*(long long)ncol > THOMAS_SCRATCH_DOUBLES) {
         snprintf(g_err, sizeof g_err, "game_gpu_op_thomas_noalloc: %d systems of %d rows need %lld doubles of "
                  "scratch, more than its %lld", ncol, n, (long long)n*(long long)ncol, (long long)THOMAS_SCRATCH_DOUBLES);
@@ -1067,6 +1096,7 @@ int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d
 {
     if (!M) return GAME_ERR_NOT_INIT;
     if (n < 1 || ncol < 1) { snprintf(g_err, sizeof g_err, "bad system size"); return GAME_ERR_ARG; }
+    NEED(c, d, e, x, xbar, rbar, cbar, dbar, ebar, work)
     adj::thomas_adjoint(n, ncol, c, d, e, x, xbar, rbar, cbar, dbar, ebar, work, M->stream);
     return op_done();
 }

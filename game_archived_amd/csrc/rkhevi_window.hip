@@ -621,6 +621,7 @@ int game_gpu_window_forward(int n_steps, double dt, const double *rho0, const do
     int rc = check_window_model();
     if (rc) return rc;
     if (n_steps < 1) { snprintf(g_err, sizeof g_err, "n_steps must be >= 1"); return GAME_ERR_ARG; }
+    if (!rho0 || !rt0 || !wind0) { snprintf(g_err, sizeof g_err, "null initial state"); return GAME_ERR_ARG; }
     if ((rc = load_controls(rho0, rt0, wind0))) return rc;
     if ((rc = game_gpu_run_steps(n_steps, dt, 1))) return rc;
     const DevState &s = M->s[GAME_STATE_NEW];
@@ -643,6 +644,10 @@ int game_gpu_window_vjp(int n_steps, double dt, int checkpoint_every, const doub
         snprintf(g_err, sizeof g_err, "n_steps and checkpoint_every must be >= 1");
         return GAME_ERR_ARG;
     }
+    if (!rho0 || !rt0 || !wind0 || !rho_bar || !rt_bar || !wind_bar) {
+        snprintf(g_err, sizeof g_err, "null initial state or gradient buffer");
+        return GAME_ERR_ARG;
+    }
     const size_t S = (size_t)M->d.s, V = (size_t)M->d.v;
     auto terminal = [&](const DevState &, const Cot &b) -> int {
         int r;
@@ -662,6 +667,10 @@ int game_gpu_window_gradient(int n_steps, double dt, int checkpoint_every, const
     if (rc) return rc;
     if (n_steps < 1 || checkpoint_every < 1 || !cost) {
         snprintf(g_err, sizeof g_err, "n_steps and checkpoint_every must be >= 1 and cost non-null");
+        return GAME_ERR_ARG;
+    }
+    if (!rho0 || !rt0 || !wind0 || !rho_target || !rho_bar || !rt_bar || !wind_bar) {
+        snprintf(g_err, sizeof g_err, "null initial state, target or gradient buffer");
         return GAME_ERR_ARG;
     }
     const DevGrid &G = M->G;

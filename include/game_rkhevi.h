@@ -302,6 +302,7 @@ long long game_gpu_bytes_allocated(void);
 /* library's stream and returns after it has finished.  The adjoints     */
 /* are exact transposes of the forward operators, terrain terms          */
 /* included, computed as gathers over reverse tables (no atomics).       */
+/* A null buffer argument is rejected with GAME_ERR_ARG.                 */
 /* The reference has no adjoints; the forward operators replace:         */
 /* ------------------------------------------------------------------ */
 /* grad, gradient_operators.c:56-74: S -> V (top and surface w are 0) */
@@ -353,7 +354,8 @@ int game_gpu_op_thomas_adjoint(int n, int ncol, const double *c, const double *d
 /* / p_0)^(R_d/c_v) (set_initial_state.c:302-314 written for rhotheta_v). */
 /* The first step takes the first-step branches (coordinator.c:299).     */
 /* The runs use the model's state slots and carried fields, which hold   */
-/* the window's last recomputed state afterwards.                        */
+/* the window's last recomputed state afterwards.  The initial state,    */
+/* the target and the gradient buffers must be given (GAME_ERR_ARG).     */
 /* ------------------------------------------------------------------ */
 /* n_steps steps from the control variables; outputs (device, may be null):
    rho, rhotheta_v, exner_pert and wind of the final state */

@@ -166,3 +166,24 @@ def test_thomas_forward_adjoint_and_coefficient_gradients(env):
         an = sum(dot(t.grad, v) for t, v in zip((tc, td, te, tr), dirs))
         print(k, fd, an)
         assert abs(fd - an) <= 1e-6 * max(abs(fd), abs(an))
+
+
+def test_null_buffers_are_argument_errors(env):
+    """a null buffer argument is answered with GAME_ERR_ARG before anything is
+    launched, and the model stays usable"""
+    import ctypes as C
+    ops, g, torch = env
+    lib = ops.lib
+    sz = ops.sizes()
+    x, y = ops.new(sz["s"]), ops.new(sz["v"])
+    p = lambda t: C.c_void_p(t.data_ptr())
+    torch.cuda.synchronize()
+    assert lib.game_gpu_op_grad(None, p(y)) == 2
+    assert lib.game_gpu_op_grad_adjoint(p(y), None) == 2
+    assert lib.game_gpu_op_vorticity_flux(p(y), None, p(y)) == 2
+    assert lib.game_gpu_op_thomas(4, 2, None, None, None, None, None, None) == 2
+    assert lib.game_gpu_window_forward(1, C.c_double(60.0), None, None, None, None, None, None, None) == 2
+    assert lib.game_gpu_window_gradient(1, C.c_double(60.0), 1, p(x), p(x), p(y), None, C.byref(C.c_double()),
+                                        p(x), p(x), p(y)) == 2
+    ops._run("grad", x, y)      # still usable
+    assert torch.isfinite(y).all()
