@@ -354,6 +354,7 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
 int game_gpu_set_config(const game_config_t *config)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!config) { snprintf(g_err, sizeof g_err, "null config"); return GAME_ERR_ARG; }
     int rc = check_config(config);
     if (rc) return rc;
     M->cfg = *config;
@@ -559,18 +560,21 @@ extern "C" {
 int game_gpu_get_forcings(game_forcings_view_t *h)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!h) return GAME_ERR_ARG;
     return M->precision == GAME_MIXED ? get_forcings_t(M->WF, h) : get_forcings_t(M->W, h);
 }
 
 int game_gpu_get_diagnostics(game_diag_view_t *h)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!h) return GAME_ERR_ARG;
     return M->precision == GAME_MIXED ? get_diagnostics_t(M->WF, h) : get_diagnostics_t(M->W, h);
 }
 
 int game_gpu_get_irrev(game_irrev_view_t *h)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!h) return GAME_ERR_ARG;
     return M->precision == GAME_MIXED ? get_irrev_t(M->WF, h) : get_irrev_t(M->W, h);
 }
 
@@ -692,6 +696,7 @@ int game_gpu_temperature_diagnostics(int k)
 int game_gpu_inner_product(const double *a, const double *b, double *out)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!a || !b || !out) { snprintf(g_err, sizeof g_err, "null buffer argument"); return GAME_ERR_ARG; }
     size_t V = (size_t)M->d.v, S = (size_t)M->d.s;
     double *da = M->s[GAME_STATE_OLD].wind, *db = M->s[GAME_STATE_NEW].wind;
     HIP_OK(hipMemcpyAsync(da, a, V*sizeof(double), hipMemcpyHostToDevice, M->stream));
@@ -836,6 +841,7 @@ extern "C" {
 int game_gpu_init_balance_terms(const double *wind, const double *density, double *v_squared, double *pvt)
 {
     if (!M) return GAME_ERR_NOT_INIT;
+    if (!wind || !density) { snprintf(g_err, sizeof g_err, "null wind or density"); return GAME_ERR_ARG; }
     if (M->precision == GAME_MIXED) return balance_terms_t(M->WF, wind, density, v_squared, pvt);
     return balance_terms_t(M->W, wind, density, v_squared, pvt);
 }

@@ -185,5 +185,9 @@ def test_null_buffers_are_argument_errors(env):
     assert lib.game_gpu_window_forward(1, C.c_double(60.0), None, None, None, None, None, None, None) == 2
     assert lib.game_gpu_window_gradient(1, C.c_double(60.0), 1, p(x), p(x), p(y), None, C.byref(C.c_double()),
                                         p(x), p(x), p(y)) == 2
+    assert lib.game_gpu_set_config(None) == 2
+    assert lib.game_gpu_get_forcings(None) == 2
+    assert lib.game_gpu_inner_product(None, None, None) == 2
+    assert lib.game_gpu_set_state(1, None) == 2
     ops._run("grad", x, y)      # still usable
     assert torch.isfinite(y).all()
