@@ -406,8 +406,10 @@ int advection_adjoint(const double *wind, const double *rho, const double *flux,
     }
     adj::grad_adjoint(G, A, advb, w.vsqb, side ? side : st, -0.5);
     adj::inner_product_adjoint(G, A, wind, w.vsqb, b_wind, side ? side : st, 2.0, 1);
-    /* TRSK flux: in the mass flux and in the PV */
-    adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, st);
+    /* TRSK flux: in the mass flux (with two streams on the side stream too: its
+       cotangent is read only by the mass flux's reverse after the join) and
+       in the PV */
+    adj::vorticity_flux_adjoint(G, A, pv, advb, w.fluxb, side ? side : st);
     const int vf_chunks = (G.L + VF_CH - 1)/VF_CH;
     if (M->PR.n_max <= 12)
         launch("k_vflux_pv_adj_rh", sadj::VFluxPvAdjRhN<12>{G, M->PR, flux, advb, w.pvb, vf_chunks},
