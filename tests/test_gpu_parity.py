@@ -3,7 +3,7 @@
 FP64: the kernels keep the reference's operation order, so dry runs agree
 to the last bit except where a transcendental function differs between the
 device math library and glibc; the bar is 1e-12 relative per field after N
-steps.  Moist runs add exp/pow in the phase
+steps (the FP64 path's tolerance against the reference).  Moist runs add exp/pow in the phase
 transitions, so they are held to 1e-10 after a few steps.
 """
 import numpy as np

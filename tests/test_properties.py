@@ -10,7 +10,7 @@ generated inputs rather than fixed cases:
 * the equation of state of the restart path (read_init_data,
   set_initial_state.c:244-336) inverts temperature_diagnostics for
   perturbed states;
-* the grid generator's self-checks (SURVEY: index.c:53-107,
+* the grid generator's self-checks (index.c:53-107,
   coriolis.c:393-427, rhombus_averaging.c:242-266) on grids of every
   resolution the tests use, Lloyd iterations included.
 """
