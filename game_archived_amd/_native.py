@@ -205,6 +205,7 @@ def host_lib():
     lib.game_grid_file_read.argtypes = [C.c_char_p, P(Dims), P(GridView), P(DualgridView), P(GridFileExtras)]
     lib.game_latlon_interpolation.argtypes = [C.c_int, DP, DP, IP, IP, IP, IP, DP]
     lib.game_sfc_properties.argtypes = [C.c_int, IP, DP, DP, DP, DP]
+    lib.game_grid_statistics_write.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, DP, DP, DP]
     lib.game_state_file_write.argtypes = [C.c_char_p, P(Dims), DP, DP, DP, DP, DP, DP, DP, DP, C.c_int]
     lib.game_state_file_read.argtypes = [C.c_char_p, P(Dims), DP, DP, DP, DP, IP]
     lib.game_nc_last_error.restype = C.c_char_p

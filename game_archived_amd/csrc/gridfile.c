@@ -210,3 +210,43 @@ int game_state_file_read(const char *path, const game_dims_t *d, double *rho, do
     if (tke_found) *tke_found = found;
     return rc ? GAME_NC_EARG : 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* the generator's statistics file                                      */
+/* ------------------------------------------------------------------ */
+/* extremes with the reference's scan (index_helpers.c:13-47: the first
+   strict improvement from element 0) */
+static void extremes(const double *x, int n, double *lo, double *hi)
+{
+    double a = x[0], b = x[0];
+    for (int i = 1; i < n; ++i) {
+        if (x[i] < a) a = x[i];
+        if (x[i] > b) b = x[i];
+    }
+    *lo = a;
+    *hi = b;
+}
+
+int game_grid_statistics_write(const char *path, const char *grid_name, int n_lloyd, int nsh, int nvh,
+                               const double *face_area_unity, const double *normal_distance,
+                               const double *normal_distance_dual)
+{
+    if (!path || !grid_name || nsh < 1 || nvh < 1 || !face_area_unity || !normal_distance || !normal_distance_dual)
+        return GAME_NC_EARG;
+    double amin, amax, dmin, dmax, ddmin, ddmax;
+    extremes(face_area_unity, nsh, &amin, &amax);
+    extremes(normal_distance + nsh, nvh, &dmin, &dmax);     /* the highest layer's edges */
+    extremes(normal_distance_dual, nvh, &ddmin, &ddmax);    /* the highest level's dual edges */
+    FILE *f = fopen(path, "w");
+    if (!f) return GAME_NC_EIO;
+    fprintf(f, "Statistical properties of grid %s:\n\n", grid_name);
+    fprintf(f, "Number of Lloyd iterations: %d\n", n_lloyd);
+    fprintf(f, "Ratio of minimum to maximum area: %lf\n", amin/amax);
+    fprintf(f, "Shortest horizontal normal distance (highest layer): %lf m\n", dmin);
+    fprintf(f, "Longest horizontal normal distance (highest layer): %lf m\n", dmax);
+    fprintf(f, "Ratio of shortest to longest horizontal normal distance: %lf\n", dmin/dmax);
+    fprintf(f, "Shortest horizontal normal distance dual (highest level): %lf m\n", ddmin);
+    fprintf(f, "Longest horizontal normal distance dual (highest level): %lf m\n", ddmax);
+    fprintf(f, "Ratio of shortest to longest dual horizontal normal distance: %lf\n", ddmin/ddmax);
+    return fclose(f) ? GAME_NC_EIO : 0;
+}

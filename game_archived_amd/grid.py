@@ -140,6 +140,19 @@ class Grid:
         if rc != 0:
             raise RuntimeError(f"grid file write failed ({rc}): {N.host_lib().game_nc_last_error().decode()}")
 
+    def write_statistics(self, path, oro_id=None):
+        """the generator's statistics file (grid_generator/src/miscellaneous.c:92-135),
+        named RES<r>_L<l>_ORO<o> inside as the generator names it"""
+        d = self.dims
+        if oro_id is None:
+            oro_id = {"flat": 0, "gaussian": 2}.get(self.orography, 0)
+        name = f"RES{d.res_id}_L{d.n_layers}_ORO{oro_id}".encode()
+        rc = N.host_lib().game_grid_statistics_write(os.fsencode(path), name, int(self.n_lloyd), d.nsh, d.nvh,
+                                                     N.ptr(self.x["pent_hex_face_unity_sphere"]),
+                                                     N.ptr(self.g["normal_distance"]), N.ptr(self.dg["normal_distance"]))
+        if rc != 0:
+            raise RuntimeError(f"statistics file write failed ({rc})")
+
     @classmethod
     def from_file(cls, path, res_id, n_layers=26, n_oro_layers=23, moisture_on=0):
         """a grid read as src/io/set_grid_properties.c:21-221 reads it; the

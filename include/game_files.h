@@ -101,6 +101,17 @@ int game_latlon_interpolation(int res_id, const double *latitude_scalar, const d
 int game_sfc_properties(int nsh, const int *is_land, double *sfc_albedo, double *sfc_rho_c, double *t_conductivity,
                         double *roughness_length);
 
+/* The generator's statistics file (grid_generator/src/miscellaneous.c:92-135,
+   written at grid_generator/src/coordinator.c:337-338 as
+   statistics/RES<r>_L<l>_ORO<o>.txt, grid_name RES<r>_L<l>_ORO<o>): the Lloyd
+   iterations and the extremes of the cells' unit-sphere areas (NSH), of the
+   highest layer's horizontal normal distances (normal_distance[NSH + e]) and
+   of the highest level's dual ones (normal_distance_dual[e]), in the
+   reference's words and %lf format. */
+int game_grid_statistics_write(const char *path, const char *grid_name, int n_lloyd, int nsh, int nvh,
+                               const double *face_area_unity, const double *normal_distance,
+                               const double *normal_distance_dual);
+
 /* ------------------------------------------------------------------ */
 /* the state file                                                        */
 /* ------------------------------------------------------------------ */

@@ -286,3 +286,48 @@ def test_state_file_round_trip_and_restart(tmp_path, moist):
     assert np.allclose(st.rho, s0.rho, rtol=0, atol=0) or moist     # vapour may be limited to saturation
     assert np.allclose(st.rhotheta_v, s0.rhotheta_v, rtol=1e-13, atol=0)
     assert np.allclose(st.exner_pert + g.g["exner_bg"], s0.exner_pert + g.g["exner_bg"], rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("res,lloyd", [(3, 0), (4, 30)])
+def test_statistics_file(tmp_path, res, lloyd):
+    """the generator's statistics file (grid_generator/src/miscellaneous.c:92-135):
+    the reference's lines, with the extremes found as the reference finds them
+    (its own compiled find_min_index/find_max_index, index_helpers.c:13-47,
+    where oracle/_ref is built; numpy's otherwise) and C's %lf formatting"""
+    import ctypes as C
+    import refharness
+    from conftest import get_grid
+    g = get_grid(res, "flat", n_lloyd=lloyd)
+    d = g.dims
+    p = tmp_path / "stats.txt"
+    g.write_statistics(str(p))
+    area = g.x["pent_hex_face_unity_sphere"]
+    nd = g.g["normal_distance"][d.nsh:d.nsh + d.nvh].copy()
+    ndd = g.dg["normal_distance"][:d.nvh].copy()
+    if refharness.available():
+        lib = refharness.lib()
+        for f in ("find_min_index", "find_max_index"):
+            getattr(lib, f).argtypes = [C.POINTER(C.c_double), C.c_int]
+            getattr(lib, f).restype = C.c_int
+
+        def ext(x):
+            xp = x.ctypes.data_as(C.POINTER(C.c_double))
+            return x[lib.find_min_index(xp, x.size)], x[lib.find_max_index(xp, x.size)]
+    else:
+        def ext(x):
+            return x.min(), x.max()
+    (amin, amax), (dmin, dmax), (ddmin, ddmax) = ext(np.ascontiguousarray(area)), ext(nd), ext(ndd)
+    assert amin == area.min() and dmax == nd.max() and ddmin == ndd.min()
+    expected = (f"Statistical properties of grid RES{res}_L{d.n_layers}_ORO0:\n\n"
+                f"Number of Lloyd iterations: {lloyd}\n"
+                f"Ratio of minimum to maximum area: {amin / amax:f}\n"
+                f"Shortest horizontal normal distance (highest layer): {dmin:f} m\n"
+                f"Longest horizontal normal distance (highest layer): {dmax:f} m\n"
+                f"Ratio of shortest to longest horizontal normal distance: {dmin / dmax:f}\n"
+                f"Shortest horizontal normal distance dual (highest level): {ddmin:f} m\n"
+                f"Longest horizontal normal distance dual (highest level): {ddmax:f} m\n"
+                f"Ratio of shortest to longest dual horizontal normal distance: {ddmin / ddmax:f}\n")
+    assert p.read_text() == expected
+    # the icosahedral grid's spread: pentagons are the smallest cells, the
+    # primal distances vary less than the dual ones
+    assert 0.5 < amin / amax < 1 and dmin / dmax > ddmin / ddmax
