@@ -62,14 +62,17 @@ def test_graph_replays_match_eager_steps(precision_name):
 
 
 @pytest.mark.parametrize("precision_name", ["fp64", "mixed"])
-def test_two_streams_match_one_stream(precision_name):
+@pytest.mark.parametrize("res,layers,n_oro", [(4, 26, 23), (3, 55, 40)])
+def test_two_streams_match_one_stream(precision_name, res, layers, n_oro):
     """the branches on the second stream (kinetic energy, pressure gradient,
     the first substep's tracer solve) change no bit against one stream, the
-    two-stream steps replayed from graphs and the one-stream steps eager"""
+    two-stream steps replayed from graphs and the one-stream steps eager, at
+    26 layers and with the 55-layer column kernels"""
     from game_archived_amd.rkhevi import FP64, MIXED
     prec = {"fp64": FP64, "mixed": MIXED}[precision_name]
-    print(f"case: two streams (graphs) vs one stream (eager), res 4 moist gaussian, {precision_name}", flush=True)
-    g = get_grid(4, "gaussian", moist=1)
+    print(f"case: two streams (graphs) vs one stream (eager), res {res} x {layers} moist gaussian, {precision_name}",
+          flush=True)
+    g = get_grid(res, "gaussian", moist=1, n_layers=layers, n_oro=n_oro)
     s0 = oracle_state(g, moist=True)
     one = trajectory(g, s0, prec, graphs=False, one_stream=True)
     two = trajectory(g, s0, prec, graphs=True)
