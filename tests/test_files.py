@@ -288,7 +288,7 @@ def test_state_file_round_trip_and_restart(tmp_path, moist):
     assert np.allclose(st.exner_pert + g.g["exner_bg"], s0.exner_pert + g.g["exner_bg"], rtol=1e-13, atol=0)
 
 
-@pytest.mark.parametrize("res,lloyd", [(3, 0), (4, 30)])
+@pytest.mark.parametrize("res,lloyd", [(3, 0), (4, 2000)])
 def test_statistics_file(tmp_path, res, lloyd):
     """the generator's statistics file (grid_generator/src/miscellaneous.c:92-135):
     the reference's lines, with the extremes found as the reference finds them
