@@ -1598,11 +1598,13 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
        row j is the flux above row j+1, formed once.  (iw dt)/vol comes from
        the per-box table the runtime keeps for the current dt: the same
        quotient, formed once instead of once per tracer and row.  The inputs
-       of the next row are loaded before the current row is formed (indices
-       clamped in range; a step past the run repeats its last row and writes
-       nothing), so the walk waits on memory once, not once per row.  Lanes
-       past the last column compute the last column's rows; phases 2 and 3
-       skip them. */
+       of every row of the run are loaded before the first row is formed
+       (indices clamped in range; a step past the run repeats its last row
+       and writes nothing), so the walk waits on memory once, not once per
+       row: about fifty loads in flight per wave, within the registers the
+       LDS-bound occupancy leaves (206 -> 200 us at 26 layers, res 7, and
+       524 -> 516 us at 55, against one row ahead).  Lanes past the last
+       column compute the last column's rows; phases 2 and 3 skip them. */
     {
         constexpr int NW = GD_THREADS/GD_NCOL;
         constexpr int PER_MAX = ((LT > 0 ? LT : GAME_MAX_LAYERS) + NW - 1)/NW;
@@ -1656,14 +1658,27 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
            loads nothing */
         if (j0 < j1) {
             double fi_m = 0.0, fr_m = 0.0;
-            if (j0 > 0) { const In xm = load(j0 - 1); iface(xm, j0 - 1, fi_m, fr_m); }
-            In cur = load(j0);
+            /* every load of the run is issued before the first row is formed:
+               the interface above the run (its flux enters the first row;
+               interface 0 stands in, unused, for the top run), then the rows
+               (a step past the run's end repeats its last row, unused) */
+            In xm;
+            {
+                const int jm = j0 > 0 ? j0 - 1 : 0;
+                const i64 wi = i + (i64)(jm + 1)*vpl, bc = i + (i64)jm*nsh;
+                xm.wo = so.wind[wi]; xm.wn = sn.wind[wi]; xm.a = area[wi];
+                xm.dlo = rho_o[bc + nsh]; xm.dup = rho_o[bc];
+            }
+            In xs[PER_MAX];
+            xs[0] = load(j0);
+#pragma unroll
+            for (int t = 1; t < PER_MAX; ++t) xs[t] = load_next(j0 + t < j1 ? j0 + t : j1 - 1, xs[t - 1]);
+            if (j0 > 0) iface(xm, j0 - 1, fi_m, fr_m);
 #pragma unroll
             for (int t = 0; t < PER_MAX; ++t) {
                 const bool valid = j0 + t < j1;
                 const int j = valid ? j0 + t : j1 - 1;
-                /* past the run's end the walk repeats its last row (unused) */
-                const In nxt = load_next(j0 + t + 1 < j1 ? j0 + t + 1 : j1 - 1, cur);
+                const In &cur = xs[t];
                 const int idx = j*GD_NCOL + col;
                 double fi, fr;
                 iface(cur, j, fi, fr);
@@ -1693,7 +1708,6 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
                 if (valid) { ce[idx] = o; fl[idx] = up; dd[idx] = d; rr[idx] = r; }
                 fi_m = fi;
                 fr_m = fr;
-                cur = nxt;
             }
         }
     }
