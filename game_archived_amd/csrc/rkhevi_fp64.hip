@@ -667,20 +667,37 @@ __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
     }
     int c0, c1;
     layer_chunk(no, bk.y, c0, c1);
-    /* the edge terms of layer l added to (aw, a1, a2) */
-    auto add_layer = [&](int l, double &aw, double &a1, double &a2, double &pw, double &p1, double &p2) {
+    /* one layer's inputs: the cell's density and rhotheta_v, and per edge
+       the wind, the neighbour's density and rhotheta_v, the weight (from the
+       cell-major copy of ipw: coalesced) and the slope */
+    struct In { double rh, rt, u[6], rn[6], rtn[6], w[6], sl[6]; };
+    auto load = [&](int l) {
+        In x;
         const i64 si = (i64)l*nsh + h;
-        const double rh = rg[si], th = s.rt[si]/rh;
-        double tw[6], t1[6], t2[6];
+        x.rh = rg[si]; x.rt = s.rt[si];
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             const i64 vi = nsh + (i64)l*G.vpl + ej[j];
             const i64 sn = (i64)l*nsh + nb[j];
-            const double u = s.wind[vi];
-            const double rn = rg[sn];
+            x.u[j] = s.wind[vi];
+            x.rn[j] = rg[sn];
+            x.rtn[j] = s.rt[sn];
+            x.w[j] = G.ipwT[((i64)l*8 + (j < ne ? j : 0))*nsh + h];
+            x.sl[j] = G.slope[vi];
+        }
+        return x;
+    };
+    /* the edge terms of a loaded layer added to (aw, a1, a2) and (pw, p1, p2) */
+    auto add_layer = [&](const In &x, double &aw, double &a1, double &a2, double &pw, double &p1, double &p2) {
+        const double rh = x.rh, th = x.rt/rh;
+        double tw[6], t1[6], t2[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const double u = x.u[j];
+            const double rn = x.rn[j];
             const double f1 = (0.5*(rh + rn))*u;
-            const double f2 = (0.5*(th + s.rt[sn]/rn))*f1;
-            const double c = -0.5*G.ipw[8*si + (j < ne ? j : 0)]*G.slope[vi];
+            const double f2 = (0.5*(th + x.rtn[j]/rn))*f1;
+            const double c = -0.5*x.w[j]*x.sl[j];
             tw[j] = c*u;
             t1[j] = c*f1;
             t2[j] = c*f2;
@@ -694,18 +711,26 @@ __global__ void __launch_bounds__(TPB) k_vcc(DevGrid G, DevState s, double *vcc)
         }
     };
     double pw = 0.0, p1 = 0.0, p2 = 0.0;
-    if (c0 > 0) {
-        double dw = 0.0, d1 = 0.0, d2 = 0.0;
-        add_layer(top + c0 - 1, pw, p1, p2, dw, d1, d2);
-    }
-    for (int k = c0; k < c1; ++k) {
-        double rw = pw, r1 = p1, r2 = p2;
-        pw = 0.0; p1 = 0.0; p2 = 0.0;
-        add_layer(top + k, rw, r1, r2, pw, p1, p2);
-        const i64 t = (i64)k*nsh + h;
-        vcc[t] = rw;
-        vcc[n + t] = r1;
-        vcc[2*n + t] = r2;
+    /* the layer above the chunk (its terms start the first level's sum),
+       then the chunk's layers, each loaded before the one before it is
+       summed (a step past the chunk reloads its last layer, unused) */
+    const int k_first = c0 > 0 ? c0 - 1 : c0;
+    In cur = load(top + k_first);
+    for (int k = k_first; k < c1; ++k) {
+        const In nxt = load(top + (k + 1 < c1 ? k + 1 : k));
+        if (k < c0) {
+            double dw = 0.0, d1 = 0.0, d2 = 0.0;
+            add_layer(cur, pw, p1, p2, dw, d1, d2);
+        } else {
+            double rw = pw, r1 = p1, r2 = p2;
+            pw = 0.0; p1 = 0.0; p2 = 0.0;
+            add_layer(cur, rw, r1, r2, pw, p1, p2);
+            const i64 t = (i64)k*nsh + h;
+            vcc[t] = rw;
+            vcc[n + t] = r1;
+            vcc[2*n + t] = r2;
+        }
+        cur = nxt;
     }
 }
 
@@ -2328,7 +2353,12 @@ static void scalar_tend_t(const DevGrid &G, const DevState &s, const DevState &t
 {
     if (!G.flat)
         /* chunks of three terrain levels (res 6 x 26, 23 terrain layers: 68 us
-           per launch; chunks of 2, 4 and 8 levels 77, 73 and 82 us) */
+           per launch; chunks of 2, 4 and 8 levels 77, 73 and 82 us).  Loading
+           each layer before the one above it is summed took the kernel's time
+           in the mountain window's gradient from 5.98 to 5.46 ms (176
+           registers, two waves per SIMD); chunks of six levels then 6.87 ms,
+           and the weights from ipw's interleaved
+           layout (eight per box) 5.68 ms */
         LAUNCH("k_vcc", k_vcc, GRID2(G.nsh, (G.n_oro + 2)/3), dim3(TPB), st, G, s, W.vcc);
     if (G.n_const == 6) {
         if (G.flat) LAUNCH("k_scalar_tend", (k_scalar_tend<true, TW, 6>), GRID2(G.nsh, G.split_cell), dim3(TPB), st, G, s,
