@@ -152,6 +152,54 @@ __device__ __forceinline__ double grad_edge_t(const DevGrid &G, const TF *f, int
     return g;
 }
 
+/* grad at the edges (gradient_operators.c:14-33 with the terrain term of
+   averaging.c:113-133) for the successive layers l0, l0 + 1, ... of one
+   edge, as manage_pressure_gradient.c:17-72 and the kinetic-energy gradient
+   use it.  The two cells' values of layer l + 1 and their vertical
+   gradients at the w level between layers l and l + 1, which layer l's
+   terrain correction reads, are carried to layer l + 1 instead of loaded
+   and formed again; each quotient is the one the layer-wise formula forms,
+   so the results are identical.  The next layer's values are loaded before
+   this layer's are combined.  Used where terrain terms exist: on a flat
+   grid the layer-wise form needs fewer registers (the flat pressure
+   gradient keeps five waves per SIMD). */
+struct EdgeCarry { double ff, ft, gwf, gwt; };
+
+template <bool FLAT, typename TF>
+__device__ __forceinline__ double grad_edge_walk(const DevGrid &G, const TF *f, int l, int l0, int e, int fr, int to,
+                                                 EdgeCarry &c)
+{
+    const int nsh = G.nsh, L = G.L, lt = L - G.n_oro;
+    const i64 vi = nsh + (i64)l*G.vpl + e;
+    if (l == l0) { c.ff = (double)f[fr + (i64)l*nsh]; c.ft = (double)f[to + (i64)l*nsh]; }
+    /* layer l + 1 (the last layer reloads itself, unused) */
+    const int ln = l + 1 < L ? l + 1 : l;
+    const double nff = (double)f[fr + (i64)ln*nsh], nft = (double)f[to + (i64)ln*nsh];
+    double g = (c.ft - c.ff)/G.nd[vi];
+    if (!FLAT && l >= lt) {
+        /* grad_w_raw at level l of both cells: carried unless layer l - 1
+           had no terrain term or lies in another chunk */
+        if (l == l0 || l == lt) { c.gwf = grad_w_raw(G, f, l, fr); c.gwt = grad_w_raw(G, f, l, to); }
+        const double *ipw = G.ipw;
+        double s = ipw[8*((i64)l*nsh + fr) + 6]*c.gwf;
+        s += ipw[8*((i64)l*nsh + to) + 6]*c.gwt;
+        if (l < L - 1) {
+            /* grad_w_raw at level l + 1 */
+            const double gf1 = (c.ff - nff)/G.nd[fr + (i64)(l + 1)*G.vpl];
+            const double gt1 = (c.ft - nft)/G.nd[to + (i64)(l + 1)*G.vpl];
+            s += ipw[8*((i64)l*nsh + fr) + 7]*gf1;
+            s += ipw[8*((i64)l*nsh + to) + 7]*gt1;
+            c.gwf = gf1;
+            c.gwt = gt1;
+        }
+        s = 0.5*s;
+        g += -G.slope[vi]*s;
+    }
+    c.ff = nff;
+    c.ft = nft;
+    return g;
+}
+
 template <bool FLAT, typename TW, int NC>
 __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT<TW> W, int first_step,
                                                  const double *gf, double chw, double ohw)
@@ -163,6 +211,7 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
     const i64 gi = (i64)G.n_cond*G.s;
     int l0, l1;
     layer_chunk(G.L, bk.y, l0, l1);
+    EdgeCarry carry;
     for (int l = l0; l < l1; ++l) {
         i64 vi = G.nsh + (i64)l*G.vpl + e;
         TW old_t = 0;
@@ -171,7 +220,9 @@ __global__ void __launch_bounds__(TPB) k_pgrad_e(DevGrid G, DevState s, DevWorkT
             W.pgrad_old[vi] = old_t;
         }
         i64 sf = (i64)l*G.nsh + fr, st = (i64)l*G.nsh + to;
-        double g = grad_edge_t<FLAT, double>(G, s.ep, l, e, fr, to);
+        double g;
+        if constexpr (FLAT) g = grad_edge_t<FLAT, double>(G, s.ep, l, e, fr, to);
+        else g = grad_edge_walk<FLAT, double>(G, s.ep, l, l0, e, fr, to, carry);
         double p1_t = GC_C_D_P*(G.thbg[st] + s.tp[st]), p1_f = GC_C_D_P*(G.thbg[sf] + s.tp[sf]);
         double nl = (0.5*(p1_t + p1_f))*g;
         double p2_t = GC_C_D_P*s.tp[st], p2_f = GC_C_D_P*s.tp[sf];
@@ -585,7 +636,10 @@ __global__ void __launch_bounds__(TPB) k_grad_e(DevGrid G, const TW *f, TW *out)
     const int fr = G.from[e], to = G.to[e];
     int l0, l1;
     layer_chunk(G.L, bk.y, l0, l1);
-    for (int l = l0; l < l1; ++l) out[G.nsh + (i64)l*G.vpl + e] = (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to);
+    EdgeCarry carry;
+    for (int l = l0; l < l1; ++l)
+        out[G.nsh + (i64)l*G.vpl + e] = FLAT ? (TW)grad_edge_t<FLAT, TW>(G, f, l, e, fr, to)
+                                             : (TW)grad_edge_walk<FLAT, TW>(G, f, l, l0, e, fr, to, carry);
 }
 
 /* grad, interior w points */
