@@ -58,6 +58,8 @@ struct DevGrid {
     int split_edge;
     /* 1: tracer solver with one (column, tracer) per lane in registers; 0: LDS-staged */
     int gd_streaming;
+    /* one tracer per block in the LDS-staged tracer solve instead of all five */
+    int gd_per_tracer;
     /* mixed form, a precision probe: bit 0 rounds the density tendencies,
        bit 1 the rhotheta_v tendency, to FP32 after the scalar tendencies
        (the precision of FP32 storage, not its bytes) */

@@ -1807,6 +1807,156 @@ __global__ void __launch_bounds__(GD_THREADS) k_gen_densities(DevGrid G, DevStat
     }
 }
 
+/* The moist tracer solve with the five tracers of a group of NCOLS columns in
+   one block (the default at 26 layers; k_gen_densities above is the
+   one-tracer-per-block form, the default otherwise).  A row's inputs that every tracer shares --
+   the old and new vertical wind and the area of the interface below it, the
+   volume and (iw dt)/volume -- are loaded once for the five tracers, and a
+   tracer's density above an interface is the one below the interface
+   above, carried down the run, so phase 1 issues 16 loads per row instead
+   of 35.  Tracer t is constituent t for the four condensates (snow, rain,
+   cloud ice, cloud water: t < 4) and constituent 5 for the vapour; the gas
+   (constituent 4) has no tracer solve.  Each coefficient is formed with
+   the arithmetic of k_gen_densities, phase 2 runs one lane per (tracer,
+   column) system and phase 3 writes the five tracers back, coalesced along
+   the columns. */
+template <int LT, int NCOLS, int NT>
+__global__ void __launch_bounds__(NT) k_gen_densities_all(DevGrid G, DevState so, DevState sn, DevState td,
+                                                          StepConsts K)
+{
+    extern __shared__ double lds[];
+    constexpr int NTR = 5, NSYS = NTR*NCOLS, NW = NT/NCOLS, L = LT, PER = (L + NW - 1)/NW;
+    static_assert(NSYS <= NT, "one lane per system in phase 2");
+    const int nsh = G.nsh, vpl = G.vpl;
+    const i64 S = G.s;
+    const int i0 = (int)xcd_block().x*NCOLS;
+    const double iw = 0.5, ew = 1.0 - iw, dt = K.dt;
+    double *ce = lds, *dd = lds + L*NSYS, *rr = lds + 2*L*NSYS;
+    unsigned char *fl = (unsigned char *)(lds + 3*L*NSYS);
+    /* tracer t's constituent and sinking speed (column_solvers.c:414-465) */
+    auto kof = [](int t) { return t < 4 ? t : 5; };
+    auto vs = [&](int t) { return t < 1 ? K.snow_v : (t < 2 ? K.rain_v : (t < 4 ? K.cloud_v : 0.0)); };
+
+    /* phase 1: rows j0..j1-1 of the five tracers of column col
+       (column_solvers.c:466-615).  Every load of the run is issued before
+       the first row is formed (a step past the run's end repeats its last
+       row and writes nothing); a run starting past the last layer loads
+       nothing.  Lanes past the last column compute the last column's rows;
+       phases 2 and 3 skip them. */
+    {
+        const int w = threadIdx.x/NCOLS, col = threadIdx.x - w*NCOLS;
+        const int i = i0 + col < nsh ? i0 + col : nsh - 1;
+        const int j0 = w*PER, j1 = j0 + PER < L ? j0 + PER : L;
+        if (j0 < j1) {
+            const double a_sfc = G.area[i + G.v - nsh];
+            /* the interface above the run (its fluxes enter the first row;
+               interface 0 stands in, unused, for the top run) */
+            const int jm = j0 > 0 ? j0 - 1 : 0;
+            const i64 wim = i + (i64)(jm + 1)*vpl;
+            const double wo_m = so.wind[wim], wn_m = sn.wind[wim], a_m = G.area[wim];
+            /* per tracer: the density of the run's first row, of the row
+               above it and of the lowest layer */
+            double r0[NTR], dup_m[NTR], r_low[NTR];
+#pragma unroll
+            for (int t = 0; t < NTR; ++t) {
+                const double *ro = so.rho + (i64)kof(t)*S;
+                r0[t] = ro[i + (i64)j0*nsh];
+                dup_m[t] = ro[i + (i64)jm*nsh];
+                r_low[t] = ro[i + S - nsh];
+            }
+            /* per row j: the interface below it (jc, clamped to the lowest
+               interface for the last row, whose fluxes are zero) */
+            double wo[PER], wn[PER], ar[PER], q[PER], qlo[PER], v[PER], dlo[PER][NTR], tb[PER][NTR];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int j = j0 + u < j1 ? j0 + u : j1 - 1;
+                const int jc = j < L - 1 ? j : L - 2;
+                const i64 wi = i + (i64)(jc + 1)*vpl, b = i + (i64)j*nsh;
+                wo[u] = so.wind[wi]; wn[u] = sn.wind[wi]; ar[u] = G.area[wi];
+                q[u] = G.idv[b]; qlo[u] = G.idv[j < L - 1 ? b + nsh : b]; v[u] = G.vol[b];
+#pragma unroll
+                for (int t = 0; t < NTR; ++t) {
+                    const i64 kS = (i64)kof(t)*S;
+                    dlo[u][t] = so.rho[kS + i + (i64)(jc + 1)*nsh];
+                    tb[u][t] = td.rho[kS + b];
+                }
+            }
+            double fi_m[NTR], fr_m[NTR];
+#pragma unroll
+            for (int t = 0; t < NTR; ++t) {
+                fi_m[t] = 0.0;
+                fr_m[t] = 0.0;
+                if (j0 > 0) {
+                    double o_ = wo_m, n_ = wn_m;
+                    if (t < 4) { o_ -= vs(t); n_ -= vs(t); }
+                    o_ = a_m*o_;
+                    n_ = a_m*n_;
+                    const double doi = (n_ >= 0.0) ? r0[t] : dup_m[t];
+                    fi_m[t] = o_;
+                    fr_m[t] = doi*n_;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const bool valid = j0 + u < j1;
+                const int j = valid ? j0 + u : j1 - 1;
+                const double qq = q[u], vv = v[u];
+#pragma unroll
+                for (int t = 0; t < NTR; ++t) {
+                    const double sv = vs(t);
+                    /* the row's own density, which is also the density above
+                       its interface below (fluxes of the last row are zero) */
+                    const double rb = u == 0 ? r0[t] : dlo[u - 1][t];
+                    double wo_ = wo[u], wn_ = wn[u];
+                    if (t < 4) { wo_ -= sv; wn_ -= sv; }
+                    wo_ = ar[u]*wo_;
+                    wn_ = ar[u]*wn_;
+                    const double doi = (wn_ >= 0.0) ? dlo[u][t] : rb;
+                    const double fi = j < L - 1 ? wo_ : 0.0;
+                    const double fr = j < L - 1 ? doi*wn_ : 0.0;
+                    double d;
+                    if (j == 0) d = (fi >= 0.0) ? 1.0 : 1.0 - qq*fi;
+                    else if (j == L - 1) {
+                        d = (fi_m[t] >= 0.0) ? 1.0 + qq*fi_m[t] : 1.0;
+                        if (t < 4) d += iw*sv*dt*a_sfc/vv;
+                    } else {
+                        d = 1.0;
+                        if (fi_m[t] >= 0.0) d += qq*fi_m[t];
+                        if (fi < 0.0) d -= qq*fi;
+                    }
+                    double r = rb + dt*tb[u][t];
+                    if (j == 0) r += ew*dt*fr/vv;
+                    else if (j == L - 1) {
+                        r += -ew*dt*fr_m[t]/vv;
+                        if (t < 4) r += -ew*sv*dt*r_low[t]*a_sfc/vv;
+                    } else r += ew*dt*(-fr_m[t] + fr)/vv;
+                    /* off-diagonals of interface j (column_solvers.c:475-488) */
+                    double o = 0.0;
+                    const bool up = fi >= 0.0;
+                    if (j < L - 1) o = up ? (-qq)*fi : qlo[u]*fi;
+                    const int idx = j*NSYS + t*NCOLS + col;
+                    if (valid) { ce[idx] = o; fl[idx] = up; dd[idx] = d; rr[idx] = r; }
+                    fi_m[t] = fi;
+                    fr_m[t] = fr;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    /* phase 2: Thomas algorithm (column_solvers.c:641-667), lane = system */
+    if (threadIdx.x < NSYS && i0 + (int)(threadIdx.x%NCOLS) < nsh)
+        thomas_lds_ce<NSYS, L>(ce, fl, dd, rr, L, threadIdx.x);
+    __syncthreads();
+    /* phase 3: clipped solution (column_solvers.c:616-637) */
+    for (int idx = threadIdx.x; idx < L*NSYS; idx += NT) {
+        const int j = idx/NSYS, sys = idx - j*NSYS, t = sys/NCOLS, col = sys - t*NCOLS, i = i0 + col;
+        if (i >= nsh) continue;
+        double x = rr[idx];
+        if (x < 0.0) x = 0.0;
+        sn.rho[(i64)kof(t)*S + i + (i64)j*nsh] = x;
+    }
+}
+
 /* The same tracer solve with one (column, tracer) per lane, the rows
    generated and eliminated in one downward sweep through registers (the
    layout k_ver_waves uses); selected with GAME_GPU_GD_STREAMING for
@@ -2554,6 +2704,19 @@ static void gen_densities_only(const DevGrid &G, const DevState &so, const DevSt
         if (G.L == 26) LAUNCH("k_gen_densities", k_gen_densities_stream<26>, g, b, st, G, so, sn, td, K);
         else if (G.L == 55) LAUNCH("k_gen_densities", k_gen_densities_stream<55>, g, b, st, G, so, sn, td, K);
         else LAUNCH("k_gen_densities", k_gen_densities_stream<0>, g, b, st, G, so, sn, td, K);
+        return;
+    }
+    if (G.n_const == 6 && G.n_cond == 4 && !G.gd_per_tracer && G.L == 26) {
+        /* five tracers per block at 26 layers: 16 columns and 256 threads
+           (51 KiB of LDS, three blocks per CU), res 7: 190 us against 200
+           for one tracer per block; 8 columns and 128 threads 224 us, 12
+           columns 196 us (256 threads) and 204 us (192).  At 55 layers 8
+           columns (54 KiB, two blocks per CU) took 701 us against 516:
+           fewer systems in flight for the longer serial elimination */
+        constexpr int NCOLS = 16, NT = 256;
+        dim3 g((G.nsh + NCOLS - 1)/NCOLS), b(NT);
+        size_t lds = (size_t)3*26*5*NCOLS*sizeof(double) + (((size_t)26*5*NCOLS + 7) & ~(size_t)7);
+        LAUNCH_SHM("k_gen_densities", (k_gen_densities_all<26, NCOLS, NT>), g, b, lds, st, G, so, sn, td, K);
         return;
     }
     const int ncol = gd_ncol(G.L == 26 ? 26 : (G.L == 55 ? 55 : 0));

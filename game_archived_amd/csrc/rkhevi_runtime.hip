@@ -218,6 +218,7 @@ static int init_model(const game_dims_t *dims, const game_grid_view_t *g, const 
         if (v >= 1 && v <= d.n_layers) G.split_cell = v;
     }
     G.gd_streaming = getenv("GAME_GPU_GD_STREAMING") ? 1 : 0;
+    G.gd_per_tracer = getenv("GAME_GPU_GD_PER_TRACER") ? 1 : 0;
     G.round_tendencies = 0;
     if (const char *e = getenv("GAME_GPU_ROUND_TENDENCIES")) {
         if (!strcmp(e, "rho")) G.round_tendencies = 1;

@@ -149,13 +149,13 @@ def test_general_terrain_path_on_flat_grid(monkeypatch):
 
 
 @pytest.mark.parametrize("switch", ["GAME_GPU_VW_STREAMING", "GAME_GPU_VW_ROWS", "GAME_GPU_GD_STREAMING",
-                                    "GAME_GPU_SH_PER_BOX", "GAME_GPU_UNFUSED_PHASE"])
+                                    "GAME_GPU_GD_PER_TRACER", "GAME_GPU_SH_PER_BOX", "GAME_GPU_UNFUSED_PHASE"])
 def test_kernel_variants(monkeypatch, switch):
     """The other variants kept for measurement -- the sound-wave solver with
     one column per lane or rows in parallel over 32 columns, the tracer
-    solver with one column per lane, the sediment heat per grid box, the
-    temperature and phase transitions as two kernels -- give the oracle's
-    result like the defaults."""
+    solver with one column per lane or with one tracer per block, the
+    sediment heat per grid box, the temperature and phase transitions as two
+    kernels -- give the oracle's result like the defaults."""
     monkeypatch.setenv(switch, "1")
     print(f"case: res 4 moist gaussian, {switch}, 2 steps", flush=True)
     g = get_grid(4, "gaussian", moist=1)
