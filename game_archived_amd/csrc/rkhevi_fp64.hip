@@ -2352,9 +2352,15 @@ __global__ void __launch_bounds__(TPB) k_temp_phase(DevGrid G, DevState s, TW *T
     if (i >= G.s) return;
     const i64 S = G.s;
     double r[6];
+#pragma unroll
     for (int k = 0; k < 6; ++k) r[k] = s.rho[k*S + i];
     const double thbg = G.thbg[i], tp = s.tp[i], exbg = G.exbg[i], ep = s.ep[i];
-    gf[i] = r[4]/density_total(G, s.rho, i);
+    /* density_total's left-to-right sum from 0 (derived_quantities.c:83-95)
+       over the six densities already loaded */
+    double rt = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rt += r[k];
+    gf[i] = r[4]/rt;
     double t = (thbg + tp)*(exbg + ep);
     t = t/(1.0 + r[5]/r[4]*(GC_M_D/GC_M_V - 1.0));
     T[i] = (TW)t;
