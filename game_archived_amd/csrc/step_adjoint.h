@@ -610,10 +610,16 @@ struct ScalarTendAdjC {
         const int l = (int)(u_/ns), c = (int)(u_ - (unsigned)l*ns);
         const int ne = c < GAME_N_PENT ? 5 : 6;
         double thvb = 0.0, rb = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const i64 k = (i64)l*G.nvh + G.adj[6*c + j];
-            thvb += P1[k];
-            rb += P2[k];
+        /* a fixed six slots, so every load issues at once; a pentagon's
+           sixth reads its first edge and adds nothing (same sums) */
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const i64 k = (i64)l*G.nvh + G.adj[6*c + (j < ne ? j : 0)];
+            const double p1 = P1[k], p2 = P2[k];
+            if (j < ne) {
+                thvb += p1;
+                rb += p2;
+            }
         }
         const double r = rho[t];
         b_rho[t] += rb - thvb*rt[t]/(r*r);
@@ -641,9 +647,25 @@ struct StvAdjS {
         split_index(t, G.nsh, l, c);
         const int ne = c < GAME_N_PENT ? 5 : 6;
         double r = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const i64 vi = eidx(G, l, G.adj[6*c + j]);
-            r += 0.5*v(vi)*Fb[vi];
+        /* a fixed six slots, so every load issues at once (v2's after v1's,
+           outside the sum); a pentagon's sixth reads its first edge and adds
+           nothing (same sum) */
+        i64 vi[6];
+        double a[6], f[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            vi[j] = eidx(G, l, G.adj[6*c + (j < ne ? j : 0)]);
+            a[j] = v1[vi[j]];
+            f[j] = Fb[vi[j]];
+        }
+        if (v2) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) a[j] = a[j] + v2[vi[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const double term = 0.5*a[j]*f[j];
+            if (j < ne) r += term;
         }
         if (with_w) {
             if (l >= 1) r += 0.5*v(widx(G, l, c))*Fb[widx(G, l, c)];
@@ -942,10 +964,16 @@ struct HalfDensAdj {
         const int ne = c < GAME_N_PENT ? 5 : 6, L = G.L;
         const i64 nv2 = 2*(i64)G.nvh;
         double r = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const int e = G.adj[6*c + j];
-            if (l >= 1) r += 0.25*dvb[(i64)l*nv2 + e];
-            if (l + 1 <= L - 1) r += 0.25*dvb[(i64)(l + 1)*nv2 + e];
+        /* a fixed six slots with the two interior loads unconditional (the
+           rows l and l + 1 <= L exist), so they issue at once; a pentagon's
+           sixth slot reads its first edge and adds nothing (same sum) */
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int e = G.adj[6*c + (j < ne ? j : 0)];
+            const double da = dvb[(i64)l*nv2 + e], db = dvb[(i64)(l + 1)*nv2 + e];
+            if (j >= ne) continue;
+            if (l >= 1) r += 0.25*da;
+            if (l + 1 <= L - 1) r += 0.25*db;
             /* the extrapolations to the top and the surface */
             if (l <= 1) {
                 const double kt = G.pv_top_dz2[e]/G.pv_top_dz1[e];
