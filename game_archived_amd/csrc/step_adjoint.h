@@ -966,14 +966,16 @@ struct HalfDensAdj {
         double r = 0.0;
         /* a fixed six slots with the two interior loads unconditional (the
            rows l and l + 1 <= L exist), so they issue at once; a pentagon's
-           sixth slot reads its first edge and adds nothing (same sum) */
+           sixth slot reads its first edge (same sum) */
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             const int e = G.adj[6*c + (j < ne ? j : 0)];
             const double da = dvb[(i64)l*nv2 + e], db = dvb[(i64)(l + 1)*nv2 + e];
+            /* a pentagon's sixth slot adds -0.0, the exact identity of IEEE
+               addition */
+            if (l >= 1) r += j < ne ? 0.25*da : -0.0;
+            if (l + 1 <= L - 1) r += j < ne ? 0.25*db : -0.0;
             if (j >= ne) continue;
-            if (l >= 1) r += 0.25*da;
-            if (l + 1 <= L - 1) r += 0.25*db;
             /* the extrapolations to the top and the surface */
             if (l <= 1) {
                 const double kt = G.pv_top_dz2[e]/G.pv_top_dz1[e];
@@ -1002,9 +1004,14 @@ struct RhombusDensAdj {
         int l, c;
         split_index(t, G.nsh, l, c);
         double r = 0.0;
+        /* every entry's load issues at once: padding entries (weight 0)
+           name edge 0, a valid index, and add -0.0, the exact identity of
+           IEEE addition (same sum) */
+#pragma unroll
         for (int k = 0; k < ADJ_RH_MAX; ++k) {
             const double w = A.rhp_w[(i64)k*G.nsh + c];
-            if (w != 0.0) r += w*dvb[(i64)l*2*G.nvh + G.nvh + A.rhp_e[(i64)k*G.nsh + c]];
+            const double d = dvb[(i64)l*2*G.nvh + G.nvh + A.rhp_e[(i64)k*G.nsh + c]];
+            r += w != 0.0 ? w*d : -0.0;
         }
         b_rho[t] += r;
     }
